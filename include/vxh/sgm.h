@@ -1,0 +1,180 @@
+/*
+ * vxh — MI355X-native semi-global matching (SGM) stereo, C ABI.
+ *
+ * Drop-in boundary for vx-vision's SGMStereo (vx-vision/src/kernels/sgm.rs).
+ * Every entry point names the reference interface it replaces.  All calls
+ * return a vxh_status; on failure vxh_last_error() returns a thread-local
+ * message (the reference's Error Display text, vx-vision/src/error.rs:25-37).
+ *
+ * Ownership: host buffers passed in are borrowed for the duration of the call;
+ * every device buffer is owned by the library (textures by their handle, the
+ * per-call scratch by the SGM handle's slot pool).  Handles are thread-safe:
+ * concurrent compute calls on one vxh_sgm each take their own scratch slot and
+ * HIP stream, as the reference's SGMStereo is Send + Sync
+ * (vx-vision/src/kernels/sgm.rs:68-69).
+ */
+#ifndef VXH_SGM_H
+#define VXH_SGM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes, one per variant of the reference's Error enum
+ * (vx-vision/src/error.rs:6-23). */
+typedef enum {
+    VXH_OK = 0,
+    VXH_ERR_DEVICE_NOT_FOUND = 1,     /* Error::DeviceNotFound          */
+    VXH_ERR_SHADER_MISSING = 2,       /* Error::ShaderMissing(name)     */
+    VXH_ERR_PIPELINE_COMPILE = 3,     /* Error::PipelineCompile(msg)    */
+    VXH_ERR_BUFFER_ALLOC = 4,         /* Error::BufferAlloc { bytes }   */
+    VXH_ERR_TEXTURE_SIZE_MISMATCH = 5,/* Error::TextureSizeMismatch     */
+    VXH_ERR_INVALID_CONFIG = 6,       /* Error::InvalidConfig(msg)      */
+    VXH_ERR_GPU = 7                   /* Error::Gpu(msg)                */
+} vxh_status;
+
+/* Mirrors SGMStereoConfig (vx-vision/src/kernels/sgm.rs:23-37) plus the path
+ * count and the optional left-right check, which the reference lacks.
+ *   num_paths: 2 = the reference's (+1,0) and (0,+1) (sgm.rs:197-200);
+ *              4 adds (-1,0) and (0,-1); 8 adds the four diagonals.
+ *   lr_check:  0 off; otherwise left pixels whose match fails
+ *              |d - D_R(x - d)| <= lr_tolerance become -1.0f. */
+typedef struct {
+    uint32_t num_disparities;
+    float p1;
+    float p2;
+    uint32_t census_radius_x;
+    uint32_t census_radius_y;
+    uint32_t num_paths;
+    uint32_t lr_check;
+    uint32_t lr_tolerance;
+} vxh_sgm_config;
+
+/* SGMStereoConfig::default (sgm.rs:39-49): D=128, P1=10, P2=120, census 4x3;
+ * num_paths 2 (the reference's behaviour), lr_check off, tolerance 1. */
+void vxh_sgm_config_default(vxh_sgm_config *cfg);
+/* SGMStereoConfig::new(num_disparities) (sgm.rs:51-59). */
+void vxh_sgm_config_new(uint32_t num_disparities, vxh_sgm_config *cfg);
+/* Validates a config for a w x h pair.  VXH_OK or VXH_ERR_INVALID_CONFIG:
+ * 1 <= D <= 256, D <= w, (2rx+1)(2ry+1)-1 <= 64 census bits, 0 <= p1, p2 <
+ * 65536 (truncated to u16 as the shader's ushort(p) conversion does,
+ * SGMStereo.metal:155-157), num_paths in {2,4,8}. */
+vxh_status vxh_sgm_config_validate(const vxh_sgm_config *cfg, uint32_t w,
+                                   uint32_t h);
+
+/* ---- Context (vx-vision/src/context.rs:17-36, vx-core/src/device.rs) ---- */
+typedef struct vxh_context vxh_context;
+/* Context::new (context.rs:26-36): device ordinal instead of the system
+ * default device; VXH_ERR_DEVICE_NOT_FOUND when no such HIP device exists,
+ * VXH_ERR_SHADER_MISSING when the gfx950 code object lacks a kernel. */
+vxh_status vxh_context_create(int device, vxh_context **out);
+void vxh_context_destroy(vxh_context *ctx);
+/* Number of HIP devices visible to the process (0 when none). */
+int vxh_device_count(void);
+
+/* ---- Textures (vx-vision/src/texture.rs) ---- */
+typedef struct vxh_texture vxh_texture;
+/* Context::texture_gray8 -> Texture::from_gray8 (context.rs:39-41,
+ * texture.rs:38-93): uploads w*h bytes; len != w*h -> INVALID_CONFIG. */
+vxh_status vxh_texture_gray8(vxh_context *ctx, const uint8_t *pixels,
+                             size_t len, uint32_t w, uint32_t h,
+                             vxh_texture **out);
+/* Context::texture_output_r32float (context.rs:64-66, texture.rs:324-349). */
+vxh_status vxh_texture_output_r32float(vxh_context *ctx, uint32_t w,
+                                       uint32_t h, vxh_texture **out);
+/* Texture::read_r32float (texture.rs:404-427): copies w*h floats to dst;
+ * len != w*h or a non-R32Float texture -> INVALID_CONFIG. */
+vxh_status vxh_texture_read_r32float(const vxh_texture *tex, float *dst,
+                                     size_t len);
+/* Replaces the bytes of an existing gray8 texture (same size). */
+vxh_status vxh_texture_write_gray8(vxh_texture *tex, const uint8_t *pixels,
+                                   size_t len);
+uint32_t vxh_texture_width(const vxh_texture *tex);
+uint32_t vxh_texture_height(const vxh_texture *tex);
+/* Device address of the texture's pixels (row-major, w*h elements). */
+void *vxh_texture_device_ptr(const vxh_texture *tex);
+void vxh_texture_destroy(vxh_texture *tex);
+
+/* ---- SGM kernel struct (vx-vision/src/kernels/sgm.rs:61-105) ---- */
+typedef struct vxh_sgm vxh_sgm;
+/* SGMStereo::new (sgm.rs:73-105): resolves the kernels once. */
+vxh_status vxh_sgm_create(vxh_context *ctx, vxh_sgm **out);
+void vxh_sgm_destroy(vxh_sgm *sgm);
+
+/* SGMStereo::compute_disparity (sgm.rs:108-265).  Synchronous, like the
+ * reference (commit + waitUntilCompleted, sgm.rs:260-261).  Sizes of left,
+ * right and output must agree (TEXTURE_SIZE_MISMATCH otherwise). */
+vxh_status vxh_sgm_compute_disparity(vxh_sgm *sgm, vxh_context *ctx,
+                                     const vxh_texture *left,
+                                     const vxh_texture *right,
+                                     vxh_texture *output,
+                                     const vxh_sgm_config *cfg);
+
+/* Camera intrinsics and depth map, as CameraIntrinsics / DepthMap
+ * (vx-vision/src/types_3d.rs:254-300, :384-430). */
+typedef struct {
+    float fx, fy, cx, cy;
+    uint32_t width, height;
+} vxh_camera_intrinsics;
+typedef struct {
+    vxh_texture *depth;          /* R32Float disparity map, owned by caller */
+    vxh_camera_intrinsics intrinsics;
+    float min_depth;             /* 0.0      */
+    float max_depth;             /* FLT_MAX  */
+} vxh_depth_map;
+/* SGMStereo::compute (sgm.rs:269-288): allocates the output texture, runs
+ * compute_disparity and wraps the disparity map (no depth conversion, as in
+ * the reference; baseline is unused there too). */
+vxh_status vxh_sgm_compute(vxh_sgm *sgm, vxh_context *ctx,
+                           const vxh_texture *left, const vxh_texture *right,
+                           const vxh_sgm_config *cfg,
+                           const vxh_camera_intrinsics *intrinsics,
+                           float baseline, vxh_depth_map *out);
+
+/* Host-buffer convenience: upload, compute_disparity, read back, the
+ * sequence of the reference's example caller
+ * (examples/depth_to_cloud_demo.rs:46-55). */
+vxh_status vxh_sgm_compute_host(vxh_sgm *sgm, const uint8_t *left,
+                                const uint8_t *right, uint32_t w, uint32_t h,
+                                const vxh_sgm_config *cfg, float *out);
+
+/* ---- Device-resident, stream-ordered form (the encode() pattern the
+ * reference's kernels follow, vx-vision/src/pipeline.rs:12-80) ---- */
+/* Scratch bytes one device-resident call needs for a w x h pair. */
+size_t vxh_sgm_workspace_bytes(uint32_t w, uint32_t h,
+                               const vxh_sgm_config *cfg);
+/* Enqueues the whole pipeline on `stream` (a hipStream_t; NULL = the null
+ * stream) with caller-owned device buffers: left/right w*h bytes, out w*h
+ * floats, workspace >= vxh_sgm_workspace_bytes.  Does not synchronize. */
+vxh_status vxh_sgm_enqueue(vxh_sgm *sgm, const uint8_t *d_left,
+                           const uint8_t *d_right, uint32_t w, uint32_t h,
+                           const vxh_sgm_config *cfg, float *d_out,
+                           void *d_workspace, size_t workspace_bytes,
+                           void *stream);
+
+/* Stage-level entry points on device buffers, for parity tests of the
+ * intermediate results.  d_census is 2*w*h uint32 (lo, hi per pixel);
+ * d_volume is w*h*D uint16, pixel-major with d innermost. */
+vxh_status vxh_sgm_census(const uint8_t *d_img, uint32_t w, uint32_t h,
+                          uint32_t rx, uint32_t ry, uint32_t *d_census,
+                          void *stream);
+vxh_status vxh_sgm_aggregate(const uint32_t *d_lc, const uint32_t *d_rc,
+                             uint32_t w, uint32_t h,
+                             const vxh_sgm_config *cfg, uint16_t *d_volume,
+                             void *stream);
+vxh_status vxh_sgm_wta(const uint16_t *d_volume, uint32_t w, uint32_t h,
+                       const vxh_sgm_config *cfg, float *d_out, void *stream);
+
+/* Thread-local message describing the last failure on this thread. */
+const char *vxh_last_error(void);
+/* Library version string. */
+const char *vxh_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,26 @@
+"""Builds the scalar CPU oracle (TEST INFRASTRUCTURE ONLY) into
+oracle/liboracle_sgm.so with g++ -O2.
+
+The reference (MisterEkole/vx-rs) is Metal + Rust and cannot be compiled in
+this environment (no Metal toolchain, no Rust toolchain), so there is no
+oracle/_ref build; the oracle is the C++ restatement in sgm_oracle.cpp.
+"""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "liboracle_sgm.so")
+
+
+def build():
+    tmp = OUT + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", "-pthread",
+           "-o", tmp, os.path.join(HERE, "sgm_oracle.cpp")]
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,143 @@
+"""ctypes binding of the scalar CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only the test suite, the smoke check and the benchmark's CPU-baseline leg use
+this module; it is the checker, never a code path of the product.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_sgm.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} missing: run `python oracle/build_oracle.py`")
+lib = ctypes.CDLL(LIB_PATH)
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+
+
+class HazardStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint64), ("updates", ctypes.c_uint64),
+                ("o1_wins", ctypes.c_uint64), ("wrap_updates", ctypes.c_uint64),
+                ("wrap_steps", ctypes.c_uint64), ("max_chain", ctypes.c_uint64),
+                ("chain_hist", ctypes.c_uint64 * 9)]
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+
+
+_sig("oracle_census", None, _vp, _u32, _u32, _u32, _u32, _vp)
+_sig("oracle_cost", _u32, _vp, _vp, _u32, _u32, _u32, _u32)
+_sig("oracle_aggregate_dir", None, _vp, _vp, _u32, _u32, _u32, _u32, _u32,
+     ctypes.c_int, ctypes.c_int, _vp)
+_sig("oracle_wta", None, _vp, _u32, _u32, _u32, _vp)
+_sig("oracle_wta_index", None, _vp, _u32, _u32, _u32, _vp)
+_sig("oracle_lr_check", None, _vp, _u32, _u32, _u32, _u32, _vp)
+_sig("oracle_direction", ctypes.c_int, _u32, ctypes.POINTER(ctypes.c_int),
+     ctypes.POINTER(ctypes.c_int))
+_sig("oracle_sgm", ctypes.c_int, _vp, _vp, _u32, _u32, _u32, ctypes.c_float,
+     ctypes.c_float, _u32, _u32, _u32, _u32, _u32, _u32, _vp)
+_sig("oracle_sgm_volume", ctypes.c_int, _vp, _vp, _u32, _u32, _u32, ctypes.c_float,
+     ctypes.c_float, _u32, _u32, _u32, _u32, _vp)
+_sig("oracle_scanline", None, _vp, _u32, _u32, _u32, _u32, _vp)
+_sig("oracle_hazard_dir", None, _vp, _vp, _u32, _u32, _u32, _u32, _u32,
+     ctypes.c_int, ctypes.c_int, ctypes.POINTER(HazardStats))
+
+
+def _u8(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+def census(img, rx=4, ry=3):
+    img = _u8(img)
+    h, w = img.shape
+    out = np.empty((h, w, 2), dtype=np.uint32)
+    lib.oracle_census(img.ctypes.data, w, h, rx, ry, out.ctypes.data)
+    return out
+
+
+def direction(i):
+    dx, dy = ctypes.c_int(), ctypes.c_int()
+    if lib.oracle_direction(i, ctypes.byref(dx), ctypes.byref(dy)):
+        raise IndexError(i)
+    return dx.value, dy.value
+
+
+def aggregate_dir(lc, rc, nd, p1, p2, dx, dy, S=None):
+    h, w = lc.shape[:2]
+    if S is None:
+        S = np.zeros((h, w, nd), dtype=np.uint16)
+    lib.oracle_aggregate_dir(np.ascontiguousarray(lc).ctypes.data,
+                             np.ascontiguousarray(rc).ctypes.data, w, h, nd, p1, p2,
+                             dx, dy, S.ctypes.data)
+    return S
+
+
+def wta(S):
+    h, w, nd = S.shape
+    out = np.empty((h, w), dtype=np.float32)
+    lib.oracle_wta(np.ascontiguousarray(S).ctypes.data, w, h, nd, out.ctypes.data)
+    return out
+
+
+def wta_index(S):
+    h, w, nd = S.shape
+    out = np.empty((h, w), dtype=np.int32)
+    lib.oracle_wta_index(np.ascontiguousarray(S).ctypes.data, w, h, nd, out.ctypes.data)
+    return out
+
+
+def lr_check(S, disp, tol=1):
+    h, w, nd = S.shape
+    disp = np.array(disp, dtype=np.float32, copy=True)
+    lib.oracle_lr_check(np.ascontiguousarray(S).ctypes.data, w, h, nd, tol,
+                        disp.ctypes.data)
+    return disp
+
+
+def sgm(left, right, nd=128, p1=10.0, p2=120.0, rx=4, ry=3, num_paths=2,
+        lr_check=False, lr_tol=1, nthreads=1):
+    left, right = _u8(left), _u8(right)
+    h, w = left.shape
+    out = np.empty((h, w), dtype=np.float32)
+    st = lib.oracle_sgm(left.ctypes.data, right.ctypes.data, w, h, nd, p1, p2, rx, ry,
+                        num_paths, int(bool(lr_check)), lr_tol, nthreads,
+                        out.ctypes.data)
+    if st:
+        raise ValueError(f"oracle rejected the config (status {st})")
+    return out
+
+
+def sgm_volume(left, right, nd=128, p1=10.0, p2=120.0, rx=4, ry=3, num_paths=2,
+               nthreads=1):
+    left, right = _u8(left), _u8(right)
+    h, w = left.shape
+    S = np.empty((h, w, nd), dtype=np.uint16)
+    st = lib.oracle_sgm_volume(left.ctypes.data, right.ctypes.data, w, h, nd, p1, p2,
+                               rx, ry, num_paths, nthreads, S.ctypes.data)
+    if st:
+        raise ValueError(f"oracle rejected the config (status {st})")
+    return S
+
+
+def scanline(costs, p1=10, p2=120):
+    costs = np.ascontiguousarray(costs, dtype=np.uint8)
+    steps, nd = costs.shape
+    out = np.empty((steps, nd), dtype=np.uint16)
+    lib.oracle_scanline(costs.ctypes.data, steps, nd, p1, p2, out.ctypes.data)
+    return out
+
+
+def hazard_dir(lc, rc, nd, p1, p2, dx, dy):
+    h, w = lc.shape[:2]
+    st = HazardStats()
+    lib.oracle_hazard_dir(np.ascontiguousarray(lc).ctypes.data,
+                          np.ascontiguousarray(rc).ctypes.data, w, h, nd, p1, p2,
+                          dx, dy, ctypes.byref(st))
+    return st

@@ -1,0 +1,98 @@
+/*
+ * Scalar CPU oracle for vx-vision's SGMStereo path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against.  Only the test suite, the smoke check and the CPU-baseline
+ * leg of the benchmark may load it; the product library never links it and
+ * never falls back to it.
+ *
+ * Every function restates the reference's Metal shader
+ * (vx-vision/shaders/SGMStereo.metal) and host orchestration
+ * (vx-vision/src/kernels/sgm.rs) in plain scalar C++; the citations are on the
+ * definitions in sgm_oracle.cpp.  Extensions that the reference lacks
+ * (reverse/diagonal paths beyond its two, the left-right check) follow the
+ * definitions written next to their functions.
+ */
+#ifndef VX_SGM_ORACLE_H
+#define VX_SGM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Census transform of one gray8 image (row-major, w*h bytes) into
+ * 2*w*h uint32 words: out[2*i] = lo bits 0..31, out[2*i+1] = hi bits 32..63. */
+void oracle_census(const uint8_t *img, uint32_t w, uint32_t h, uint32_t rx,
+                   uint32_t ry, uint32_t *out);
+
+/* Matching cost C(x, y, d) for one pixel (Hamming distance of census words). */
+uint32_t oracle_cost(const uint32_t *lc, const uint32_t *rc, uint32_t w,
+                     uint32_t x, uint32_t y, uint32_t d);
+
+/* Aggregates one scan direction (dx, dy in {-1,0,1}, not both 0) and adds its
+ * per-path costs into the summed volume S (u16, (y*w+x)*nd+d), mod 2^16. */
+void oracle_aggregate_dir(const uint32_t *lc, const uint32_t *rc, uint32_t w,
+                          uint32_t h, uint32_t nd, uint32_t p1, uint32_t p2,
+                          int dx, int dy, uint16_t *S);
+
+/* Winner-take-all with parabola subpixel refinement into a w*h float map. */
+void oracle_wta(const uint16_t *S, uint32_t w, uint32_t h, uint32_t nd,
+                float *out);
+
+/* Integer argmin disparity for every left pixel (the WTA's best_d). */
+void oracle_wta_index(const uint16_t *S, uint32_t w, uint32_t h, uint32_t nd,
+                      int32_t *out);
+
+/* Left-right consistency check applied to a WTA map in place. */
+void oracle_lr_check(const uint16_t *S, uint32_t w, uint32_t h, uint32_t nd,
+                     uint32_t tol, float *disp);
+
+/* Direction table: index -> (dx, dy).  Paths 2, 4 and 8 use the first
+ * num_paths entries. */
+int oracle_direction(uint32_t index, int *dx, int *dy);
+
+/* Full pipeline.  Returns 0 on success, a non-zero code for an invalid config
+ * (the same codes as the product's vxh_status).  nthreads <= 1 runs the
+ * single-core scalar path; more threads split independent scanlines and give
+ * identical bits. */
+int oracle_sgm(const uint8_t *left, const uint8_t *right, uint32_t w,
+               uint32_t h, uint32_t nd, float p1, float p2, uint32_t rx,
+               uint32_t ry, uint32_t num_paths, uint32_t lr_check,
+               uint32_t lr_tol, uint32_t nthreads, float *out);
+
+/* Same, but also returns the summed volume (w*h*nd u16) for volume-level
+ * comparisons. */
+int oracle_sgm_volume(const uint8_t *left, const uint8_t *right, uint32_t w,
+                      uint32_t h, uint32_t nd, float p1, float p2, uint32_t rx,
+                      uint32_t ry, uint32_t num_paths, uint32_t nthreads,
+                      uint16_t *S);
+
+/* Single-scanline recurrence on caller-supplied costs, for hand-derived
+ * known-answer vectors: costs is steps*nd bytes; out receives steps*nd u16
+ * per-path aggregated values. */
+void oracle_scanline(const uint8_t *costs, uint32_t steps, uint32_t nd,
+                     uint32_t p1, uint32_t p2, uint16_t *out);
+
+/* Hazard statistics over one direction: counts of pixel-steps and (px,d)
+ * updates in which the in-place o1 term won, or an arithmetic wrap occurred. */
+typedef struct {
+    uint64_t steps;          /* pixel-steps with step > 0                     */
+    uint64_t updates;        /* (pixel-step, d) updates with step > 0         */
+    uint64_t o1_wins;        /* updates where o1 was strictly the smallest    */
+    uint64_t wrap_updates;   /* updates with a wrap in o1/o2/o3 or in agg     */
+    uint64_t wrap_steps;     /* pixel-steps containing at least one wrap      */
+    uint64_t max_chain;      /* longest run of consecutive o1 wins over d     */
+    uint64_t chain_hist[9];  /* o1-win runs of length 1,2,3-4,...,>=129       */
+} oracle_hazard_stats;
+
+void oracle_hazard_dir(const uint32_t *lc, const uint32_t *rc, uint32_t w,
+                       uint32_t h, uint32_t nd, uint32_t p1, uint32_t p2,
+                       int dx, int dy, oracle_hazard_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

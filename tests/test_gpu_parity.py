@@ -1,0 +1,217 @@
+"""GPU parity: every stage of the HIP path against the scalar CPU oracle, bit
+for bit (census words, the u16 summed volume, and the fp32 disparity map
+compared as raw bits), through the C ABI."""
+import ctypes
+import hashlib
+import json
+import os
+import threading
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import vxsgm
+from sgm_cases import bench_pair, checkerboard_pair, random_pair
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "gpu test needs a visible device"
+    return torch
+
+
+def dev(a):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def gpu_census(img, rx=4, ry=3):
+    torch = _torch()
+    h, w = img.shape
+    d_img = dev(img)
+    out = torch.empty(h * w * 2, dtype=torch.int32, device="cuda")
+    vxsgm.check(vxsgm.lib.vxh_sgm_census(d_img.data_ptr(), w, h, rx, ry,
+                                         out.data_ptr(), None))
+    torch.cuda.synchronize()
+    return out.cpu().numpy().view(np.uint32).reshape(h, w, 2)
+
+
+def gpu_volume(lc, rc, cfg):
+    torch = _torch()
+    h, w = lc.shape[:2]
+    d_lc = dev(lc.view(np.int32))
+    d_rc = dev(rc.view(np.int32))
+    vol = torch.empty(h * w * cfg.num_disparities, dtype=torch.int16, device="cuda")
+    c = cfg.to_c()
+    vxsgm.check(vxsgm.lib.vxh_sgm_aggregate(d_lc.data_ptr(), d_rc.data_ptr(), w, h,
+                                            ctypes.byref(c), vol.data_ptr(), None))
+    torch.cuda.synchronize()
+    return vol.cpu().numpy().view(np.uint16).reshape(h, w, cfg.num_disparities)
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def assert_same_disp(got, want):
+    got = np.asarray(got, dtype=np.float32).reshape(want.shape)
+    diff = bits(got) != bits(want)
+    if diff.any():
+        idx = np.argwhere(diff)[:5]
+        raise AssertionError(f"{diff.sum()} of {diff.size} pixels differ, first at "
+                             f"{idx.tolist()}: got {got[diff][:5]} want {want[diff][:5]}")
+
+
+@pytest.mark.parametrize("w,h,rx,ry", [(64, 48, 4, 3), (33, 7, 4, 3), (5, 3, 4, 3),
+                                       (100, 40, 1, 1), (90, 20, 31, 0), (40, 90, 0, 31),
+                                       (77, 33, 3, 4), (1, 1, 4, 3), (128, 2, 2, 2)])
+def test_census_matches_oracle(w, h, rx, ry):
+    img = np.random.default_rng(w * 1000 + h).integers(0, 256, (h, w), dtype=np.uint8)
+    np.testing.assert_array_equal(gpu_census(img, rx, ry), O.census(img, rx, ry))
+
+
+VOLUME_CASES = [
+    # (w, h, D, paths)
+    (24, 16, 8, 2), (24, 16, 8, 4), (24, 16, 8, 8),
+    (40, 30, 16, 8), (64, 9, 1, 8), (31, 17, 2, 8), (70, 5, 7, 4),
+    (65, 12, 64, 8), (80, 11, 65, 8), (150, 9, 100, 8), (130, 13, 128, 8),
+    (140, 6, 129, 4), (200, 7, 192, 8), (256, 5, 255, 8), (300, 6, 256, 8),
+    (257, 1, 64, 8), (3, 40, 3, 8), (1, 1, 1, 8),
+]
+
+
+@pytest.mark.parametrize("w,h,D,paths", VOLUME_CASES)
+def test_volume_matches_oracle(w, h, D, paths):
+    left, right = random_pair(w, h, seed=w * 7 + h * 13 + D)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=paths)
+    lc, rc = O.census(left), O.census(right)
+    want = O.sgm_volume(left, right, D, num_paths=paths)
+    got = gpu_volume(lc, rc, cfg)
+    if not np.array_equal(got, want):
+        bad = np.argwhere(got != want)
+        raise AssertionError(f"{len(bad)} volume entries differ; first {bad[:5].tolist()}")
+
+
+@pytest.mark.parametrize("p1,p2", [(10, 120), (0, 0), (1, 2), (200, 1000),
+                                   (65535, 65535), (10.9, 120.99), (3000, 60000)])
+def test_volume_penalties(p1, p2):
+    left, right = random_pair(96, 20, seed=int(p1 + p2))
+    cfg = vxsgm.SGMStereoConfig(num_disparities=48, p1=p1, p2=p2, num_paths=8)
+    want = O.sgm_volume(left, right, 48, p1=p1, p2=p2, num_paths=8)
+    got = gpu_volume(O.census(left), O.census(right), cfg)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_end_to_end_qvga_d64_4path(sgm):
+    """Workload qvga_d64_4path: 320x240, D=64, 4 paths, bit-exact."""
+    left, right, _, cfg = vxsgm.workload_pair("qvga_d64_4path")
+    got = sgm.compute_host(left, right, 320, 240, cfg)
+    assert_same_disp(got, O.sgm(left, right, 64, num_paths=4, nthreads=8))
+
+
+@pytest.mark.parametrize("paths", [2, 4, 8])
+def test_reference_test_image(ctx, sgm, paths):
+    """The reference's own SGM test input (test_kernels.rs:780-834)."""
+    left, right = checkerboard_pair()
+    cfg = vxsgm.SGMStereoConfig.new(32)
+    cfg.num_paths = paths
+    lt = ctx.texture_gray8(left, 64, 64)
+    rt = ctx.texture_gray8(right, 64, 64)
+    out = ctx.texture_output_r32float(64, 64)
+    sgm.compute_disparity(ctx, lt, rt, out, cfg)
+    result = out.read_r32float()
+    assert result.size == 64 * 64
+    assert np.isfinite(result).any()
+    assert_same_disp(result, O.sgm(left, right, 32, num_paths=paths))
+
+
+@pytest.mark.parametrize("w,h", [(320, 240), (640, 480)])
+def test_reference_bench_images(sgm, w, h):
+    """The reference's criterion inputs (kernel_benchmarks.rs:189-228), D=64."""
+    left, right = bench_pair(w, h)
+    cfg = vxsgm.SGMStereoConfig.new(64)
+    got = sgm.compute_host(left, right, w, h, cfg)
+    assert_same_disp(got, O.sgm(left, right, 64, num_paths=2, nthreads=8))
+
+
+def test_constant_image_gives_zero(sgm):
+    img = np.full((37, 90), 117, dtype=np.uint8)
+    for paths in (2, 4, 8):
+        cfg = vxsgm.SGMStereoConfig(num_disparities=40, num_paths=paths)
+        got = sgm.compute_host(img, img, 90, 37, cfg)
+        assert (bits(got) == bits(np.zeros_like(got))).all()
+
+
+@pytest.mark.parametrize("tol", [0, 1, 3])
+def test_lr_check_matches_oracle(sgm, tol):
+    left, right, _ = vxsgm.synth_pair(160, 96, 48, 5, 1, 99 + tol)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=48, num_paths=8, lr_check=True,
+                                lr_tolerance=tol)
+    got = sgm.compute_host(left, right, 160, 96, cfg)
+    want = O.sgm(left, right, 48, num_paths=8, lr_check=True, lr_tol=tol)
+    assert_same_disp(got, want)
+    assert (want == -1.0).any()
+
+
+def test_size_mismatch_and_invalid_config(ctx, sgm):
+    a = ctx.texture_gray8(np.zeros((20, 30), np.uint8), 30, 20)
+    b = ctx.texture_gray8(np.zeros((21, 30), np.uint8), 30, 21)
+    out = ctx.texture_output_r32float(30, 20)
+    with pytest.raises(vxsgm.TextureSizeMismatch):
+        sgm.compute_disparity(ctx, a, b, out, vxsgm.SGMStereoConfig.new(16))
+    with pytest.raises(vxsgm.InvalidConfig):
+        sgm.compute_disparity(ctx, a, a, out, vxsgm.SGMStereoConfig.new(31))
+    with pytest.raises(vxsgm.InvalidConfig):
+        ctx.texture_gray8(np.zeros(10, np.uint8), 3, 3)
+
+
+def test_compute_returns_depth_map(ctx, sgm):
+    left, right = checkerboard_pair()
+    lt, rt = ctx.texture_gray8(left, 64, 64), ctx.texture_gray8(right, 64, 64)
+    k = vxsgm.CameraIntrinsics(500.0, 500.0, 32.0, 32.0, 64, 64)
+    dm = sgm.compute(ctx, lt, rt, vxsgm.SGMStereoConfig.new(32), k, 0.1)
+    assert dm.min_depth == 0.0 and dm.max_depth == np.finfo(np.float32).max
+    assert_same_disp(dm.depth.read_r32float(), O.sgm(left, right, 32))
+
+
+def test_concurrent_calls_match_serial(sgm):
+    """One handle shared by threads (the reference is Send + Sync)."""
+    pairs = [vxsgm.synth_pair(200, 120, 64, 4, 1, s)[:2] for s in range(6)]
+    cfg = vxsgm.SGMStereoConfig(num_disparities=64, num_paths=8)
+    serial = [sgm.compute_host(l, r, 200, 120, cfg) for l, r in pairs]
+    results = [None] * len(pairs)
+
+    def run(i):
+        l, r = pairs[i]
+        results[i] = sgm.compute_host(l, r, 200, 120, cfg)
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(len(pairs))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for a, b in zip(results, serial):
+        assert_same_disp(a, b)
+
+
+def test_720p_d128_8path_bit_exact(sgm):
+    """Workload 720p_d128_8path at full size against the threaded oracle."""
+    left, right, _, cfg = vxsgm.workload_pair("720p_d128_8path")
+    got = sgm.compute_host(left, right, 1280, 720, cfg)
+    want = O.sgm(left, right, 128, num_paths=8, nthreads=16)
+    assert_same_disp(got, want)
+
+
+def test_1080p_d256_8path_matches_golden_hash(sgm):
+    """Workload 1080p_d256_8path: the GPU map's SHA-256 equals the oracle's,
+    recorded in tests/golden/workload_hashes.json."""
+    with open(os.path.join(GOLDEN, "workload_hashes.json")) as f:
+        golden = json.load(f)
+    left, right, _, cfg = vxsgm.workload_pair("1080p_d256_8path")
+    got = sgm.compute_host(left, right, 1920, 1080, cfg)
+    assert hashlib.sha256(bits(got).tobytes()).hexdigest() == golden["1080p_d256_8path"]

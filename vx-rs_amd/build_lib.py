@@ -1,0 +1,55 @@
+"""Builds the gfx950 SGM library (vxsgm/libvxsgm.so) in-tree with hipcc.
+
+Run as a script (``python vx-rs_amd/build_lib.py``) or call ``build()``.
+Every HIP source is compiled for gfx950 only, with packed-fp32 instruction
+selection disabled and IEEE fp32 division (no fast-math), then linked into one
+shared library that exports the C ABI declared in include/vxh/*.h.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+OUT = os.path.join(HERE, "vxsgm", "libvxsgm.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_SOURCES = ["sgm_kernels.hip", "sgm_runtime.hip"]
+CXX_SOURCES = ["synth.cpp"]
+
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+          "-I", CSRC, "-Wall"]
+HIP_FLAGS = ["--offload-arch=gfx950", "-ffp-contract=off",
+             "-fhip-fp32-correctly-rounded-divide-sqrt",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-munsafe-fp-atomics"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=True):
+    os.makedirs(OBJ, exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        o = os.path.join(OBJ, src + ".o")
+        _run([HIPCC, "-x", "hip", *HIP_FLAGS, *COMMON, "-c",
+              os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
+    for src in CXX_SOURCES:
+        o = os.path.join(OBJ, src + ".o")
+        _run(["g++", *COMMON, "-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
+    tmp = OUT + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,45 @@
+// Internal declarations shared by the SGM kernels and the host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace vxh {
+
+// Path directions in path-index order.  num_paths 2 uses the first two (the
+// reference's scan_dirs, vx-vision/src/kernels/sgm.rs:197-200), 4 the first
+// four, 8 all of them.
+constexpr int kDirX[8] = {+1, 0, -1, 0, +1, -1, +1, -1};
+constexpr int kDirY[8] = {0, +1, 0, -1, +1, +1, -1, -1};
+
+// Number of scanlines of direction `dir` in a w x h image: every pixel whose
+// predecessor lies outside the image starts one.
+inline int num_scanlines(int dir, int w, int h) {
+    int dx = kDirX[dir], dy = kDirY[dir];
+    if (dy == 0) return h;
+    if (dx == 0) return w;
+    return w + h - 1;
+}
+
+struct AggParams {
+    const uint2 *lc;       // left census, w*h (lo, hi)
+    const uint2 *rc;       // right census
+    uint16_t *vol;         // summed volume, (y*w + x)*D + d
+    int w, h, D;
+    uint32_t p1, p2;       // penalties already truncated to u16
+    int dir;
+    int nscan;
+    int accumulate;        // 0: store (first path), 1: add mod 2^16
+};
+
+hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
+                         uint2 *out, hipStream_t s);
+hipError_t launch_aggregate(const AggParams &p, hipStream_t s);
+hipError_t launch_wta(const uint16_t *vol, int w, int h, int D, float *out,
+                      int16_t *best, hipStream_t s);
+hipError_t launch_lr_check(const uint16_t *vol, int w, int h, int D,
+                           int tol, const int16_t *best, float *out,
+                           hipStream_t s);
+
+}  // namespace vxh

@@ -1,0 +1,338 @@
+"""Python binding of the vxh SGM C ABI (include/vxh/sgm.h), shaped like the
+reference's Rust API so callers and tests read the same way:
+
+    Context()                       vx-vision/src/context.rs:26-36
+    Context.texture_gray8           vx-vision/src/context.rs:39-41
+    Context.texture_output_r32float vx-vision/src/context.rs:64-66
+    Texture.read_r32float           vx-vision/src/texture.rs:404-427
+    SGMStereoConfig                 vx-vision/src/kernels/sgm.rs:23-59
+    SGMStereo(ctx)                  vx-vision/src/kernels/sgm.rs:73-105
+    SGMStereo.compute_disparity     vx-vision/src/kernels/sgm.rs:108-265
+    SGMStereo.compute               vx-vision/src/kernels/sgm.rs:269-288
+    Error and its variants          vx-vision/src/error.rs:6-23
+
+The library is the in-tree gfx950 build (vxsgm/libvxsgm.so).  Importing this
+module without it raises ImportError: there is no CPU fallback.
+"""
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvxsgm.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `python vx-rs_amd/build_lib.py`")
+lib = ctypes.CDLL(LIB_PATH)
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_sz = ctypes.c_size_t
+
+
+class vxh_sgm_config(ctypes.Structure):
+    _fields_ = [("num_disparities", _u32), ("p1", ctypes.c_float),
+                ("p2", ctypes.c_float), ("census_radius_x", _u32),
+                ("census_radius_y", _u32), ("num_paths", _u32),
+                ("lr_check", _u32), ("lr_tolerance", _u32)]
+
+
+class vxh_camera_intrinsics(ctypes.Structure):
+    _fields_ = [("fx", ctypes.c_float), ("fy", ctypes.c_float),
+                ("cx", ctypes.c_float), ("cy", ctypes.c_float),
+                ("width", _u32), ("height", _u32)]
+
+
+class vxh_depth_map(ctypes.Structure):
+    _fields_ = [("depth", _vp), ("intrinsics", vxh_camera_intrinsics),
+                ("min_depth", ctypes.c_float), ("max_depth", ctypes.c_float)]
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_cfgp = ctypes.POINTER(vxh_sgm_config)
+_sig("vxh_last_error", ctypes.c_char_p)
+_sig("vxh_version", ctypes.c_char_p)
+_sig("vxh_device_count", ctypes.c_int)
+_sig("vxh_sgm_config_default", None, _cfgp)
+_sig("vxh_sgm_config_new", None, _u32, _cfgp)
+_sig("vxh_sgm_config_validate", ctypes.c_int, _cfgp, _u32, _u32)
+_sig("vxh_context_create", ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp))
+_sig("vxh_context_destroy", None, _vp)
+_sig("vxh_texture_gray8", ctypes.c_int, _vp, _vp, _sz, _u32, _u32, ctypes.POINTER(_vp))
+_sig("vxh_texture_output_r32float", ctypes.c_int, _vp, _u32, _u32, ctypes.POINTER(_vp))
+_sig("vxh_texture_read_r32float", ctypes.c_int, _vp, _vp, _sz)
+_sig("vxh_texture_write_gray8", ctypes.c_int, _vp, _vp, _sz)
+_sig("vxh_texture_width", _u32, _vp)
+_sig("vxh_texture_height", _u32, _vp)
+_sig("vxh_texture_device_ptr", _vp, _vp)
+_sig("vxh_texture_destroy", None, _vp)
+_sig("vxh_sgm_create", ctypes.c_int, _vp, ctypes.POINTER(_vp))
+_sig("vxh_sgm_destroy", None, _vp)
+_sig("vxh_sgm_compute_disparity", ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _cfgp)
+_sig("vxh_sgm_compute", ctypes.c_int, _vp, _vp, _vp, _vp, _cfgp,
+     ctypes.POINTER(vxh_camera_intrinsics), ctypes.c_float,
+     ctypes.POINTER(vxh_depth_map))
+_sig("vxh_sgm_compute_host", ctypes.c_int, _vp, _vp, _vp, _u32, _u32, _cfgp, _vp)
+_sig("vxh_sgm_workspace_bytes", _sz, _u32, _u32, _cfgp)
+_sig("vxh_sgm_enqueue", ctypes.c_int, _vp, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp,
+     _sz, _vp)
+_sig("vxh_sgm_census", ctypes.c_int, _vp, _u32, _u32, _u32, _u32, _vp, _vp)
+_sig("vxh_sgm_aggregate", ctypes.c_int, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp)
+_sig("vxh_sgm_wta", ctypes.c_int, _vp, _u32, _u32, _cfgp, _vp, _vp)
+_sig("vxh_synth_pair", ctypes.c_int, _u32, _u32, _u32, _u32, _u32,
+     ctypes.c_uint64, _vp, _vp, _vp)
+
+
+# ---- errors (vx-vision/src/error.rs:6-23) ----
+class Error(Exception):
+    """Base of every failure the library reports."""
+    code = -1
+
+
+class DeviceNotFound(Error):
+    code = 1
+
+
+class ShaderMissing(Error):
+    code = 2
+
+
+class PipelineCompile(Error):
+    code = 3
+
+
+class BufferAlloc(Error):
+    code = 4
+
+
+class TextureSizeMismatch(Error):
+    code = 5
+
+
+class InvalidConfig(Error):
+    code = 6
+
+
+class Gpu(Error):
+    code = 7
+
+
+_ERRORS = {c.code: c for c in (DeviceNotFound, ShaderMissing, PipelineCompile,
+                               BufferAlloc, TextureSizeMismatch, InvalidConfig, Gpu)}
+
+
+def check(status):
+    """Raises the Error subclass for a non-zero vxh_status."""
+    if status != 0:
+        msg = lib.vxh_last_error().decode(errors="replace")
+        raise _ERRORS.get(status, Error)(msg)
+
+
+def device_count():
+    return lib.vxh_device_count()
+
+
+# ---- config (vx-vision/src/kernels/sgm.rs:23-59) ----
+@dataclass
+class SGMStereoConfig:
+    num_disparities: int = 128
+    p1: float = 10.0
+    p2: float = 120.0
+    census_radius_x: int = 4
+    census_radius_y: int = 3
+    num_paths: int = 2
+    lr_check: bool = False
+    lr_tolerance: int = 1
+
+    @classmethod
+    def new(cls, num_disparities):
+        return cls(num_disparities=num_disparities)
+
+    def to_c(self):
+        return vxh_sgm_config(self.num_disparities, self.p1, self.p2,
+                              self.census_radius_x, self.census_radius_y,
+                              self.num_paths, int(bool(self.lr_check)),
+                              self.lr_tolerance)
+
+    def validate(self, w, h):
+        c = self.to_c()
+        check(lib.vxh_sgm_config_validate(ctypes.byref(c), w, h))
+
+
+def _as_u8(a):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    return a
+
+
+class Texture:
+    """Device texture owned by this object (R8 gray or R32 float)."""
+
+    def __init__(self, handle):
+        self._h = _vp(handle)
+
+    @property
+    def width(self):
+        return lib.vxh_texture_width(self._h)
+
+    @property
+    def height(self):
+        return lib.vxh_texture_height(self._h)
+
+    @property
+    def device_ptr(self):
+        return lib.vxh_texture_device_ptr(self._h)
+
+    def read_r32float(self):
+        out = np.empty(self.width * self.height, dtype=np.float32)
+        check(lib.vxh_texture_read_r32float(self._h, out.ctypes.data, out.size))
+        return out
+
+    def write_gray8(self, pixels):
+        p = _as_u8(pixels).ravel()
+        check(lib.vxh_texture_write_gray8(self._h, p.ctypes.data, p.size))
+
+    def close(self):
+        if self._h:
+            lib.vxh_texture_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One HIP device (Context::new, vx-vision/src/context.rs:26-36)."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        check(lib.vxh_context_create(device, ctypes.byref(h)))
+        self._h = h
+        self.device = device
+
+    def texture_gray8(self, pixels, width, height):
+        p = _as_u8(pixels).ravel()
+        h = _vp()
+        check(lib.vxh_texture_gray8(self._h, p.ctypes.data, p.size, width, height,
+                                    ctypes.byref(h)))
+        return Texture(h.value)
+
+    def texture_output_r32float(self, width, height):
+        h = _vp()
+        check(lib.vxh_texture_output_r32float(self._h, width, height, ctypes.byref(h)))
+        return Texture(h.value)
+
+    def close(self):
+        if self._h:
+            lib.vxh_context_destroy(self._h)
+            self._h = _vp()
+
+
+@dataclass
+class CameraIntrinsics:
+    fx: float
+    fy: float
+    cx: float
+    cy: float
+    width: int
+    height: int
+
+
+@dataclass
+class DepthMap:
+    depth: Texture
+    intrinsics: CameraIntrinsics
+    min_depth: float
+    max_depth: float
+
+
+class SGMStereo:
+    """SGM kernel struct (vx-vision/src/kernels/sgm.rs:61-288)."""
+
+    def __init__(self, ctx):
+        h = _vp()
+        check(lib.vxh_sgm_create(ctx._h, ctypes.byref(h)))
+        self._h = h
+        self._ctx = ctx
+
+    def compute_disparity(self, ctx, left, right, output, config):
+        c = config.to_c()
+        check(lib.vxh_sgm_compute_disparity(self._h, ctx._h, left._h, right._h,
+                                            output._h, ctypes.byref(c)))
+
+    def compute(self, ctx, left, right, config, intrinsics, baseline):
+        c = config.to_c()
+        k = vxh_camera_intrinsics(intrinsics.fx, intrinsics.fy, intrinsics.cx,
+                                  intrinsics.cy, intrinsics.width, intrinsics.height)
+        out = vxh_depth_map()
+        check(lib.vxh_sgm_compute(self._h, ctx._h, left._h, right._h, ctypes.byref(c),
+                                  ctypes.byref(k), baseline, ctypes.byref(out)))
+        return DepthMap(Texture(out.depth), intrinsics, out.min_depth, out.max_depth)
+
+    def compute_host(self, left, right, width, height, config):
+        """Host arrays in, host disparity map out (upload, compute, read back)."""
+        c = config.to_c()
+        l = _as_u8(left).ravel()
+        r = _as_u8(right).ravel()
+        out = np.empty(width * height, dtype=np.float32)
+        check(lib.vxh_sgm_compute_host(self._h, l.ctypes.data, r.ctypes.data, width,
+                                       height, ctypes.byref(c), out.ctypes.data))
+        return out
+
+    def enqueue(self, d_left, d_right, width, height, config, d_out, d_workspace,
+                workspace_bytes, stream=None):
+        c = config.to_c()
+        check(lib.vxh_sgm_enqueue(self._h, d_left, d_right, width, height,
+                                  ctypes.byref(c), d_out, d_workspace,
+                                  workspace_bytes, stream))
+
+    def close(self):
+        if self._h:
+            lib.vxh_sgm_destroy(self._h)
+            self._h = _vp()
+
+
+def workspace_bytes(width, height, config):
+    c = config.to_c()
+    return lib.vxh_sgm_workspace_bytes(width, height, ctypes.byref(c))
+
+
+def synth_pair(width, height, num_disparities, n_rects, n_planes, seed):
+    """Seeded integer-only synthetic pair (include/vxh/synth.h).
+
+    Returns (left, right, truth) as (h, w) uint8, uint8 and int32 arrays."""
+    left = np.empty((height, width), dtype=np.uint8)
+    right = np.empty((height, width), dtype=np.uint8)
+    truth = np.empty((height, width), dtype=np.int32)
+    check(lib.vxh_synth_pair(width, height, num_disparities, n_rects, n_planes,
+                             seed, left.ctypes.data, right.ctypes.data,
+                             truth.ctypes.data))
+    return left, right, truth
+
+
+# Workloads used by the benchmark and the parity tests: (w, h, D, paths,
+# rectangles, slanted planes, seed).
+WORKLOADS = {
+    "qvga_d64_4path": (320, 240, 64, 4, 6, 0, 0x53470000),
+    "720p_d128_8path": (1280, 720, 128, 8, 12, 2, 0x53470001),
+    "1080p_d256_8path": (1920, 1080, 256, 8, 12, 2, 0x53470002),
+}
+
+
+def workload_pair(name):
+    w, h, d, paths, nr, npl, seed = WORKLOADS[name]
+    left, right, truth = synth_pair(w, h, d, nr, npl, seed)
+    cfg = SGMStereoConfig(num_disparities=d, p1=10.0, p2=120.0, num_paths=paths)
+    return left, right, truth, cfg

@@ -157,7 +157,9 @@ vxh_status vxh_sgm_enqueue(vxh_sgm *sgm, const uint8_t *d_left,
 
 /* Stage-level entry points on device buffers, for parity tests of the
  * intermediate results.  d_census is 2*w*h uint32 (lo, hi per pixel);
- * d_volume is w*h*D uint16, pixel-major with d innermost. */
+ * d_volume is w*h*vxh_sgm_volume_stride(D) uint16, pixel-major with d
+ * innermost (entries d >= D of each pixel are padding). */
+uint32_t vxh_sgm_volume_stride(uint32_t num_disparities);
 vxh_status vxh_sgm_census(const uint8_t *d_img, uint32_t w, uint32_t h,
                           uint32_t rx, uint32_t ry, uint32_t *d_census,
                           void *stream);
@@ -165,6 +167,15 @@ vxh_status vxh_sgm_aggregate(const uint32_t *d_lc, const uint32_t *d_rc,
                              uint32_t w, uint32_t h,
                              const vxh_sgm_config *cfg, uint16_t *d_volume,
                              void *stream);
+/* vxh_sgm_aggregate with an explicit lane layout: `lanes` lanes of a wave
+ * own one scanline (16, 32 or 64) and each owns `per_lane` consecutive
+ * disparities (lanes * per_lane >= D); 0, 0 selects the default layout.
+ * Unsupported layouts return VXH_ERR_INVALID_CONFIG. */
+vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,
+                                    uint32_t w, uint32_t h,
+                                    const vxh_sgm_config *cfg, uint32_t lanes,
+                                    uint32_t per_lane, uint16_t *d_volume,
+                                    void *stream);
 vxh_status vxh_sgm_wta(const uint16_t *d_volume, uint32_t w, uint32_t h,
                        const vxh_sgm_config *cfg, float *d_out, void *stream);
 

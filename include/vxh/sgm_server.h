@@ -1,0 +1,75 @@
+/*
+ * vxh SGM serving loop: admits stereo requests as they arrive, sizes
+ * admission by each request's cost-volume bytes against an HBM budget, keeps
+ * device buffers resident in a slot pool, overlaps admitted requests on
+ * several HIP streams, and reports latency percentiles beside throughput.
+ *
+ * No reference counterpart: the reference computes one pair per synchronous
+ * SGMStereo::compute_disparity call (vx-vision/src/kernels/sgm.rs:108-265);
+ * its TexturePool (vx-vision/src/pool.rs:17-147) is the pattern the slot pool
+ * follows (reuse keyed by size, hit counters).  Every result is bit-identical
+ * to vxh_sgm_compute_disparity on the same inputs and config.
+ */
+#ifndef VXH_SGM_SERVER_H
+#define VXH_SGM_SERVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "vxh/sgm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    uint64_t hbm_budget_bytes;  /* sum of admitted requests' volume bytes   */
+    uint32_t max_inflight;      /* admitted requests at once (>= 1)         */
+    uint32_t num_streams;       /* HIP streams the admitted work spreads on */
+} vxh_server_config;
+
+typedef struct {
+    uint64_t submitted;
+    uint64_t completed;
+    uint64_t rejected;               /* larger than the whole budget        */
+    uint64_t peak_admitted_bytes;    /* max of the admitted volume bytes    */
+    uint64_t peak_inflight;
+    uint64_t slot_hits;              /* requests served from a pooled slot  */
+    uint64_t slot_misses;            /* requests that allocated a new slot  */
+    double p50_ms, p99_ms, max_ms;   /* latency: submit -> result on host   */
+    double mean_queue_ms;            /* submit -> admission                 */
+    double pxd_total;                /* sum of W*H*D over completed         */
+    double first_submit_s, last_complete_s;  /* steady-clock seconds       */
+} vxh_server_stats;
+
+typedef struct vxh_server vxh_server;
+
+/* Cost-volume bytes a request is admitted against (W*H*stride(D)*2). */
+uint64_t vxh_server_request_bytes(uint32_t w, uint32_t h, const vxh_sgm_config *cfg);
+
+vxh_status vxh_server_create(int device, const vxh_server_config *cfg,
+                             vxh_server **out);
+/* Queues a request.  left/right (w*h bytes) are copied before the call
+ * returns; out (w*h floats) must stay valid until the request completes.
+ * A request whose volume alone exceeds the budget is rejected with
+ * VXH_ERR_BUFFER_ALLOC; an invalid config with VXH_ERR_INVALID_CONFIG. */
+vxh_status vxh_server_submit(vxh_server *srv, const uint8_t *left,
+                             const uint8_t *right, uint32_t w, uint32_t h,
+                             const vxh_sgm_config *cfg, float *out,
+                             uint64_t *request_id);
+/* Blocks until request `id` has completed; returns its latency in ms. */
+vxh_status vxh_server_wait(vxh_server *srv, uint64_t id, double *latency_ms);
+/* Blocks until every submitted request has completed. */
+vxh_status vxh_server_drain(vxh_server *srv);
+vxh_status vxh_server_get_stats(vxh_server *srv, vxh_server_stats *out);
+/* Clears latency samples and counters (not the slot pool). */
+void vxh_server_reset_stats(vxh_server *srv);
+void vxh_server_destroy(vxh_server *srv);
+/* Message of the last server failure on this thread. */
+const char *vxh_server_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -113,3 +113,30 @@ def test_synth_pair_deterministic_and_sane():
         np.testing.assert_array_equal(x, y)
     assert not np.array_equal(a[0], c[0])
     assert a[2].min() >= 0 and a[2].max() <= 15
+
+
+def test_server_request_bytes_is_volume_size():
+    cfg = vxsgm.SGMStereoConfig(num_disparities=100)
+    assert vxsgm.request_bytes(640, 480, cfg) == 640 * 480 * 112 * 2
+    cfg = vxsgm.SGMStereoConfig(num_disparities=128)
+    assert vxsgm.request_bytes(1920, 1080, cfg) == 1920 * 1080 * 128 * 2
+
+
+@pytest.mark.parametrize("kw", [dict(hbm_budget_bytes=0), dict(max_inflight=0),
+                                dict(num_streams=0)])
+def test_server_rejects_zero_limits(kw):
+    with pytest.raises(vxsgm.InvalidConfig):
+        vxsgm.SGMServer(**kw)
+
+
+def test_server_without_device():
+    if vxsgm.device_count() > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(vxsgm.DeviceNotFound):
+        vxsgm.SGMServer()
+
+
+def test_batch_rejects_empty_and_bad_device():
+    h = ctypes.c_void_p()
+    st = vxsgm.lib.vxh_sgm_batch_create(0, None, 0, ctypes.byref(h))
+    assert st == vxsgm.InvalidConfig.code

@@ -41,17 +41,20 @@ def gpu_census(img, rx=4, ry=3):
     return out.cpu().numpy().view(np.uint32).reshape(h, w, 2)
 
 
-def gpu_volume(lc, rc, cfg):
+def gpu_volume(lc, rc, cfg, layout=(0, 0)):
     torch = _torch()
     h, w = lc.shape[:2]
+    D = cfg.num_disparities
+    Dp = vxsgm.lib.vxh_sgm_volume_stride(D)
     d_lc = dev(lc.view(np.int32))
     d_rc = dev(rc.view(np.int32))
-    vol = torch.empty(h * w * cfg.num_disparities, dtype=torch.int16, device="cuda")
+    vol = torch.empty(h * w * Dp, dtype=torch.int16, device="cuda")
     c = cfg.to_c()
-    vxsgm.check(vxsgm.lib.vxh_sgm_aggregate(d_lc.data_ptr(), d_rc.data_ptr(), w, h,
-                                            ctypes.byref(c), vol.data_ptr(), None))
+    vxsgm.check(vxsgm.lib.vxh_sgm_aggregate_layout(d_lc.data_ptr(), d_rc.data_ptr(), w, h,
+                                                   ctypes.byref(c), layout[0], layout[1],
+                                                   vol.data_ptr(), None))
     torch.cuda.synchronize()
-    return vol.cpu().numpy().view(np.uint16).reshape(h, w, cfg.num_disparities)
+    return vol.cpu().numpy().view(np.uint16).reshape(h, w, Dp)[:, :, :D]
 
 
 def bits(a):
@@ -95,6 +98,31 @@ def test_volume_matches_oracle(w, h, D, paths):
     if not np.array_equal(got, want):
         bad = np.argwhere(got != want)
         raise AssertionError(f"{len(bad)} volume entries differ; first {bad[:5].tolist()}")
+
+
+LAYOUTS = [(64, 1), (64, 2), (64, 4), (32, 1), (32, 2), (32, 4), (32, 8), (16, 4),
+           (16, 8), (16, 16)]
+
+
+@pytest.mark.parametrize("L,K", LAYOUTS)
+@pytest.mark.parametrize("D", [1, 5, 16, 32, 63, 64, 100, 128, 200, 256])
+def test_volume_every_layout(L, K, D):
+    """Every lane layout the autotuner may pick gives the oracle's volume."""
+    if L * K < D:
+        pytest.skip("layout covers fewer disparities than D")
+    w, h = max(D, 48) + 7, 11
+    left, right = random_pair(w, h, seed=L * 31 + K * 7 + D)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=8)
+    want = O.sgm_volume(left, right, D, num_paths=8)
+    got = gpu_volume(O.census(left), O.census(right), cfg, (L, K))
+    np.testing.assert_array_equal(got, want)
+
+
+def test_unsupported_layout_rejected():
+    left, right = random_pair(64, 4, seed=1)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=64, num_paths=2)
+    with pytest.raises(vxsgm.InvalidConfig):
+        gpu_volume(O.census(left), O.census(right), cfg, (16, 2))
 
 
 @pytest.mark.parametrize("p1,p2", [(10, 120), (0, 0), (1, 2), (200, 1000),

@@ -1,0 +1,393 @@
+// Batched SGM: many pairs of uneven size and disparity range in one
+// persistent-kernel launch, captured in a hipGraph.
+//
+// The reference runs one pair per command buffer (vx-vision/src/kernels/
+// sgm.rs:135-261) and its Pipeline batches encodes of different kernels into
+// one command buffer (vx-vision/src/pipeline.rs:12-80).  Here a batch is a
+// work list of wave-sized items, each a slice of one pair's stage:
+//   phase 0            census rows of the left or right image
+//   phase 1 .. P       scanline groups of path direction i-1 (P = num_paths)
+//   phase P + 1        winner-take-all pixels
+//   phase P + 2        left-right check pixels (only with lr_check)
+// Items are ordered phase-major across pairs.  A wave takes the next item
+// from a global counter, waits until every item of the same pair's previous
+// phase has signalled completion, runs the slice with the same device code as
+// the per-pair kernels, and signals.  Every item depends only on items earlier
+// in the list, and an item is taken only by a running wave, so the waits
+// always end.  The summed volume of a pair is updated by one direction at a
+// time, exactly as in the per-pair path, so the output is bit-identical.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "sgm_device.h"
+#include "vxh/sgm.h"
+#include "vxh/sgm_batch.h"
+
+namespace vxh {
+namespace {
+
+using namespace dev;
+
+constexpr int kMaxPhase = 11;   // census + 8 paths + wta + lr
+
+enum ItemKind : int { kCensusL = 0, kCensusR = 1, kAgg = 2, kWta = 3, kLr = 4 };
+
+struct PairDesc {
+    const uint8_t *left, *right;
+    float *out;
+    uint2 *lc, *rc;
+    uint16_t *vol;
+    int16_t *best;
+    int w, h, D, Dp;
+    uint32_t p1, p2;
+    int rx, ry;
+    int layout;   // index into kBatchLayouts
+};
+
+struct Item {
+    int pair;
+    int kind;
+    int dir;      // path index for kAgg
+    int first;    // first row / scanline / pixel
+    int count;    // rows / pixels (scanlines: one wave's group)
+    int phase;
+};
+
+// Layouts the persistent kernel can run, by index.
+constexpr int kBatchLayouts[][2] = {{64, 1}, {64, 2}, {64, 4}, {32, 2}, {32, 4},
+                                    {32, 8}, {16, 4}, {16, 8}, {16, 16}};
+constexpr int kNumBatchLayouts = sizeof(kBatchLayouts) / sizeof(kBatchLayouts[0]);
+
+__device__ __forceinline__ int dir_x(int i) {
+    return i == 0 || i == 4 || i == 6 ? 1 : (i == 2 || i == 5 || i == 7 ? -1 : 0);
+}
+__device__ __forceinline__ int dir_y(int i) {
+    return i == 1 || i == 4 || i == 5 ? 1 : (i == 3 || i == 6 || i == 7 ? -1 : 0);
+}
+__device__ __forceinline__ int dir_nscan(int i, int w, int h) {
+    return dir_y(i) == 0 ? h : (dir_x(i) == 0 ? w : w + h - 1);
+}
+
+__global__ void batch_reset_kernel(uint32_t *counters, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) counters[i] = 0;
+}
+
+__global__ void __launch_bounds__(256)
+batch_kernel(const PairDesc *__restrict__ pairs, const Item *__restrict__ items,
+             int nitems, uint32_t *next, uint32_t *done,
+             const uint32_t *__restrict__ need) {
+    const int lane = threadIdx.x & 63;
+    while (true) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = __hip_atomic_fetch_add(next, 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        idx = (uint32_t)__builtin_amdgcn_readfirstlane((int)idx);
+        if (idx >= (uint32_t)nitems) break;
+        const Item it = items[idx];
+        const PairDesc pd = pairs[it.pair];
+        if (it.phase > 0) {
+            const int c = it.pair * kMaxPhase + it.phase - 1;
+            const uint32_t target = need[c];
+            if (lane == 0) {
+                while (__hip_atomic_load(done + c, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT) < target)
+                    __builtin_amdgcn_s_sleep(2);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        if (it.kind == kCensusL || it.kind == kCensusR) {
+            const uint8_t *img = it.kind == kCensusL ? pd.left : pd.right;
+            uint2 *out = it.kind == kCensusL ? pd.lc : pd.rc;
+            for (int y = it.first; y < it.first + it.count; ++y)
+                for (int x = lane; x < pd.w; x += 64)
+                    out[(size_t)y * pd.w + x] = census_pixel(img, pd.w, pd.h, pd.rx, pd.ry, x, y);
+        } else if (it.kind == kAgg) {
+            AggParams p;
+            p.lc = pd.lc;
+            p.rc = pd.rc;
+            p.vol = pd.vol;
+            p.w = pd.w;
+            p.h = pd.h;
+            p.D = pd.D;
+            p.Dp = pd.Dp;
+            p.p1 = pd.p1;
+            p.p2 = pd.p2;
+            p.dir = it.dir;
+            p.nscan = dir_nscan(it.dir, pd.w, pd.h);
+            p.accumulate = it.dir > 0;
+            const int dx = dir_x(it.dir), dy = dir_y(it.dir);
+            switch (pd.layout) {
+            case 0: agg_wave<64, 1>(p, dx, dy, it.first); break;
+            case 1: agg_wave<64, 2>(p, dx, dy, it.first); break;
+            case 2: agg_wave<64, 4>(p, dx, dy, it.first); break;
+            case 3: agg_wave<32, 2>(p, dx, dy, it.first); break;
+            case 4: agg_wave<32, 4>(p, dx, dy, it.first); break;
+            case 5: agg_wave<32, 8>(p, dx, dy, it.first); break;
+            case 6: agg_wave<16, 4>(p, dx, dy, it.first); break;
+            case 7: agg_wave<16, 8>(p, dx, dy, it.first); break;
+            default: agg_wave<16, 16>(p, dx, dy, it.first); break;
+            }
+        } else if (it.kind == kWta) {
+            for (int i = it.first + lane; i < it.first + it.count; i += 64)
+                wta_pixel(pd.vol, pd.D, pd.Dp, (size_t)i, pd.out, pd.best);
+        } else {
+            for (int i = it.first + lane; i < it.first + it.count; i += 64)
+                lr_pixel(pd.vol, pd.w, pd.D, pd.Dp, it.dir, (size_t)i, pd.best, pd.out);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (lane == 0)
+            __hip_atomic_fetch_add(done + it.pair * kMaxPhase + it.phase, 1u,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+int layout_index(int L, int K) {
+    for (int i = 0; i < kNumBatchLayouts; ++i)
+        if (kBatchLayouts[i][0] == L && kBatchLayouts[i][1] == K) return i;
+    return -1;
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+}  // namespace
+}  // namespace vxh
+
+struct vxh_sgm_batch {
+    int device = 0;
+    int npairs = 0;
+    int nitems = 0;
+    int grid = 0;
+    void *slab = nullptr;       // per-pair workspaces
+    void *meta = nullptr;       // pairs, items, counters
+    vxh::PairDesc *d_pairs = nullptr;
+    vxh::Item *d_items = nullptr;
+    uint32_t *d_need = nullptr;
+    uint32_t *d_counters = nullptr;   // [0] = next, [1..] = done
+    int ncounters = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<vxh_sgm_batch_stats> stats;
+};
+
+namespace {
+thread_local std::string g_batch_error;
+vxh_status bfail(vxh_status st, const std::string &m) {
+    g_batch_error = m;
+    return st;
+}
+}  // namespace
+
+extern "C" {
+
+const char *vxh_sgm_batch_last_error(void) { return g_batch_error.c_str(); }
+
+vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs, uint32_t n,
+                                vxh_sgm_batch **out) {
+    using namespace vxh;
+    if (!out || (!pairs && n)) return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: null argument");
+    *out = nullptr;
+    if (n == 0 || n > 65535) return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: batch size");
+    if (hipSetDevice(device) != hipSuccess)
+        return bfail(VXH_ERR_DEVICE_NOT_FOUND, "no HIP device found");
+
+    std::vector<PairDesc> pd(n);
+    std::vector<size_t> off(n);
+    size_t slab = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const vxh_sgm_pair_desc &q = pairs[i];
+        if (vxh_sgm_config_validate(&q.cfg, q.width, q.height) != VXH_OK)
+            return bfail(VXH_ERR_INVALID_CONFIG,
+                         std::string("pair ") + std::to_string(i) + ": " + vxh_last_error());
+        if (!q.d_left || !q.d_right || !q.d_out)
+            return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: null pair buffer");
+        const int D = (int)q.cfg.num_disparities;
+        Layout lay = (q.lanes == 0 && q.per_lane == 0) ? default_layout(D)
+                                                       : Layout{(int)q.lanes, (int)q.per_lane};
+        const int li = layout_index(lay.L, lay.K);
+        if (li < 0 || lay.L * lay.K < D)
+            return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
+        const size_t npx = (size_t)q.width * q.height;
+        const int Dp = volume_stride(D);
+        off[i] = slab;
+        slab += align256(npx * 8) * 2 + align256(npx * (size_t)Dp * 2) + align256(npx * 2);
+        PairDesc &d = pd[i];
+        d.left = q.d_left;
+        d.right = q.d_right;
+        d.out = q.d_out;
+        d.w = (int)q.width;
+        d.h = (int)q.height;
+        d.D = D;
+        d.Dp = Dp;
+        d.p1 = (uint32_t)q.cfg.p1 & 0xFFFFu;
+        d.p2 = (uint32_t)q.cfg.p2 & 0xFFFFu;
+        d.rx = (int)q.cfg.census_radius_x;
+        d.ry = (int)q.cfg.census_radius_y;
+        d.layout = li;
+    }
+
+    vxh_sgm_batch *b = new vxh_sgm_batch();
+    b->device = device;
+    b->npairs = (int)n;
+    if (hipMalloc(&b->slab, slab) != hipSuccess) {
+        delete b;
+        return bfail(VXH_ERR_BUFFER_ALLOC, "buffer allocation failed (" +
+                                               std::to_string(slab) + " bytes)");
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        char *base = (char *)b->slab + off[i];
+        const size_t npx = (size_t)pd[i].w * pd[i].h;
+        pd[i].lc = (uint2 *)base;
+        pd[i].rc = (uint2 *)(base + align256(npx * 8));
+        pd[i].vol = (uint16_t *)(base + 2 * align256(npx * 8));
+        pd[i].best = (int16_t *)(base + 2 * align256(npx * 8) +
+                                 align256(npx * (size_t)pd[i].Dp * 2));
+    }
+
+    // Work list, phase-major; within a phase, larger pairs first.
+    std::vector<int> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+        return (size_t)pd[a].w * pd[a].h * pd[a].D > (size_t)pd[c].w * pd[c].h * pd[c].D;
+    });
+    std::vector<Item> items;
+    std::vector<uint32_t> need((size_t)n * kMaxPhase, 0);
+    b->stats.resize(n);
+    auto push = [&](int p, int kind, int dir, int first, int count, int phase) {
+        items.push_back(Item{p, kind, dir, first, count, phase});
+        need[(size_t)p * kMaxPhase + phase]++;
+    };
+    int maxphase = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        maxphase = std::max(maxphase, (int)pairs[i].cfg.num_paths + 2 +
+                                          (pairs[i].cfg.lr_check ? 1 : 0));
+    for (int phase = 0; phase < maxphase; ++phase) {
+        for (int p : order) {
+            const vxh_sgm_pair_desc &q = pairs[p];
+            const int P = (int)q.cfg.num_paths;
+            const PairDesc &d = pd[p];
+            if (phase == 0) {
+                const int rows = std::max(1, 4096 / d.w);
+                for (int y = 0; y < d.h; y += rows) {
+                    push(p, kCensusL, 0, y, std::min(rows, d.h - y), 0);
+                    push(p, kCensusR, 0, y, std::min(rows, d.h - y), 0);
+                }
+            } else if (phase <= P) {
+                const int dir = phase - 1;
+                const int G = 64 / kBatchLayouts[d.layout][0];
+                const int ns = num_scanlines(dir, d.w, d.h);
+                for (int s = 0; s < ns; s += G) push(p, kAgg, dir, s, G, phase);
+            } else if (phase == P + 1) {
+                const int npx = d.w * d.h, chunk = 64 * 64;
+                for (int i = 0; i < npx; i += chunk)
+                    push(p, kWta, 0, i, std::min(chunk, npx - i), phase);
+            } else if (phase == P + 2 && q.cfg.lr_check) {
+                const int npx = d.w * d.h, chunk = 64 * 64;
+                for (int i = 0; i < npx; i += chunk)
+                    push(p, kLr, (int)std::min<uint32_t>(q.cfg.lr_tolerance, 1024u), i,
+                         std::min(chunk, npx - i), phase);
+            }
+        }
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        b->stats[i].items = 0;
+        for (int ph = 0; ph < kMaxPhase; ++ph) b->stats[i].items += need[(size_t)i * kMaxPhase + ph];
+        b->stats[i].lanes = (uint32_t)kBatchLayouts[pd[i].layout][0];
+        b->stats[i].per_lane = (uint32_t)kBatchLayouts[pd[i].layout][1];
+    }
+    b->nitems = (int)items.size();
+    b->ncounters = 1 + (int)n * kMaxPhase;
+    const size_t meta = align256(n * sizeof(PairDesc)) + align256(items.size() * sizeof(Item)) +
+                        align256(need.size() * 4) + align256((size_t)b->ncounters * 4);
+    if (hipMalloc(&b->meta, meta) != hipSuccess) {
+        vxh_sgm_batch_destroy(b);
+        return bfail(VXH_ERR_BUFFER_ALLOC, "buffer allocation failed (" +
+                                               std::to_string(meta) + " bytes)");
+    }
+    char *m = (char *)b->meta;
+    b->d_pairs = (PairDesc *)m;
+    m += align256(n * sizeof(PairDesc));
+    b->d_items = (Item *)m;
+    m += align256(items.size() * sizeof(Item));
+    b->d_need = (uint32_t *)m;
+    m += align256(need.size() * 4);
+    b->d_counters = (uint32_t *)m;
+    hipError_t e = hipMemcpy(b->d_pairs, pd.data(), n * sizeof(PairDesc), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_items, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_need, need.data(), need.size() * 4, hipMemcpyHostToDevice);
+    int cus = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        vxh_sgm_batch_destroy(b);
+        return bfail(VXH_ERR_GPU, std::string("GPU error: batch setup: ") + hipGetErrorString(e));
+    }
+    // Persistent grid: 8 waves per CU (two 256-thread workgroups).
+    b->grid = std::max(1, cus) * 2;
+
+    // Capture [reset counters, persistent kernel] once; replays reuse it.
+    e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(batch_reset_kernel, dim3((b->ncounters + 255) / 256), dim3(256), 0,
+                           b->stream, b->d_counters, b->ncounters);
+        hipLaunchKernelGGL(batch_kernel, dim3(b->grid), dim3(256), 0, b->stream, b->d_pairs,
+                           b->d_items, b->nitems, b->d_counters, b->d_counters + 1, b->d_need);
+        e = hipStreamEndCapture(b->stream, &b->graph);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        vxh_sgm_batch_destroy(b);
+        return bfail(VXH_ERR_GPU, std::string("GPU error: batch graph: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return VXH_OK;
+}
+
+vxh_status vxh_sgm_batch_launch(vxh_sgm_batch *b, void *stream) {
+    if (!b) return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: null batch");
+    if (hipSetDevice(b->device) != hipSuccess) return bfail(VXH_ERR_DEVICE_NOT_FOUND, "no HIP device found");
+    hipError_t e = hipGraphLaunch(b->exec, stream ? (hipStream_t)stream : b->stream);
+    if (e != hipSuccess)
+        return bfail(VXH_ERR_GPU, std::string("GPU error: graph launch: ") + hipGetErrorString(e));
+    return VXH_OK;
+}
+
+vxh_status vxh_sgm_batch_run(vxh_sgm_batch *b) {
+    vxh_status st = vxh_sgm_batch_launch(b, nullptr);
+    if (st != VXH_OK) return st;
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess)
+        return bfail(VXH_ERR_GPU, std::string("GPU error: batch sync: ") + hipGetErrorString(e));
+    return VXH_OK;
+}
+
+vxh_status vxh_sgm_batch_stats_get(const vxh_sgm_batch *b, uint32_t pair,
+                                   vxh_sgm_batch_stats *out) {
+    if (!b || !out || pair >= (uint32_t)b->npairs)
+        return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: batch stats");
+    *out = b->stats[pair];
+    return VXH_OK;
+}
+
+uint32_t vxh_sgm_batch_items(const vxh_sgm_batch *b) { return b ? (uint32_t)b->nitems : 0; }
+
+void vxh_sgm_batch_destroy(vxh_sgm_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->meta) (void)hipFree(b->meta);
+    if (b->slab) (void)hipFree(b->slab);
+    delete b;
+}
+
+}  // extern "C"

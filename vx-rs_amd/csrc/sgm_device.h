@@ -1,0 +1,355 @@
+// Device functions shared by the per-pair kernels (sgm_kernels.hip) and the
+// batched persistent kernel (sgm_batch.hip).
+//
+// SGM stereo device code for gfx950 (wave64).
+//
+// Stages and the reference code whose results they reproduce bit for bit:
+//   census_kernel     vx-vision/shaders/SGMStereo.metal:25-66
+//   aggregate_kernel  vx-vision/shaders/SGMStereo.metal:72-173 (one launch per
+//                     direction; diagonals start wherever the predecessor
+//                     pixel lies outside the image)
+//   wta_kernel        vx-vision/shaders/SGMStereo.metal:179-214
+//   lr_kernel         left-right check (no reference counterpart)
+//
+// Aggregation layout: a group of L lanes of one wave owns one scanline; lane
+// `sub` of the group owns disparities d = sub*K .. sub*K+K-1, so the pixel's
+// disparity slice lives in registers, the summed volume segment of a lane is
+// one packed 2K-byte vector access, and every cross-lane step (neighbour
+// values, the min over d) is a DPP / permlane move inside the wave.
+//
+// The reference recurrence updates prev[] in place over ascending d, so the
+// d-1 term sees the value already produced in the same step
+// (SGMStereo.metal:155,167), and every sum wraps mod 2^16.  The d, d+1 and P2
+// terms do not depend on d-1 and are evaluated for all d in parallel (A[d]);
+// the d-1 chain runs sequentially inside each lane and is carried across lanes
+// by fixed-point rounds: every lane first assumes no carry-in, then
+// re-evaluates whenever the carry-in from the lane below changes
+// min(A[first], o1).  Lane 0's carry-in is fixed, so after r rounds lanes
+// 0..r are final; the loop ends when no lane changes, which is exactly the
+// sequential result including every wrap.
+#pragma once
+
+#include "sgm_internal.h"
+
+namespace vxh {
+namespace dev {
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// DPP controls (gfx9 encoding).
+constexpr int kQuadXor1 = 0xB1;      // quad_perm [1,0,3,2]
+constexpr int kQuadXor2 = 0x4E;      // quad_perm [2,3,0,1]
+constexpr int kRowShl1 = 0x101;      // lane i <- lane i+1 within a row of 16
+constexpr int kRowShr1 = 0x111;      // lane i <- lane i-1 within a row of 16
+constexpr int kWaveShl1 = 0x130;     // lane i <- lane i+1 across the wave
+constexpr int kWaveShr1 = 0x138;     // lane i <- lane i-1 across the wave
+constexpr int kRowMirror = 0x140;    // lane i <- lane 15-i within a row
+constexpr int kRowHalfMirror = 0x141;// lane i <- lane 7-i within 8 lanes
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+// Min over the L lanes of a group, broadcast to all of them.
+template <int L>
+__device__ __forceinline__ uint32_t group_min(uint32_t v) {
+    if constexpr (L >= 2) v = min(v, dpp_mov<kQuadXor1>(v));
+    if constexpr (L >= 4) v = min(v, dpp_mov<kQuadXor2>(v));
+    if constexpr (L >= 8) v = min(v, dpp_mov<kRowHalfMirror>(v));
+    if constexpr (L >= 16) v = min(v, dpp_mov<kRowMirror>(v));
+    if constexpr (L >= 32) {
+        auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        v = min(v, min((uint32_t)p[0], (uint32_t)p[1]));
+    }
+    if constexpr (L >= 64) {
+        auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        v = min(v, min((uint32_t)p[0], (uint32_t)p[1]));
+    }
+    return v;
+}
+
+// Value held by lane sub-1 / sub+1 of the same group (lanes at the group
+// edge receive a neighbour group's value, which callers mask).
+template <int L>
+__device__ __forceinline__ uint32_t from_lower_lane(uint32_t v) {
+    if constexpr (L <= 16) return dpp_mov<kRowShr1>(v);
+    else return dpp_mov<kWaveShr1>(v);
+}
+template <int L>
+__device__ __forceinline__ uint32_t from_upper_lane(uint32_t v) {
+    if constexpr (L <= 16) return dpp_mov<kRowShl1>(v);
+    else return dpp_mov<kWaveShl1>(v);
+}
+
+__device__ __forceinline__ void scan_start(int s, int dx, int dy, int w, int h,
+                                           int &sx, int &sy, int &len) {
+    if (dy == 0) {
+        sy = s;
+        sx = dx > 0 ? 0 : w - 1;
+    } else if (dx == 0) {
+        sx = s;
+        sy = dy > 0 ? 0 : h - 1;
+    } else if (s < w) {
+        sx = s;
+        sy = dy > 0 ? 0 : h - 1;
+    } else {
+        int j = s - w;
+        sx = dx > 0 ? 0 : w - 1;
+        sy = dy > 0 ? j + 1 : h - 2 - j;
+    }
+    int lx = dx > 0 ? w - sx : (dx < 0 ? sx + 1 : 0x7FFFFFFF);
+    int ly = dy > 0 ? h - sy : (dy < 0 ? sy + 1 : 0x7FFFFFFF);
+    len = lx < ly ? lx : ly;
+}
+
+// K consecutive u16 volume entries of one lane, packed two per dword.
+template <int K>
+struct Seg {
+    static constexpr int N = K >= 2 ? K / 2 : 1;
+    uint32_t w[N];
+    __device__ __forceinline__ void load(const uint16_t *p) {
+        if constexpr (K == 1) w[0] = *p;
+        else if constexpr (K == 2) w[0] = *(const uint32_t *)p;
+        else if constexpr (K == 4) {
+            uint2 v = *(const uint2 *)p;
+            w[0] = v.x; w[1] = v.y;
+        } else {
+#pragma unroll
+            for (int j = 0; j < K / 8; ++j) {
+                uint4 v = ((const uint4 *)p)[j];
+                w[4 * j] = v.x; w[4 * j + 1] = v.y; w[4 * j + 2] = v.z; w[4 * j + 3] = v.w;
+            }
+        }
+    }
+    __device__ __forceinline__ void store(uint16_t *p) const {
+        if constexpr (K == 1) *p = (uint16_t)w[0];
+        else if constexpr (K == 2) *(uint32_t *)p = w[0];
+        else if constexpr (K == 4) *(uint2 *)p = make_uint2(w[0], w[1]);
+        else {
+#pragma unroll
+            for (int j = 0; j < K / 8; ++j)
+                ((uint4 *)p)[j] = make_uint4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
+        }
+    }
+    // w = pack(agg) (+ w if accumulate), each half mod 2^16.
+    __device__ __forceinline__ void set(const uint32_t *agg, bool accumulate) {
+        if constexpr (K == 1) {
+            w[0] = accumulate ? ((w[0] + agg[0]) & 0xFFFFu) : agg[0];
+        } else {
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                uint32_t pk = agg[2 * j] | (agg[2 * j + 1] << 16);
+                if (accumulate) {
+                    u16x2 a = __builtin_bit_cast(u16x2, pk);
+                    u16x2 b = __builtin_bit_cast(u16x2, w[j]);
+                    pk = __builtin_bit_cast(uint32_t, (u16x2)(a + b));
+                }
+                w[j] = pk;
+            }
+        }
+    }
+};
+
+// Census word of pixel (x, y) (SGMStereo.metal:31-65).
+__device__ __forceinline__ uint2 census_pixel(const uint8_t *__restrict__ img, int w,
+                                              int h, int rx, int ry, int x, int y) {
+    const uint32_t c = img[(size_t)y * w + x];
+    uint32_t lo = 0, hi = 0;
+    int bit = 0;
+    for (int dy = -ry; dy <= ry; ++dy) {
+        int sy = min(max(y + dy, 0), h - 1);
+        const uint8_t *row = img + (size_t)sy * w;
+        for (int dx = -rx; dx <= rx; ++dx) {
+            if (dx == 0 && dy == 0) continue;
+            int sx = min(max(x + dx, 0), w - 1);
+            if ((uint32_t)row[sx] < c) {
+                if (bit < 32) lo |= 1u << bit;
+                else hi |= 1u << (bit - 32);
+            }
+            ++bit;
+        }
+    }
+    return make_uint2(lo, hi);
+}
+
+// One wave aggregates scanlines s_first .. s_first + 64/L - 1 of direction
+// (dx, dy) (those below p.nscan), adding into the summed volume.  Wave-level:
+// every lane of the wave must call it with the same arguments.
+template <int L, int K>
+__device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int s_first) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane / L, sub = lane % L;
+    const int s = s_first + g;
+    const int w = p.w, h = p.h, D = p.D, Dp = p.Dp;
+    const uint32_t P1 = p.p1, P2 = p.p2;
+    const bool accumulate = p.accumulate != 0;
+    const int d0 = sub * K;
+    const bool lane_real = d0 < D;
+    const bool lane_store = d0 < Dp;
+
+    int sx = 0, sy = 0, len = 0;
+    if (s < p.nscan) scan_start(s, dx, dy, w, h, sx, sy, len);
+    int maxlen = len;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, o));
+    if (maxlen == 0) return;
+
+    uint32_t prev[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) prev[i] = 0;
+    uint32_t pm = 0;
+
+    // Loads for one step, issued one step ahead.
+    uint2 lv_n = make_uint2(0u, 0u);
+    uint2 rv_n[K];
+    Seg<K> sv_n;
+    auto issue = [&](int t) {
+        const bool a = t < len;
+        const int x = sx + t * dx, y = sy + t * dy;
+        const size_t pix = a ? (size_t)y * w + x : 0;
+        lv_n = p.lc[pix];
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const int d = d0 + i;
+            const size_t ri = (a && x - d >= 0) ? pix - d : pix;
+            rv_n[i] = p.rc[ri];
+        }
+        if (accumulate && a && lane_store) sv_n.load(p.vol + pix * (size_t)Dp + d0);
+    };
+    issue(0);
+
+    for (int t = 0; t < maxlen; ++t) {
+        const bool act = t < len;
+        const int x = sx + t * dx, y = sy + t * dy;
+        const size_t pix = act ? (size_t)y * w + x : 0;
+        const uint2 lv = lv_n;
+        uint2 rv[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) rv[i] = rv_n[i];
+        Seg<K> sv = sv_n;
+        if (t + 1 < maxlen) issue(t + 1);
+
+        uint32_t C[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const int d = d0 + i;
+            const uint32_t c = __popc(lv.x ^ rv[i].x) + __popc(lv.y ^ rv[i].y);
+            C[i] = x - d >= 0 ? c : 64u;
+        }
+
+        uint32_t agg[K];
+        if (t == 0) {
+#pragma unroll
+            for (int i = 0; i < K; ++i) agg[i] = C[i];
+        } else {
+            const uint32_t o3 = (pm + P2) & 0xFFFFu;
+            const uint32_t nb = from_upper_lane<L>(prev[0]);
+            uint32_t A[K], B[K];
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                const int d = d0 + i;
+                const uint32_t nx = (i + 1 < K) ? prev[(i + 1) % K] : nb;
+                const uint32_t o2 = (d + 1 < D) ? ((nx + P1) & 0xFFFFu) : 0xFFFFu;
+                A[i] = min(min(prev[i], o2), o3);
+                B[i] = (C[i] - pm) & 0xFFFFu;
+            }
+            // Speculative pass: no carry-in from the lane below.
+            {
+                uint32_t o1 = 0xFFFFu;
+#pragma unroll
+                for (int i = 0; i < K; ++i) {
+                    agg[i] = (B[i] + min(A[i], o1)) & 0xFFFFu;
+                    o1 = (agg[i] + P1) & 0xFFFFu;
+                }
+            }
+            // Fixed-point rounds over the carry-in (at most L).
+            const bool live = act && lane_real && sub > 0;
+            uint32_t e_used = A[0];
+            for (int r = 0; r < L; ++r) {
+                const uint32_t up = from_lower_lane<L>(agg[K - 1]);
+                const uint32_t e = min(A[0], (up + P1) & 0xFFFFu);
+                const bool ch = live && e != e_used;
+                if (__builtin_amdgcn_ballot_w64(ch) == 0) break;
+                if (ch) {
+                    e_used = e;
+                    agg[0] = (B[0] + e) & 0xFFFFu;
+                    uint32_t o1 = (agg[0] + P1) & 0xFFFFu;
+#pragma unroll
+                    for (int i = 1; i < K; ++i) {
+                        agg[i] = (B[i] + min(A[i], o1)) & 0xFFFFu;
+                        o1 = (agg[i] + P1) & 0xFFFFu;
+                    }
+                }
+            }
+        }
+
+        uint32_t m = 0xFFFFu;
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            if (d0 + i < D) m = min(m, agg[i]);
+        pm = group_min<L>(m);
+#pragma unroll
+        for (int i = 0; i < K; ++i) prev[i] = agg[i];
+
+        if (act && lane_store) {
+            sv.set(agg, accumulate);
+            sv.store(p.vol + pix * (size_t)Dp + d0);
+        }
+    }
+}
+
+// Winner-take-all and parabola refinement of pixel i (SGMStereo.metal:185-213).
+__device__ __forceinline__ void wta_pixel(const uint16_t *__restrict__ vol, int D, int Dp,
+                                          size_t i, float *__restrict__ out,
+                                          int16_t *__restrict__ best) {
+    const uint16_t *v = vol + i * Dp;
+    uint32_t bc = 0xFFFFu;
+    int bd = 0;
+    for (int d = 0; d < D; ++d) {
+        const uint32_t c = v[d];
+        if (c < bc) {
+            bc = c;
+            bd = d;
+        }
+    }
+    float disp = (float)bd;
+    if (bd > 0 && bd < D - 1) {
+        const float c0 = (float)v[bd - 1];
+        const float c1 = (float)bc;
+        const float c2 = (float)v[bd + 1];
+        const float den = 2.0f * (c0 + c2 - 2.0f * c1);
+        if (fabsf(den) > 1e-6f) disp = disp + (c0 - c2) / den;
+    }
+    out[i] = disp;
+    if (best) best[i] = (int16_t)bd;
+}
+
+// Left-right check of pixel i: D_R(x') is the lowest d minimising
+// S(x'+d, y, d) over x'+d < w; the pixel is kept iff x - best >= 0 and
+// |best - D_R(x - best)| <= tol, otherwise it becomes -1.0f.
+__device__ __forceinline__ void lr_pixel(const uint16_t *__restrict__ vol, int w, int D,
+                                         int Dp, int tol, size_t i,
+                                         const int16_t *__restrict__ best,
+                                         float *__restrict__ out) {
+    const int x = (int)(i % w), y = (int)(i / w);
+    const int d = best[i];
+    const int xr = x - d;
+    bool ok = false;
+    if (xr >= 0) {
+        uint32_t bc = 0xFFFFu;
+        int bd = 0;
+        for (int k = 0; k < D && xr + k < w; ++k) {
+            const uint32_t c = vol[((size_t)y * w + xr + k) * Dp + k];
+            if (c < bc) {
+                bc = c;
+                bd = k;
+            }
+        }
+        ok = abs(d - bd) <= tol;
+    }
+    if (!ok) out[i] = -1.0f;
+}
+
+}  // namespace dev
+}  // namespace vxh

@@ -22,11 +22,22 @@ inline int num_scanlines(int dir, int w, int h) {
     return w + h - 1;
 }
 
+// Per-pixel stride, in u16 elements, of the summed cost volume: D rounded up
+// to 16 so every pixel's disparity slice starts 32-byte aligned and each lane's
+// packed segment of it never straddles a pixel.
+inline int volume_stride(int D) { return (D + 15) & ~15; }
+
+// Lane layout of the aggregation kernel: L lanes own one scanline, each lane
+// K consecutive disparities (L * K >= D).
+struct Layout {
+    int L, K;
+};
+
 struct AggParams {
     const uint2 *lc;       // left census, w*h (lo, hi)
     const uint2 *rc;       // right census
-    uint16_t *vol;         // summed volume, (y*w + x)*D + d
-    int w, h, D;
+    uint16_t *vol;         // summed volume, (y*w + x)*Dp + d
+    int w, h, D, Dp;
     uint32_t p1, p2;       // penalties already truncated to u16
     int dir;
     int nscan;
@@ -35,10 +46,13 @@ struct AggParams {
 
 hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
                          uint2 *out, hipStream_t s);
-hipError_t launch_aggregate(const AggParams &p, hipStream_t s);
-hipError_t launch_wta(const uint16_t *vol, int w, int h, int D, float *out,
-                      int16_t *best, hipStream_t s);
-hipError_t launch_lr_check(const uint16_t *vol, int w, int h, int D,
+// Default lane layout for D disparities.
+Layout default_layout(int D);
+bool layout_supported(Layout l, int D);
+hipError_t launch_aggregate(const AggParams &p, Layout l, hipStream_t s);
+hipError_t launch_wta(const uint16_t *vol, int w, int h, int D, int Dp,
+                      float *out, int16_t *best, hipStream_t s);
+hipError_t launch_lr_check(const uint16_t *vol, int w, int h, int D, int Dp,
                            int tol, const int16_t *best, float *out,
                            hipStream_t s);
 

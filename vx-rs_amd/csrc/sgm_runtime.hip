@@ -67,7 +67,7 @@ Layout workspace_layout(uint32_t w, uint32_t h, uint32_t D) {
     l.lc = 0;
     l.rc = l.lc + align_up(n * 8);
     l.vol = l.rc + align_up(n * 8);
-    l.best = l.vol + align_up(n * D * 2);
+    l.best = l.vol + align_up(n * (size_t)vxh::volume_stride((int)D) * 2);
     l.total = l.best + align_up(n * 2);
     return l;
 }
@@ -274,6 +274,31 @@ size_t vxh_sgm_workspace_bytes(uint32_t w, uint32_t h, const vxh_sgm_config *c) 
     return workspace_layout(w, h, c->num_disparities).total;
 }
 
+static vxh_status aggregate_impl(const uint2 *lc, const uint2 *rc, uint32_t w,
+                                 uint32_t h, const vxh_sgm_config *c, uint16_t *vol,
+                                 vxh::Layout lay, hipStream_t st) {
+    const int D = (int)c->num_disparities;
+    if (!vxh::layout_supported(lay, D))
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
+    vxh::AggParams p{};
+    p.lc = lc;
+    p.rc = rc;
+    p.vol = vol;
+    p.w = (int)w;
+    p.h = (int)h;
+    p.D = D;
+    p.Dp = vxh::volume_stride(D);
+    p.p1 = (uint32_t)c->p1 & 0xFFFFu;
+    p.p2 = (uint32_t)c->p2 & 0xFFFFu;
+    for (uint32_t i = 0; i < c->num_paths; ++i) {
+        p.dir = (int)i;
+        p.nscan = vxh::num_scanlines((int)i, (int)w, (int)h);
+        p.accumulate = i > 0;
+        VXH_HIP(vxh::launch_aggregate(p, lay, st), "aggregate");
+    }
+    return VXH_OK;
+}
+
 static vxh_status enqueue_impl(const uint8_t *d_left, const uint8_t *d_right,
                                uint32_t w, uint32_t h, const vxh_sgm_config *c,
                                float *d_out, void *d_ws, size_t ws_bytes,
@@ -299,24 +324,14 @@ static vxh_status enqueue_impl(const uint8_t *d_left, const uint8_t *d_right,
             "census");
     VXH_HIP(vxh::launch_census(d_right, w, h, c->census_radius_x, c->census_radius_y, rc, st),
             "census");
-    vxh::AggParams p{};
-    p.lc = lc;
-    p.rc = rc;
-    p.vol = vol;
-    p.w = (int)w;
-    p.h = (int)h;
-    p.D = D;
-    p.p1 = (uint32_t)c->p1 & 0xFFFFu;
-    p.p2 = (uint32_t)c->p2 & 0xFFFFu;
-    for (uint32_t i = 0; i < c->num_paths; ++i) {
-        p.dir = (int)i;
-        p.nscan = vxh::num_scanlines((int)i, (int)w, (int)h);
-        p.accumulate = i > 0;
-        VXH_HIP(vxh::launch_aggregate(p, st), "aggregate");
-    }
-    VXH_HIP(vxh::launch_wta(vol, w, h, D, d_out, c->lr_check ? best : nullptr, st), "wta");
+    vxh_status v2 = aggregate_impl(lc, rc, w, h, c, vol, vxh::default_layout(D), st);
+    if (v2 != VXH_OK) return v2;
+    const int Dp = vxh::volume_stride(D);
+    VXH_HIP(vxh::launch_wta(vol, w, h, D, Dp, d_out, c->lr_check ? best : nullptr, st),
+            "wta");
     if (c->lr_check)
-        VXH_HIP(vxh::launch_lr_check(vol, w, h, D, (int)c->lr_tolerance, best, d_out, st),
+        VXH_HIP(vxh::launch_lr_check(vol, w, h, D, Dp, (int)c->lr_tolerance, best, d_out,
+                                     st),
                 "lr_check");
     return VXH_OK;
 }
@@ -450,29 +465,27 @@ vxh_status vxh_sgm_census(const uint8_t *d_img, uint32_t w, uint32_t h, uint32_t
     return VXH_OK;
 }
 
-vxh_status vxh_sgm_aggregate(const uint32_t *d_lc, const uint32_t *d_rc, uint32_t w,
-                             uint32_t h, const vxh_sgm_config *c, uint16_t *d_volume,
-                             void *stream) {
+uint32_t vxh_sgm_volume_stride(uint32_t nd) { return (uint32_t)vxh::volume_stride((int)nd); }
+
+vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,
+                                    uint32_t w, uint32_t h, const vxh_sgm_config *c,
+                                    uint32_t lanes, uint32_t per_lane,
+                                    uint16_t *d_volume, void *stream) {
     vxh_status v = vxh_sgm_config_validate(c, w, h);
     if (v != VXH_OK) return v;
     if (!d_lc || !d_rc || !d_volume)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null device buffer");
-    vxh::AggParams p{};
-    p.lc = (const uint2 *)d_lc;
-    p.rc = (const uint2 *)d_rc;
-    p.vol = d_volume;
-    p.w = (int)w;
-    p.h = (int)h;
-    p.D = (int)c->num_disparities;
-    p.p1 = (uint32_t)c->p1 & 0xFFFFu;
-    p.p2 = (uint32_t)c->p2 & 0xFFFFu;
-    for (uint32_t i = 0; i < c->num_paths; ++i) {
-        p.dir = (int)i;
-        p.nscan = vxh::num_scanlines((int)i, (int)w, (int)h);
-        p.accumulate = i > 0;
-        VXH_HIP(vxh::launch_aggregate(p, (hipStream_t)stream), "aggregate");
-    }
-    return VXH_OK;
+    vxh::Layout lay = (lanes == 0 && per_lane == 0)
+                          ? vxh::default_layout((int)c->num_disparities)
+                          : vxh::Layout{(int)lanes, (int)per_lane};
+    return aggregate_impl((const uint2 *)d_lc, (const uint2 *)d_rc, w, h, c, d_volume, lay,
+                          (hipStream_t)stream);
+}
+
+vxh_status vxh_sgm_aggregate(const uint32_t *d_lc, const uint32_t *d_rc, uint32_t w,
+                             uint32_t h, const vxh_sgm_config *c, uint16_t *d_volume,
+                             void *stream) {
+    return vxh_sgm_aggregate_layout(d_lc, d_rc, w, h, c, 0, 0, d_volume, stream);
 }
 
 vxh_status vxh_sgm_wta(const uint16_t *d_volume, uint32_t w, uint32_t h,
@@ -482,7 +495,8 @@ vxh_status vxh_sgm_wta(const uint16_t *d_volume, uint32_t w, uint32_t h,
     if (c->lr_check)
         return fail(VXH_ERR_INVALID_CONFIG,
                     "invalid config: the stage-level WTA entry point has no LR check");
-    VXH_HIP(vxh::launch_wta(d_volume, w, h, (int)c->num_disparities, d_out, nullptr,
+    const int D = (int)c->num_disparities;
+    VXH_HIP(vxh::launch_wta(d_volume, w, h, D, vxh::volume_stride(D), d_out, nullptr,
                             (hipStream_t)stream),
             "wta");
     return VXH_OK;

@@ -89,6 +89,9 @@ _sig("vxh_sgm_enqueue", ctypes.c_int, _vp, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp
 _sig("vxh_sgm_census", ctypes.c_int, _vp, _u32, _u32, _u32, _u32, _vp, _vp)
 _sig("vxh_sgm_aggregate", ctypes.c_int, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp)
 _sig("vxh_sgm_wta", ctypes.c_int, _vp, _u32, _u32, _cfgp, _vp, _vp)
+_sig("vxh_sgm_volume_stride", _u32, _u32)
+_sig("vxh_sgm_aggregate_layout", ctypes.c_int, _vp, _vp, _u32, _u32, _cfgp, _u32, _u32,
+     _vp, _vp)
 _sig("vxh_synth_pair", ctypes.c_int, _u32, _u32, _u32, _u32, _u32,
      ctypes.c_uint64, _vp, _vp, _vp)
 
@@ -302,6 +305,193 @@ class SGMStereo:
         if self._h:
             lib.vxh_sgm_destroy(self._h)
             self._h = _vp()
+
+
+class vxh_sgm_pair_desc(ctypes.Structure):
+    _fields_ = [("d_left", _vp), ("d_right", _vp), ("d_out", _vp), ("width", _u32),
+                ("height", _u32), ("cfg", vxh_sgm_config), ("lanes", _u32),
+                ("per_lane", _u32)]
+
+
+class vxh_sgm_batch_stats(ctypes.Structure):
+    _fields_ = [("items", _u32), ("lanes", _u32), ("per_lane", _u32)]
+
+
+_sig("vxh_sgm_batch_create", ctypes.c_int, ctypes.c_int, ctypes.POINTER(vxh_sgm_pair_desc),
+     _u32, ctypes.POINTER(_vp))
+_sig("vxh_sgm_batch_launch", ctypes.c_int, _vp, _vp)
+_sig("vxh_sgm_batch_run", ctypes.c_int, _vp)
+_sig("vxh_sgm_batch_stats_get", ctypes.c_int, _vp, _u32, ctypes.POINTER(vxh_sgm_batch_stats))
+_sig("vxh_sgm_batch_items", _u32, _vp)
+_sig("vxh_sgm_batch_destroy", None, _vp)
+_sig("vxh_sgm_batch_last_error", ctypes.c_char_p)
+
+
+def _check_batch(status):
+    if status != 0:
+        msg = lib.vxh_sgm_batch_last_error().decode(errors="replace")
+        raise _ERRORS.get(status, Error)(msg)
+
+
+class SGMBatch:
+    """Many pairs in one persistent launch replayed from a hipGraph.
+
+    `pairs` is a list of (d_left, d_right, d_out, width, height, config,
+    schedule) with device addresses; schedule is (lanes, per_lane) or None."""
+
+    def __init__(self, pairs, device=0):
+        arr = (vxh_sgm_pair_desc * len(pairs))()
+        for i, (l, r, o, w, h, cfg, sched) in enumerate(pairs):
+            lanes, per = sched if sched else (0, 0)
+            arr[i] = vxh_sgm_pair_desc(l, r, o, w, h, cfg.to_c(), lanes, per)
+        h = _vp()
+        _check_batch(lib.vxh_sgm_batch_create(device, arr, len(pairs), ctypes.byref(h)))
+        self._h = h
+        self.n = len(pairs)
+
+    def launch(self, stream=None):
+        _check_batch(lib.vxh_sgm_batch_launch(self._h, stream))
+
+    def run(self):
+        _check_batch(lib.vxh_sgm_batch_run(self._h))
+
+    def stats(self, i):
+        s = vxh_sgm_batch_stats()
+        _check_batch(lib.vxh_sgm_batch_stats_get(self._h, i, ctypes.byref(s)))
+        return {"items": s.items, "lanes": s.lanes, "per_lane": s.per_lane}
+
+    @property
+    def items(self):
+        return lib.vxh_sgm_batch_items(self._h)
+
+    def close(self):
+        if self._h:
+            lib.vxh_sgm_batch_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class vxh_server_config(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", _u32),
+                ("num_streams", _u32)]
+
+
+class vxh_server_stats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("rejected", ctypes.c_uint64), ("peak_admitted_bytes", ctypes.c_uint64),
+                ("peak_inflight", ctypes.c_uint64), ("slot_hits", ctypes.c_uint64),
+                ("slot_misses", ctypes.c_uint64), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("mean_queue_ms", ctypes.c_double), ("pxd_total", ctypes.c_double),
+                ("first_submit_s", ctypes.c_double), ("last_complete_s", ctypes.c_double)]
+
+
+_sig("vxh_server_request_bytes", ctypes.c_uint64, _u32, _u32, _cfgp)
+_sig("vxh_server_create", ctypes.c_int, ctypes.c_int, ctypes.POINTER(vxh_server_config),
+     ctypes.POINTER(_vp))
+_sig("vxh_server_submit", ctypes.c_int, _vp, _vp, _vp, _u32, _u32, _cfgp, _vp,
+     ctypes.POINTER(ctypes.c_uint64))
+_sig("vxh_server_wait", ctypes.c_int, _vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double))
+_sig("vxh_server_drain", ctypes.c_int, _vp)
+_sig("vxh_server_get_stats", ctypes.c_int, _vp, ctypes.POINTER(vxh_server_stats))
+_sig("vxh_server_reset_stats", None, _vp)
+_sig("vxh_server_destroy", None, _vp)
+_sig("vxh_server_last_error", ctypes.c_char_p)
+
+
+def _check_server(status):
+    if status != 0:
+        msg = lib.vxh_server_last_error().decode(errors="replace")
+        raise _ERRORS.get(status, Error)(msg)
+
+
+def request_bytes(width, height, config):
+    c = config.to_c()
+    return lib.vxh_server_request_bytes(width, height, ctypes.byref(c))
+
+
+class SGMServer:
+    """Serving loop with HBM-budget admission (include/vxh/sgm_server.h)."""
+
+    def __init__(self, hbm_budget_bytes=64 << 30, max_inflight=16, num_streams=8, device=0):
+        cfg = vxh_server_config(hbm_budget_bytes, max_inflight, num_streams)
+        h = _vp()
+        _check_server(lib.vxh_server_create(device, ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, left, right, width, height, config, out):
+        """Queues a request; `out` (float32, w*h) is filled on completion."""
+        l = _as_u8(left).ravel()
+        r = _as_u8(right).ravel()
+        assert out.dtype == np.float32 and out.flags.c_contiguous and out.size == width * height
+        c = config.to_c()
+        rid = ctypes.c_uint64()
+        _check_server(lib.vxh_server_submit(self._h, l.ctypes.data, r.ctypes.data, width,
+                                            height, ctypes.byref(c), out.ctypes.data,
+                                            ctypes.byref(rid)))
+        self._keep[rid.value] = out
+        return rid.value
+
+    def wait(self, rid):
+        lat = ctypes.c_double()
+        try:
+            _check_server(lib.vxh_server_wait(self._h, rid, ctypes.byref(lat)))
+        finally:
+            self._keep.pop(rid, None)
+        return lat.value
+
+    def drain(self):
+        _check_server(lib.vxh_server_drain(self._h))
+
+    def stats(self):
+        s = vxh_server_stats()
+        _check_server(lib.vxh_server_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in vxh_server_stats._fields_}
+
+    def reset_stats(self):
+        lib.vxh_server_reset_stats(self._h)
+
+    def close(self):
+        if self._h:
+            lib.vxh_server_destroy(self._h)
+            self._h = _vp()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+MIX_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080)]
+MIX_DISPARITIES = [64, 128, 256]
+
+
+def mixture(n, seed=0x5347_0003, sizes=MIX_SIZES, disparities=MIX_DISPARITIES):
+    """The batched / served workload mixture: n pairs, each a size and D drawn
+    with a seeded generator, 8 paths.  Returns [(w, h, D, pair_seed)]."""
+    import numpy as _np
+    rng = _np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        w, h = sizes[int(rng.integers(len(sizes)))]
+        d = disparities[int(rng.integers(len(disparities)))]
+        out.append((w, h, d, int(seed * 1000 + i)))
+    return out
+
+
+def mixture_pair(w, h, d, pair_seed):
+    left, right, _ = synth_pair(w, h, d, 12 if w >= 1280 else 6, 2 if w >= 1280 else 0,
+                                pair_seed)
+    cfg = SGMStereoConfig(num_disparities=d, p1=10.0, p2=120.0, num_paths=8)
+    return left, right, cfg
 
 
 def workspace_bytes(width, height, config):

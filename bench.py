@@ -38,7 +38,131 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle threads for the CPU baseline (0 = all host cores)")
+    ap.add_argument("--forms", default="plain,batched,served",
+                    help="comma list of forms to run after the plain headline")
+    ap.add_argument("--batch-pairs", type=int, default=64)
+    ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--rates", default="200,500,1000")
+    ap.add_argument("--requests", type=int, default=300)
+    ap.add_argument("--no-tune", action="store_true")
     return ap.parse_args()
+
+
+def batched_leg(vx, torch, np, args, sgm, tuner):
+    """64 pairs of the mixture in one persistent graph launch vs a loop of
+    plain per-pair calls on one stream; every map compared bit for bit."""
+    mix = vx.mixture(args.batch_pairs)
+    host, dev_in, outs_b, outs_p, descs = [], [], [], [], []
+    for w, h, d, seed in mix:
+        left, right, cfg = vx.mixture_pair(w, h, d, seed)
+        host.append((w, h, d, cfg))
+        dl, dr = torch.from_numpy(left).cuda(), torch.from_numpy(right).cuda()
+        dev_in.append((dl, dr))
+        ob = torch.empty(w * h, dtype=torch.float32, device="cuda")
+        op = torch.empty(w * h, dtype=torch.float32, device="cuda")
+        outs_b.append(ob)
+        outs_p.append(op)
+        sched = None if (args.no_tune or tuner is None) else tuner.tune_batch(w, h, d, 8)
+        descs.append((dl.data_ptr(), dr.data_ptr(), ob.data_ptr(), w, h, cfg, sched))
+    stream = torch.cuda.current_stream()
+    batch = vx.SGMBatch(descs)
+    batch.launch(stream.cuda_stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.batch_reps):
+        batch.launch(stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_batch = e0.elapsed_time(e1) / args.batch_reps
+    # Plain loop: the per-pair path for each pair, one after another.
+    wsb = max(vx.workspace_bytes(w, h, cfg) for w, h, _, cfg in host)
+    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+
+    def plain_loop():
+        for (w, h, d, cfg), (dl, dr), op in zip(host, dev_in, outs_p):
+            sgm.enqueue(dl.data_ptr(), dr.data_ptr(), w, h, cfg, op.data_ptr(),
+                        ws.data_ptr(), wsb, stream.cuda_stream)
+
+    plain_loop()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(max(1, args.batch_reps // 2)):
+        plain_loop()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_plain = e0.elapsed_time(e1) / max(1, args.batch_reps // 2)
+    exact = all(torch.equal(a.view(torch.int32), b.view(torch.int32))
+                for a, b in zip(outs_b, outs_p))
+    pxd = sum(w * h * d for w, h, d, _ in host)
+    alg = sum(w * h * (4 * d + 6) for w, h, d, _ in host)
+    batch.close()
+    return {
+        "pairs": len(host), "gpxd": round(pxd / 1e9, 4),
+        "ms_batch": round(ms_batch, 3), "ms_plain_loop": round(ms_plain, 3),
+        "value_batch": round(pxd / (ms_batch * 1e-3) / 1e9, 3),
+        "value_plain_loop": round(pxd / (ms_plain * 1e-3) / 1e9, 3),
+        "unit": "Gpx*disp/s",
+        "roofline_frac_batch": round(alg / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "bit_exact_vs_plain": bool(exact),
+        "form": "persistent kernel replayed from a hipGraph",
+    }
+
+
+def served_leg(vx, np, args):
+    """Poisson arrivals of the mixture against the serving loop and against a
+    FIFO queue into the plain per-pair path (one request at a time)."""
+    import threading
+    import time as _t
+    pool = []
+    for w, h, d, seed in vx.mixture(32, seed=0x5347_0004):
+        left, right, cfg = vx.mixture_pair(w, h, d, seed)
+        pool.append((left, right, w, h, cfg))
+    ref = [None] * len(pool)
+    out = {}
+    for name, kw in (("served", dict(max_inflight=32, num_streams=8)),
+                     ("plain_fifo", dict(max_inflight=1, num_streams=1))):
+        srv = vx.SGMServer(hbm_budget_bytes=64 << 30, **kw)
+        # Warm the slot pool and capture reference maps.
+        for k, (l, r, w, h, cfg) in enumerate(pool):
+            o = np.empty(w * h, np.float32)
+            srv.wait(srv.submit(l, r, w, h, cfg, o))
+            if ref[k] is None:
+                ref[k] = o
+            elif not np.array_equal(o.view(np.uint32), ref[k].view(np.uint32)):
+                out["bit_exact"] = False
+        rates = {}
+        for rate in [float(x) for x in args.rates.split(",")]:
+            rng = np.random.default_rng(int(rate))
+            gaps = rng.exponential(1.0 / rate, args.requests)
+            srv.reset_stats()
+            outs = [None] * args.requests
+            t0 = _t.perf_counter()
+            t = t0
+            for i in range(args.requests):
+                t += gaps[i]
+                while _t.perf_counter() < t:
+                    pass
+                l, r, w, h, cfg = pool[i % len(pool)]
+                outs[i] = np.empty(w * h, np.float32)
+                srv.submit(l, r, w, h, cfg, outs[i])
+            srv.drain()
+            st = srv.stats()
+            span = st["last_complete_s"] - st["first_submit_s"]
+            exact = all(np.array_equal(o.view(np.uint32), ref[i % len(pool)].view(np.uint32))
+                        for i, o in enumerate(outs))
+            rates[str(int(rate))] = {
+                "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                "gpxd_per_s": round(st["pxd_total"] / span / 1e9, 3),
+                "offered_gpxd_per_s": round(rate * st["pxd_total"] / st["completed"] / 1e9, 3),
+                "peak_inflight": st["peak_inflight"],
+                "peak_admitted_gb": round(st["peak_admitted_bytes"] / 2**30, 3),
+                "bit_exact": bool(exact)}
+        srv.close()
+        out[name] = rates
+    out["requests_per_rate"] = args.requests
+    out["budget_gb"] = 64
+    return out
 
 
 def stage_times(vx, torch, d_left, d_right, w, h, cfg, d_out, ws, reps=10):
@@ -94,6 +218,13 @@ def main():
     ws_bytes = vx.workspace_bytes(w, h, cfg)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
+
+    tuner = None
+    plain_sched = None
+    if not args.no_tune:
+        from vxsgm.tuner import Tuner
+        tuner = Tuner(path=None)
+        plain_sched = tuner.tune_plain(sgm, w, h, D, paths)
 
     def step():
         sgm.enqueue(d_left.data_ptr(), d_right.data_ptr(), w, h, cfg, d_out.data_ptr(),
@@ -158,6 +289,14 @@ def main():
                       f"{threads} threads over independent scanlines, {cpu_s:.2f} s"}
         result["bit_exact"] = bool((gpu_disp.view(np.uint32) ==
                                     want.astype(np.float32).view(np.uint32)).all())
+    result["config"]["schedule"] = list(plain_sched) if plain_sched else "default"
+    forms = set(args.forms.split(","))
+    if "batched" in forms:
+        print(f"bench: batched leg ({args.batch_pairs} pairs)", file=sys.stderr, flush=True)
+        result["batched"] = batched_leg(vx, torch, np, args, sgm, tuner)
+    if "served" in forms:
+        print(f"bench: served leg (rates {args.rates})", file=sys.stderr, flush=True)
+        result["served"] = served_leg(vx, np, args)
     print(json.dumps(result), flush=True)
 
 

@@ -179,6 +179,13 @@ vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,
 vxh_status vxh_sgm_wta(const uint16_t *d_volume, uint32_t w, uint32_t h,
                        const vxh_sgm_config *cfg, float *d_out, void *stream);
 
+/* Selects the aggregation lane layout this handle uses for pairs with exactly
+ * `num_disparities` disparities (lanes 16/32/64, per_lane as for
+ * vxh_sgm_aggregate_layout); 0, 0 restores the default.  Results do not
+ * depend on the layout, only the speed does. */
+vxh_status vxh_sgm_set_layout(vxh_sgm *sgm, uint32_t num_disparities,
+                              uint32_t lanes, uint32_t per_lane);
+
 /* Thread-local message describing the last failure on this thread. */
 const char *vxh_last_error(void);
 /* Library version string. */

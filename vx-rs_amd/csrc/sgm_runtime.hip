@@ -97,7 +97,14 @@ struct vxh_sgm {
     std::mutex mu;
     std::vector<Slot *> free_slots;
     std::vector<Slot *> all_slots;
+    vxh::Layout layouts[257] = {};   // per D; L == 0 selects the default
 };
+
+static vxh::Layout layout_for(vxh_sgm *s, int D) {
+    std::lock_guard<std::mutex> g(s->mu);
+    vxh::Layout l = s->layouts[D];
+    return l.L == 0 ? vxh::default_layout(D) : l;
+}
 
 extern "C" {
 
@@ -299,7 +306,7 @@ static vxh_status aggregate_impl(const uint2 *lc, const uint2 *rc, uint32_t w,
     return VXH_OK;
 }
 
-static vxh_status enqueue_impl(const uint8_t *d_left, const uint8_t *d_right,
+static vxh_status enqueue_impl(vxh_sgm *s, const uint8_t *d_left, const uint8_t *d_right,
                                uint32_t w, uint32_t h, const vxh_sgm_config *c,
                                float *d_out, void *d_ws, size_t ws_bytes,
                                hipStream_t st) {
@@ -324,7 +331,7 @@ static vxh_status enqueue_impl(const uint8_t *d_left, const uint8_t *d_right,
             "census");
     VXH_HIP(vxh::launch_census(d_right, w, h, c->census_radius_x, c->census_radius_y, rc, st),
             "census");
-    vxh_status v2 = aggregate_impl(lc, rc, w, h, c, vol, vxh::default_layout(D), st);
+    vxh_status v2 = aggregate_impl(lc, rc, w, h, c, vol, layout_for(s, D), st);
     if (v2 != VXH_OK) return v2;
     const int Dp = vxh::volume_stride(D);
     VXH_HIP(vxh::launch_wta(vol, w, h, D, Dp, d_out, c->lr_check ? best : nullptr, st),
@@ -341,7 +348,7 @@ vxh_status vxh_sgm_enqueue(vxh_sgm *s, const uint8_t *d_left, const uint8_t *d_r
                            float *d_out, void *d_ws, size_t ws_bytes, void *stream) {
     if (!s) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null handle");
     VXH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
-    return enqueue_impl(d_left, d_right, w, h, c, d_out, d_ws, ws_bytes,
+    return enqueue_impl(s, d_left, d_right, w, h, c, d_out, d_ws, ws_bytes,
                         (hipStream_t)stream);
 }
 
@@ -404,7 +411,7 @@ vxh_status vxh_sgm_compute_disparity(vxh_sgm *s, vxh_context *ctx,
     Slot *sl = nullptr;
     v = acquire_slot(s, workspace_layout(w, h, c->num_disparities).total, &sl);
     if (v != VXH_OK) return v;
-    v = enqueue_impl((const uint8_t *)left->dptr, (const uint8_t *)right->dptr, w, h,
+    v = enqueue_impl(s, (const uint8_t *)left->dptr, (const uint8_t *)right->dptr, w, h,
                      c, (float *)out->dptr, sl->ws, sl->ws_bytes, sl->stream);
     hipError_t e = hipStreamSynchronize(sl->stream);
     release_slot(s, sl);
@@ -462,6 +469,17 @@ vxh_status vxh_sgm_census(const uint8_t *d_img, uint32_t w, uint32_t h, uint32_t
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: census window over 64 bits");
     VXH_HIP(vxh::launch_census(d_img, w, h, rx, ry, (uint2 *)d_census, (hipStream_t)stream),
             "census");
+    return VXH_OK;
+}
+
+vxh_status vxh_sgm_set_layout(vxh_sgm *s, uint32_t nd, uint32_t lanes, uint32_t per_lane) {
+    if (!s || nd < 1 || nd > 256)
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: layout target");
+    vxh::Layout l{(int)lanes, (int)per_lane};
+    if (!(lanes == 0 && per_lane == 0) && !vxh::layout_supported(l, (int)nd))
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
+    std::lock_guard<std::mutex> g(s->mu);
+    s->layouts[nd] = l;
     return VXH_OK;
 }
 

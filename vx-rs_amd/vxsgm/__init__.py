@@ -90,6 +90,7 @@ _sig("vxh_sgm_census", ctypes.c_int, _vp, _u32, _u32, _u32, _u32, _vp, _vp)
 _sig("vxh_sgm_aggregate", ctypes.c_int, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp)
 _sig("vxh_sgm_wta", ctypes.c_int, _vp, _u32, _u32, _cfgp, _vp, _vp)
 _sig("vxh_sgm_volume_stride", _u32, _u32)
+_sig("vxh_sgm_set_layout", ctypes.c_int, _vp, _u32, _u32, _u32)
 _sig("vxh_sgm_aggregate_layout", ctypes.c_int, _vp, _vp, _u32, _u32, _cfgp, _u32, _u32,
      _vp, _vp)
 _sig("vxh_synth_pair", ctypes.c_int, _u32, _u32, _u32, _u32, _u32,
@@ -293,6 +294,11 @@ class SGMStereo:
         check(lib.vxh_sgm_compute_host(self._h, l.ctypes.data, r.ctypes.data, width,
                                        height, ctypes.byref(c), out.ctypes.data))
         return out
+
+    def set_layout(self, num_disparities, schedule):
+        """Lane layout (lanes, per_lane) for pairs with this D; None = default."""
+        lanes, per = schedule if schedule else (0, 0)
+        check(lib.vxh_sgm_set_layout(self._h, num_disparities, lanes, per))
 
     def enqueue(self, d_left, d_right, width, height, config, d_out, d_workspace,
                 workspace_bytes, stream=None):

@@ -1,0 +1,118 @@
+"""Autotuner for the aggregation schedule (lane layout) of each pair class.
+
+A schedule is (lanes, per_lane): `lanes` lanes of a wave own one scanline and
+each owns `per_lane` consecutive disparities.  Results never depend on it;
+speed does, and differently for the two forms: a lone pair is bound by the
+length of its longest scanline (few, wide groups win), a batch by total
+instruction throughput (more disparities per lane win).  The tuner therefore
+times every candidate in the form it will serve:
+
+  tune_batch(w, h, D, paths)  a batch of copies of one class, persistent
+                              kernel replayed from its hipGraph;
+  tune_plain(w, h, D, paths)  the per-pair pipeline of one pair.
+
+Timings use HIP events on the launch stream; the median of `reps` replays
+after one warm-up wins.  Winners are cached in a JSON file keyed by form and
+class so that later runs skip the sweep.
+"""
+import json
+import os
+import statistics
+
+import vxsgm
+
+LAYOUTS = [(64, 1), (64, 2), (64, 4), (32, 2), (32, 4), (32, 8), (16, 4), (16, 8),
+           (16, 16)]
+DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+
+
+def candidates(D):
+    """Layouts that cover D without more than doubling it (padding lanes)."""
+    return [(L, K) for L, K in LAYOUTS if L * K >= D and L * K < 2 * max(D, 32)]
+
+
+def _key(form, w, h, D, paths):
+    return f"{form}:{w}x{h}:D{D}:P{paths}"
+
+
+class Tuner:
+    def __init__(self, path=DEFAULT_CACHE):
+        self.path = path
+        self.table = {}
+        if path and os.path.exists(path):
+            with open(path) as f:
+                self.table = json.load(f)
+
+    def save(self):
+        if self.path:
+            with open(self.path, "w") as f:
+                json.dump(self.table, f, indent=1, sort_keys=True)
+
+    def lookup(self, form, w, h, D, paths):
+        e = self.table.get(_key(form, w, h, D, paths))
+        return tuple(e["schedule"]) if e else None
+
+    def tune_batch(self, w, h, D, paths, copies=4, reps=5, force=False):
+        key = _key("batch", w, h, D, paths)
+        if not force and key in self.table:
+            return tuple(self.table[key]["schedule"])
+        import torch
+        left, right, cfg = vxsgm.mixture_pair(w, h, D, 0x5451)
+        cfg.num_paths = paths
+        dl = torch.from_numpy(left).cuda()
+        dr = torch.from_numpy(right).cuda()
+        outs = [torch.empty(w * h, dtype=torch.float32, device="cuda") for _ in range(copies)]
+        stream = torch.cuda.current_stream()
+        times = {}
+        for sched in candidates(D):
+            b = vxsgm.SGMBatch([(dl.data_ptr(), dr.data_ptr(), o.data_ptr(), w, h, cfg, sched)
+                                for o in outs])
+            b.launch(stream.cuda_stream)
+            ms = []
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                b.launch(stream.cuda_stream)
+                e1.record(stream)
+                e1.synchronize()
+                ms.append(e0.elapsed_time(e1))
+            b.close()
+            times[f"{sched[0]}x{sched[1]}"] = statistics.median(ms)
+        best = min(times, key=times.get)
+        sched = tuple(int(v) for v in best.split("x"))
+        self.table[key] = {"schedule": list(sched), "ms": times}
+        return sched
+
+    def tune_plain(self, sgm, w, h, D, paths, reps=5, force=False):
+        key = _key("plain", w, h, D, paths)
+        if not force and key in self.table:
+            return tuple(self.table[key]["schedule"])
+        import torch
+        left, right, cfg = vxsgm.mixture_pair(w, h, D, 0x5452)
+        cfg.num_paths = paths
+        dl = torch.from_numpy(left).cuda()
+        dr = torch.from_numpy(right).cuda()
+        out = torch.empty(w * h, dtype=torch.float32, device="cuda")
+        wsb = vxsgm.workspace_bytes(w, h, cfg)
+        ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+        stream = torch.cuda.current_stream()
+        times = {}
+        for sched in candidates(D):
+            sgm.set_layout(D, sched)
+            sgm.enqueue(dl.data_ptr(), dr.data_ptr(), w, h, cfg, out.data_ptr(),
+                        ws.data_ptr(), wsb, stream.cuda_stream)
+            ms = []
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                sgm.enqueue(dl.data_ptr(), dr.data_ptr(), w, h, cfg, out.data_ptr(),
+                            ws.data_ptr(), wsb, stream.cuda_stream)
+                e1.record(stream)
+                e1.synchronize()
+                ms.append(e0.elapsed_time(e1))
+            times[f"{sched[0]}x{sched[1]}"] = statistics.median(ms)
+        best = min(times, key=times.get)
+        sched = tuple(int(v) for v in best.split("x"))
+        sgm.set_layout(D, sched)
+        self.table[key] = {"schedule": list(sched), "ms": times}
+        return sched

@@ -173,6 +173,28 @@ __device__ __forceinline__ uint2 census_pixel(const uint8_t *__restrict__ img, i
     return make_uint2(lo, hi);
 }
 
+// Raw buffer resource over `bytes` bytes at `base`: loads outside
+// [0, bytes) return 0 instead of faulting, so census reads at x - d < 0 need no
+// guard (their cost is replaced by 64 afterwards).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+__device__ __forceinline__ uint2 load_census(__amdgpu_buffer_rsrc_t r, int byte_off) {
+    auto v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
+    return make_uint2(v[0], v[1]);
+}
+
+// Loads of one step: left census of the pixel, right census at x-d for the
+// lane's K disparities, and (when accumulating) the lane's volume segment.
+template <int K>
+struct StepBuf {
+    uint2 lv;
+    uint2 rv[K];
+    Seg<K> sv;
+};
+
 // One wave aggregates scanlines s_first .. s_first + 64/L - 1 of direction
 // (dx, dy) (those below p.nscan), adding into the summed volume.  Wave-level:
 // every lane of the wave must call it with the same arguments.
@@ -182,11 +204,14 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
     const int g = lane / L, sub = lane % L;
     const int s = s_first + g;
     const int w = p.w, h = p.h, D = p.D, Dp = p.Dp;
-    const uint32_t P1 = p.p1, P2 = p.p2;
+    const uint16_t P1 = (uint16_t)p.p1, P2 = (uint16_t)p.p2;
     const bool accumulate = p.accumulate != 0;
     const int d0 = sub * K;
     const bool lane_real = d0 < D;
     const bool lane_store = d0 < Dp;
+    const uint32_t npx = (uint32_t)w * (uint32_t)h;
+    const __amdgpu_buffer_rsrc_t rl = make_rsrc(p.lc, npx * 8u);
+    const __amdgpu_buffer_rsrc_t rr = make_rsrc(p.rc, npx * 8u);
 
     int sx = 0, sy = 0, len = 0;
     if (s < p.nscan) scan_start(s, dx, dy, w, h, sx, sy, len);
@@ -194,110 +219,118 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, o));
     if (maxlen == 0) return;
+    const int pstep = dy * w + dx;
+    const int pix0 = sy * w + sx;
 
-    uint32_t prev[K];
+    // o2 is 0xFFFF where d + 1 >= D (SGMStereo.metal:156).
+    uint16_t om[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) om[i] = (d0 + i + 1 < D) ? 0 : 0xFFFF;
+
+    uint16_t prev[K];
 #pragma unroll
     for (int i = 0; i < K; ++i) prev[i] = 0;
-    uint32_t pm = 0;
+    uint16_t pm = 0;
 
-    // Loads for one step, issued one step ahead.
-    uint2 lv_n = make_uint2(0u, 0u);
-    uint2 rv_n[K];
-    Seg<K> sv_n;
-    auto issue = [&](int t) {
+    auto issue = [&](StepBuf<K> &b, int t) {
         const bool a = t < len;
-        const int x = sx + t * dx, y = sy + t * dy;
-        const size_t pix = a ? (size_t)y * w + x : 0;
-        lv_n = p.lc[pix];
+        const int pix = a ? pix0 + t * pstep : 0;
+        b.lv = load_census(rl, pix * 8);
+        const int base = (pix - d0 - (K - 1)) * 8;
 #pragma unroll
-        for (int i = 0; i < K; ++i) {
-            const int d = d0 + i;
-            const size_t ri = (a && x - d >= 0) ? pix - d : pix;
-            rv_n[i] = p.rc[ri];
-        }
-        if (accumulate && a && lane_store) sv_n.load(p.vol + pix * (size_t)Dp + d0);
+        for (int i = 0; i < K; ++i) b.rv[i] = load_census(rr, base + 8 * (K - 1 - i));
+        if (accumulate && a && lane_store) b.sv.load(p.vol + (uint32_t)pix * (uint32_t)Dp + d0);
     };
-    issue(0);
 
-    for (int t = 0; t < maxlen; ++t) {
+    auto step = [&](StepBuf<K> &cur, StepBuf<K> &nxt, int t) {
         const bool act = t < len;
-        const int x = sx + t * dx, y = sy + t * dy;
-        const size_t pix = act ? (size_t)y * w + x : 0;
-        const uint2 lv = lv_n;
-        uint2 rv[K];
-#pragma unroll
-        for (int i = 0; i < K; ++i) rv[i] = rv_n[i];
-        Seg<K> sv = sv_n;
-        if (t + 1 < maxlen) issue(t + 1);
+        const int pix = act ? pix0 + t * pstep : 0;
+        const int x = sx + t * dx;
+        if (t + 1 < maxlen) issue(nxt, t + 1);
 
-        uint32_t C[K];
+        uint16_t C[K];
+        const uint2 lv = cur.lv;
+        if (__builtin_amdgcn_ballot_w64(act && x < d0 + K - 1) != 0) {
 #pragma unroll
-        for (int i = 0; i < K; ++i) {
-            const int d = d0 + i;
-            const uint32_t c = __popc(lv.x ^ rv[i].x) + __popc(lv.y ^ rv[i].y);
-            C[i] = x - d >= 0 ? c : 64u;
+            for (int i = 0; i < K; ++i) {
+                const uint32_t c = __popc(lv.x ^ cur.rv[i].x) + __popc(lv.y ^ cur.rv[i].y);
+                C[i] = (uint16_t)(x >= d0 + i ? c : 64u);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < K; ++i)
+                C[i] = (uint16_t)(__popc(lv.x ^ cur.rv[i].x) + __popc(lv.y ^ cur.rv[i].y));
         }
 
-        uint32_t agg[K];
+        uint16_t agg[K];
         if (t == 0) {
 #pragma unroll
             for (int i = 0; i < K; ++i) agg[i] = C[i];
         } else {
-            const uint32_t o3 = (pm + P2) & 0xFFFFu;
-            const uint32_t nb = from_upper_lane<L>(prev[0]);
-            uint32_t A[K], B[K];
+            const uint16_t o3 = (uint16_t)(pm + P2);
+            const uint16_t nb = (uint16_t)from_upper_lane<L>((uint32_t)prev[0]);
+            uint16_t A[K], B[K];
 #pragma unroll
             for (int i = 0; i < K; ++i) {
-                const int d = d0 + i;
-                const uint32_t nx = (i + 1 < K) ? prev[(i + 1) % K] : nb;
-                const uint32_t o2 = (d + 1 < D) ? ((nx + P1) & 0xFFFFu) : 0xFFFFu;
+                const uint16_t nx = (i + 1 < K) ? prev[(i + 1) % K] : nb;
+                const uint16_t o2 = (uint16_t)((uint16_t)(nx + P1) | om[i]);
                 A[i] = min(min(prev[i], o2), o3);
-                B[i] = (C[i] - pm) & 0xFFFFu;
+                B[i] = (uint16_t)(C[i] - pm);
             }
             // Speculative pass: no carry-in from the lane below.
             {
-                uint32_t o1 = 0xFFFFu;
+                uint16_t o1 = 0xFFFF;
 #pragma unroll
                 for (int i = 0; i < K; ++i) {
-                    agg[i] = (B[i] + min(A[i], o1)) & 0xFFFFu;
-                    o1 = (agg[i] + P1) & 0xFFFFu;
+                    agg[i] = (uint16_t)(B[i] + min(A[i], o1));
+                    o1 = (uint16_t)(agg[i] + P1);
                 }
             }
             // Fixed-point rounds over the carry-in (at most L).
             const bool live = act && lane_real && sub > 0;
-            uint32_t e_used = A[0];
+            uint16_t e_used = A[0];
             for (int r = 0; r < L; ++r) {
-                const uint32_t up = from_lower_lane<L>(agg[K - 1]);
-                const uint32_t e = min(A[0], (up + P1) & 0xFFFFu);
+                const uint16_t up = (uint16_t)from_lower_lane<L>((uint32_t)agg[K - 1]);
+                const uint16_t e = min(A[0], (uint16_t)(up + P1));
                 const bool ch = live && e != e_used;
                 if (__builtin_amdgcn_ballot_w64(ch) == 0) break;
                 if (ch) {
                     e_used = e;
-                    agg[0] = (B[0] + e) & 0xFFFFu;
-                    uint32_t o1 = (agg[0] + P1) & 0xFFFFu;
+                    agg[0] = (uint16_t)(B[0] + e);
+                    uint16_t o1 = (uint16_t)(agg[0] + P1);
 #pragma unroll
                     for (int i = 1; i < K; ++i) {
-                        agg[i] = (B[i] + min(A[i], o1)) & 0xFFFFu;
-                        o1 = (agg[i] + P1) & 0xFFFFu;
+                        agg[i] = (uint16_t)(B[i] + min(A[i], o1));
+                        o1 = (uint16_t)(agg[i] + P1);
                     }
                 }
             }
         }
 
-        uint32_t m = 0xFFFFu;
+        uint16_t m = 0xFFFF;
 #pragma unroll
-        for (int i = 0; i < K; ++i)
-            if (d0 + i < D) m = min(m, agg[i]);
-        pm = group_min<L>(m);
+        for (int i = 0; i < K; ++i) m = min(m, (uint16_t)(agg[i] | (uint16_t)(d0 + i < D ? 0 : 0xFFFF)));
+        pm = (uint16_t)group_min<L>((uint32_t)m);
 #pragma unroll
         for (int i = 0; i < K; ++i) prev[i] = agg[i];
 
         if (act && lane_store) {
-            sv.set(agg, accumulate);
-            sv.store(p.vol + pix * (size_t)Dp + d0);
+            uint32_t a32[K];
+#pragma unroll
+            for (int i = 0; i < K; ++i) a32[i] = agg[i];
+            cur.sv.set(a32, accumulate);
+            cur.sv.store(p.vol + (uint32_t)pix * (uint32_t)Dp + d0);
         }
+    };
+
+    StepBuf<K> b0, b1;
+    issue(b0, 0);
+    for (int t = 0; t < maxlen; t += 2) {
+        step(b0, b1, t);
+        if (t + 1 < maxlen) step(b1, b0, t + 1);
     }
 }
+
 
 // Winner-take-all and parabola refinement of pixel i (SGMStereo.metal:185-213).
 __device__ __forceinline__ void wta_pixel(const uint16_t *__restrict__ vol, int D, int Dp,

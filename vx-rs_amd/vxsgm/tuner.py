@@ -28,7 +28,7 @@ DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning
 
 def candidates(D):
     """Layouts that cover D without more than doubling it (padding lanes)."""
-    return [(L, K) for L, K in LAYOUTS if L * K >= D and L * K < 2 * max(D, 32)]
+    return [(L, K) for L, K in LAYOUTS if L * K >= D and L * K <= 2 * max(D, 32)]
 
 
 def _key(form, w, h, D, paths):

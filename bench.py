@@ -1,20 +1,26 @@
 """SGM stereo benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms plain,batched,served]
 
-A step is one pass of the plain SGM pipeline (census of both images, path
-aggregation over every direction, winner-take-all) over one synthetic
-rectified pair already resident in HBM.  The default workload is
-720p_d128_8path: 1280x720, D=128, 8 paths, P1=10, P2=120, census 9x7.
+Headline (`value`): the plain per-pair path.  A step is one pass of the SGM
+pipeline (census of both images, path aggregation over every direction,
+winner-take-all) over one synthetic rectified pair already resident in HBM,
+on the workload 720p_d128_8path (1280x720, D=128, 8 paths, P1=10, P2=120,
+census 9x7).  value = W*H*D*K / t / 1e9 Gpx*disp/s over K timed steps.
 
-Prints one JSON line.  `value` is Gpx*disp/s = W*H*D*K / t / 1e9 over the K
-timed steps.  `roofline` prices the per-pair pipeline against HBM: its
-algorithmic bytes are W*H*(4*D + 6) (the u16 summed volume written once and
-read once, both gray8 inputs, the f32 output) divided by the pipeline's
-average duration measured with HIP events on the launch stream.
-`cpu_baseline` times the scalar C++ oracle (the checker) on the same pair with
-all host threads and also confirms that the timed GPU output is bit-identical
-to it.
+`roofline` prices the per-pair pipeline against HBM: algorithmic bytes per
+pair are W*H*(4*D + 6) (the u16 summed volume written once and read once,
+both gray8 inputs, the f32 output), divided by the pipeline's average HIP-event
+time on the launch stream.  `cpu_baseline` times the scalar C++ oracle (the
+checker) on the same pair with the host's threads and confirms that the timed
+GPU map is bit-identical to it.
+
+`batched`: the 64-pair mixture of sizes {320x240, 640x480, 1280x720,
+1920x1080} x D {64, 128, 256}, 8 paths, as one persistent launch replayed from
+a hipGraph, against a loop of plain per-pair calls; all maps compared bit for
+bit.  `served`: Poisson arrivals at 200/500/1000 req/s of the same mixture
+against the serving loop and against a FIFO queue into the plain path;
+p50/p99 latency and Gpx*disp/s for each.
 """
 import argparse
 import ctypes
@@ -37,9 +43,8 @@ def parse():
     ap.add_argument("--workload", default="720p_d128_8path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0,
-                    help="oracle threads for the CPU baseline (0 = all host cores)")
-    ap.add_argument("--forms", default="plain,batched,served",
-                    help="comma list of forms to run after the plain headline")
+                    help="oracle threads for the CPU baseline (0 = host cores, max 16)")
+    ap.add_argument("--forms", default="plain,batched,served")
     ap.add_argument("--batch-pairs", type=int, default=64)
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--rates", default="200,500,1000")
@@ -48,57 +53,87 @@ def parse():
     return ap.parse_args()
 
 
-def batched_leg(vx, torch, np, args, sgm, tuner):
-    """64 pairs of the mixture in one persistent graph launch vs a loop of
-    plain per-pair calls on one stream; every map compared bit for bit."""
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def stage_times(vx, hip, d_left, d_right, w, h, cfg, sched, d_out, ws, stream, reps=10):
+    """Per-stage HIP-event times (ms) of the same pipeline, on the launch stream."""
+    csz = ((w * h * 8 + 255) // 256) * 256
+    lc, rc, vol = ws.ptr, ws.ptr + csz, ws.ptr + 2 * csz
+    c = cfg.to_c()
+    L, K = sched if sched else (0, 0)
+    ev = [hip.Event() for _ in range(4)]
+    acc = [0.0, 0.0, 0.0]
+    for _ in range(reps):
+        ev[0].record(stream)
+        vx.check(vx.lib.vxh_sgm_census(d_left.ptr, w, h, cfg.census_radius_x,
+                                       cfg.census_radius_y, lc, stream.handle))
+        vx.check(vx.lib.vxh_sgm_census(d_right.ptr, w, h, cfg.census_radius_x,
+                                       cfg.census_radius_y, rc, stream.handle))
+        ev[1].record(stream)
+        vx.check(vx.lib.vxh_sgm_aggregate_layout(lc, rc, w, h, ctypes.byref(c), L, K, vol,
+                                                 stream.handle))
+        ev[2].record(stream)
+        vx.check(vx.lib.vxh_sgm_wta(vol, w, h, ctypes.byref(c), d_out.ptr, stream.handle))
+        ev[3].record(stream)
+        ev[3].synchronize()
+        for i in range(3):
+            acc[i] += ev[i].elapsed_ms(ev[i + 1])
+    return {"census_x2": acc[0] / reps, "aggregate_all_paths": acc[1] / reps,
+            "wta": acc[2] / reps}
+
+
+def batched_leg(vx, hip, np, args, sgm, tuner):
+    """The 64-pair mixture in one persistent graph launch vs a loop of plain
+    per-pair calls on one stream; every map compared bit for bit."""
     mix = vx.mixture(args.batch_pairs)
-    host, dev_in, outs_b, outs_p, descs = [], [], [], [], []
+    host, bufs, descs = [], [], []
     for w, h, d, seed in mix:
         left, right, cfg = vx.mixture_pair(w, h, d, seed)
         host.append((w, h, d, cfg))
-        dl, dr = torch.from_numpy(left).cuda(), torch.from_numpy(right).cuda()
-        dev_in.append((dl, dr))
-        ob = torch.empty(w * h, dtype=torch.float32, device="cuda")
-        op = torch.empty(w * h, dtype=torch.float32, device="cuda")
-        outs_b.append(ob)
-        outs_p.append(op)
+        dl, dr = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
+        ob, op = hip.DeviceBuffer(w * h * 4), hip.DeviceBuffer(w * h * 4)
+        bufs.append((dl, dr, ob, op))
         sched = None if (args.no_tune or tuner is None) else tuner.tune_batch(w, h, d, 8)
-        descs.append((dl.data_ptr(), dr.data_ptr(), ob.data_ptr(), w, h, cfg, sched))
-    stream = torch.cuda.current_stream()
+        descs.append((dl.ptr, dr.ptr, ob.ptr, w, h, cfg, sched))
+    log(f"batched: {len(host)} pairs generated and tuned")
+    stream = hip.Stream()
     batch = vx.SGMBatch(descs)
-    batch.launch(stream.cuda_stream)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    batch.launch(stream.handle)
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
     e0.record(stream)
     for _ in range(args.batch_reps):
-        batch.launch(stream.cuda_stream)
+        batch.launch(stream.handle)
     e1.record(stream)
-    torch.cuda.synchronize()
-    ms_batch = e0.elapsed_time(e1) / args.batch_reps
-    # Plain loop: the per-pair path for each pair, one after another.
+    e1.synchronize()
+    ms_batch = e0.elapsed_ms(e1) / args.batch_reps
+    log(f"batched: {ms_batch:.3f} ms per batch")
     wsb = max(vx.workspace_bytes(w, h, cfg) for w, h, _, cfg in host)
-    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+    ws = hip.DeviceBuffer(wsb)
 
     def plain_loop():
-        for (w, h, d, cfg), (dl, dr), op in zip(host, dev_in, outs_p):
-            sgm.enqueue(dl.data_ptr(), dr.data_ptr(), w, h, cfg, op.data_ptr(),
-                        ws.data_ptr(), wsb, stream.cuda_stream)
+        for (w, h, d, cfg), (dl, dr, _, op) in zip(host, bufs):
+            sgm.enqueue(dl.ptr, dr.ptr, w, h, cfg, op.ptr, ws.ptr, wsb, stream.handle)
 
     plain_loop()
-    torch.cuda.synchronize()
+    stream.synchronize()
+    reps = max(1, args.batch_reps // 2)
     e0.record(stream)
-    for _ in range(max(1, args.batch_reps // 2)):
+    for _ in range(reps):
         plain_loop()
     e1.record(stream)
-    torch.cuda.synchronize()
-    ms_plain = e0.elapsed_time(e1) / max(1, args.batch_reps // 2)
-    exact = all(torch.equal(a.view(torch.int32), b.view(torch.int32))
-                for a, b in zip(outs_b, outs_p))
+    e1.synchronize()
+    ms_plain = e0.elapsed_ms(e1) / reps
+    log(f"batched: plain loop {ms_plain:.3f} ms")
+    exact = all(np.array_equal(ob.download(np.uint32), op.download(np.uint32))
+                for _, _, ob, op in bufs)
     pxd = sum(w * h * d for w, h, d, _ in host)
     alg = sum(w * h * (4 * d + 6) for w, h, d, _ in host)
     batch.close()
     return {
-        "pairs": len(host), "gpxd": round(pxd / 1e9, 4),
+        "pairs": len(host), "gpxd_per_batch": round(pxd / 1e9, 4),
         "ms_batch": round(ms_batch, 3), "ms_plain_loop": round(ms_plain, 3),
         "value_batch": round(pxd / (ms_batch * 1e-3) / 1e9, 3),
         "value_plain_loop": round(pxd / (ms_plain * 1e-3) / 1e9, 3),
@@ -112,18 +147,15 @@ def batched_leg(vx, torch, np, args, sgm, tuner):
 def served_leg(vx, np, args):
     """Poisson arrivals of the mixture against the serving loop and against a
     FIFO queue into the plain per-pair path (one request at a time)."""
-    import threading
-    import time as _t
     pool = []
     for w, h, d, seed in vx.mixture(32, seed=0x5347_0004):
         left, right, cfg = vx.mixture_pair(w, h, d, seed)
         pool.append((left, right, w, h, cfg))
     ref = [None] * len(pool)
-    out = {}
+    out = {"requests_per_rate": args.requests, "budget_gb": 64, "bit_exact": True}
     for name, kw in (("served", dict(max_inflight=32, num_streams=8)),
                      ("plain_fifo", dict(max_inflight=1, num_streams=1))):
         srv = vx.SGMServer(hbm_budget_bytes=64 << 30, **kw)
-        # Warm the slot pool and capture reference maps.
         for k, (l, r, w, h, cfg) in enumerate(pool):
             o = np.empty(w * h, np.float32)
             srv.wait(srv.submit(l, r, w, h, cfg, o))
@@ -137,11 +169,10 @@ def served_leg(vx, np, args):
             gaps = rng.exponential(1.0 / rate, args.requests)
             srv.reset_stats()
             outs = [None] * args.requests
-            t0 = _t.perf_counter()
-            t = t0
+            t = time.perf_counter()
             for i in range(args.requests):
                 t += gaps[i]
-                while _t.perf_counter() < t:
+                while time.perf_counter() < t:
                     pass
                 l, r, w, h, cfg = pool[i % len(pool)]
                 outs[i] = np.empty(w * h, np.float32)
@@ -151,47 +182,17 @@ def served_leg(vx, np, args):
             span = st["last_complete_s"] - st["first_submit_s"]
             exact = all(np.array_equal(o.view(np.uint32), ref[i % len(pool)].view(np.uint32))
                         for i, o in enumerate(outs))
+            out["bit_exact"] = out["bit_exact"] and exact
             rates[str(int(rate))] = {
                 "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
                 "gpxd_per_s": round(st["pxd_total"] / span / 1e9, 3),
                 "offered_gpxd_per_s": round(rate * st["pxd_total"] / st["completed"] / 1e9, 3),
                 "peak_inflight": st["peak_inflight"],
-                "peak_admitted_gb": round(st["peak_admitted_bytes"] / 2**30, 3),
-                "bit_exact": bool(exact)}
+                "peak_admitted_gb": round(st["peak_admitted_bytes"] / 2**30, 3)}
+            log(f"served[{name}] {rate:.0f} req/s: {rates[str(int(rate))]}")
         srv.close()
         out[name] = rates
-    out["requests_per_rate"] = args.requests
-    out["budget_gb"] = 64
     return out
-
-
-def stage_times(vx, torch, d_left, d_right, w, h, cfg, d_out, ws, reps=10):
-    """Per-stage HIP-event times (ms) of the same pipeline, on the launch stream."""
-    lay_census = w * h * 8
-    base = ws.data_ptr()
-    lc, rc = base, base + ((lay_census + 255) // 256) * 256
-    vol = rc + ((lay_census + 255) // 256) * 256
-    c = cfg.to_c()
-    stream = torch.cuda.current_stream()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    acc = [0.0, 0.0, 0.0]
-    for _ in range(reps):
-        ev[0].record(stream)
-        vx.check(vx.lib.vxh_sgm_census(d_left.data_ptr(), w, h, cfg.census_radius_x,
-                                       cfg.census_radius_y, lc, stream.cuda_stream))
-        vx.check(vx.lib.vxh_sgm_census(d_right.data_ptr(), w, h, cfg.census_radius_x,
-                                       cfg.census_radius_y, rc, stream.cuda_stream))
-        ev[1].record(stream)
-        vx.check(vx.lib.vxh_sgm_aggregate(lc, rc, w, h, ctypes.byref(c), vol,
-                                          stream.cuda_stream))
-        ev[2].record(stream)
-        vx.check(vx.lib.vxh_sgm_wta(vol, w, h, ctypes.byref(c), d_out.data_ptr(),
-                                    stream.cuda_stream))
-        ev[3].record(stream)
-        torch.cuda.synchronize()
-        for i in range(3):
-            acc[i] += ev[i].elapsed_time(ev[i + 1])
-    return {"census_x2": acc[0] / reps, "aggregate": acc[1] / reps, "wta": acc[2] / reps}
 
 
 def main():
@@ -202,22 +203,21 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
-    import torch
     import vxsgm as vx
+    from vxsgm import hip
 
-    if not torch.cuda.is_available():
+    if vx.device_count() < 1:
         print("bench.py needs a visible HIP device", file=sys.stderr)
         sys.exit(1)
     w, h, D, paths, _, _, _ = vx.WORKLOADS[args.workload]
     left, right, _, cfg = vx.workload_pair(args.workload)
     ctx = vx.Context(0)
     sgm = vx.SGMStereo(ctx)
-    d_left = torch.from_numpy(left).cuda()
-    d_right = torch.from_numpy(right).cuda()
-    d_out = torch.empty(w * h, dtype=torch.float32, device="cuda")
+    d_left, d_right = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
+    d_out = hip.DeviceBuffer(w * h * 4)
     ws_bytes = vx.workspace_bytes(w, h, cfg)
-    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
-    stream = torch.cuda.current_stream()
+    ws = hip.DeviceBuffer(ws_bytes)
+    stream = hip.Stream()
 
     tuner = None
     plain_sched = None
@@ -225,26 +225,28 @@ def main():
         from vxsgm.tuner import Tuner
         tuner = Tuner(path=None)
         plain_sched = tuner.tune_plain(sgm, w, h, D, paths)
+        log(f"plain schedule {plain_sched}: {tuner.table}")
 
     def step():
-        sgm.enqueue(d_left.data_ptr(), d_right.data_ptr(), w, h, cfg, d_out.data_ptr(),
-                    ws.data_ptr(), ws_bytes, stream.cuda_stream)
+        sgm.enqueue(d_left.ptr, d_right.ptr, w, h, cfg, d_out.ptr, ws.ptr, ws_bytes,
+                    stream.handle)
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
     t0 = time.perf_counter()
     e0.record(stream)
     for _ in range(args.steps):
         step()
     e1.record(stream)
-    torch.cuda.synchronize()
+    e1.synchronize()
     wall = time.perf_counter() - t0
-    ms_event = e0.elapsed_time(e1) / args.steps
+    ms_event = e0.elapsed_ms(e1) / args.steps
     ms_per_step = wall * 1e3 / args.steps
-    gpu_disp = d_out.cpu().numpy().reshape(h, w)
-    stages = stage_times(vx, torch, d_left, d_right, w, h, cfg, d_out, ws)
+    gpu_disp = d_out.download(np.float32).reshape(h, w)
+    stages = stage_times(vx, hip, d_left, d_right, w, h, cfg, plain_sched, d_out, ws, stream)
+    log(f"plain: {ms_per_step:.3f} ms/step, stages {stages}")
 
     pxd = w * h * D
     value = pxd * args.steps / wall / 1e9
@@ -265,7 +267,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "width": w, "height": h,
                    "num_disparities": D, "num_paths": paths, "p1": cfg.p1,
-                   "p2": cfg.p2, "census": "9x7", "form": "plain per-pair"},
+                   "p2": cfg.p2, "census": "9x7", "form": "plain per-pair",
+                   "schedule": list(plain_sched) if plain_sched else "default"},
         "roofline": {"bound": "hbm", "kernel": "per-pair pipeline (census x2, "
                      "aggregate per path, wta)", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -289,13 +292,13 @@ def main():
                       f"{threads} threads over independent scanlines, {cpu_s:.2f} s"}
         result["bit_exact"] = bool((gpu_disp.view(np.uint32) ==
                                     want.astype(np.float32).view(np.uint32)).all())
-    result["config"]["schedule"] = list(plain_sched) if plain_sched else "default"
+        log(f"cpu baseline {cpu_s:.2f} s, bit_exact {result['bit_exact']}")
     forms = set(args.forms.split(","))
     if "batched" in forms:
-        print(f"bench: batched leg ({args.batch_pairs} pairs)", file=sys.stderr, flush=True)
-        result["batched"] = batched_leg(vx, torch, np, args, sgm, tuner)
+        log(f"batched leg ({args.batch_pairs} pairs)")
+        result["batched"] = batched_leg(vx, hip, np, args, sgm, tuner)
     if "served" in forms:
-        print(f"bench: served leg (rates {args.rates})", file=sys.stderr, flush=True)
+        log(f"served leg (rates {args.rates})")
         result["served"] = served_leg(vx, np, args)
     print(json.dumps(result), flush=True)
 

@@ -13,48 +13,34 @@ import pytest
 import pyoracle as O
 import vxsgm
 from sgm_cases import bench_pair, checkerboard_pair, random_pair
+from vxsgm import hip
 
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
-def _torch():
-    import torch
-    assert torch.cuda.is_available(), "gpu test needs a visible device"
-    return torch
-
-
-def dev(a):
-    torch = _torch()
-    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
-
-
 def gpu_census(img, rx=4, ry=3):
-    torch = _torch()
     h, w = img.shape
-    d_img = dev(img)
-    out = torch.empty(h * w * 2, dtype=torch.int32, device="cuda")
-    vxsgm.check(vxsgm.lib.vxh_sgm_census(d_img.data_ptr(), w, h, rx, ry,
-                                         out.data_ptr(), None))
-    torch.cuda.synchronize()
-    return out.cpu().numpy().view(np.uint32).reshape(h, w, 2)
+    d_img = hip.DeviceBuffer.from_array(img)
+    out = hip.DeviceBuffer(h * w * 8)
+    vxsgm.check(vxsgm.lib.vxh_sgm_census(d_img.ptr, w, h, rx, ry, out.ptr, None))
+    hip.synchronize()
+    return out.download(np.uint32).reshape(h, w, 2)
 
 
 def gpu_volume(lc, rc, cfg, layout=(0, 0)):
-    torch = _torch()
     h, w = lc.shape[:2]
     D = cfg.num_disparities
     Dp = vxsgm.lib.vxh_sgm_volume_stride(D)
-    d_lc = dev(lc.view(np.int32))
-    d_rc = dev(rc.view(np.int32))
-    vol = torch.empty(h * w * Dp, dtype=torch.int16, device="cuda")
+    d_lc = hip.DeviceBuffer.from_array(lc)
+    d_rc = hip.DeviceBuffer.from_array(rc)
+    vol = hip.DeviceBuffer(h * w * Dp * 2)
     c = cfg.to_c()
-    vxsgm.check(vxsgm.lib.vxh_sgm_aggregate_layout(d_lc.data_ptr(), d_rc.data_ptr(), w, h,
-                                                   ctypes.byref(c), layout[0], layout[1],
-                                                   vol.data_ptr(), None))
-    torch.cuda.synchronize()
-    return vol.cpu().numpy().view(np.uint16).reshape(h, w, Dp)[:, :, :D]
+    vxsgm.check(vxsgm.lib.vxh_sgm_aggregate_layout(d_lc.ptr, d_rc.ptr, w, h, ctypes.byref(c),
+                                                   layout[0], layout[1], vol.ptr, None))
+    hip.synchronize()
+    return vol.download(np.uint16).reshape(h, w, Dp)[:, :, :D]
 
 
 def bits(a):

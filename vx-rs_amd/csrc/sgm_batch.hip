@@ -120,6 +120,7 @@ batch_kernel(const PairDesc *__restrict__ pairs, const Item *__restrict__ items,
             p.dir = it.dir;
             p.nscan = dir_nscan(it.dir, pd.w, pd.h);
             p.accumulate = it.dir > 0;
+            p.wta_out = it.count < 0 ? pd.out : nullptr;
             const int dx = dir_x(it.dir), dy = dir_y(it.dir);
             switch (pd.layout) {
             case 0: agg_wave<64, 1>(p, dx, dy, it.first); break;
@@ -281,8 +282,11 @@ vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs, uint
                 const int dir = phase - 1;
                 const int G = 64 / kBatchLayouts[d.layout][0];
                 const int ns = num_scanlines(dir, d.w, d.h);
-                for (int s = 0; s < ns; s += G) push(p, kAgg, dir, s, G, phase);
-            } else if (phase == P + 1) {
+                // Without the LR check the last path also picks the winner
+                // (count -1), so the pair needs no separate WTA phase.
+                const int cnt = (dir == P - 1 && !q.cfg.lr_check) ? -1 : G;
+                for (int s = 0; s < ns; s += G) push(p, kAgg, dir, s, cnt, phase);
+            } else if (phase == P + 1 && q.cfg.lr_check) {
                 const int npx = d.w * d.h, chunk = 64 * 64;
                 for (int i = 0; i < npx; i += chunk)
                     push(p, kWta, 0, i, std::min(chunk, npx - i), phase);

@@ -51,22 +51,34 @@ __device__ __forceinline__ uint32_t dpp_mov(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
 }
 
-// Min over the L lanes of a group, broadcast to all of them.
-template <int L>
-__device__ __forceinline__ uint32_t group_min(uint32_t v) {
-    if constexpr (L >= 2) v = min(v, dpp_mov<kQuadXor1>(v));
-    if constexpr (L >= 4) v = min(v, dpp_mov<kQuadXor2>(v));
-    if constexpr (L >= 8) v = min(v, dpp_mov<kRowHalfMirror>(v));
-    if constexpr (L >= 16) v = min(v, dpp_mov<kRowMirror>(v));
+// Reduction with `op` over the L lanes of a group, broadcast to all of them.
+// Each step pairs a lane with one partner (xor 1, xor 2, the mirrored lane in
+// 8 and in 16, then the other 16- and 32-lane halves).
+template <int L, typename Op>
+__device__ __forceinline__ uint32_t group_reduce(uint32_t v, Op op) {
+    if constexpr (L >= 2) v = op(v, dpp_mov<kQuadXor1>(v));
+    if constexpr (L >= 4) v = op(v, dpp_mov<kQuadXor2>(v));
+    if constexpr (L >= 8) v = op(v, dpp_mov<kRowHalfMirror>(v));
+    if constexpr (L >= 16) v = op(v, dpp_mov<kRowMirror>(v));
     if constexpr (L >= 32) {
         auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
-        v = min(v, min((uint32_t)p[0], (uint32_t)p[1]));
+        v = op(v, op((uint32_t)p[0], (uint32_t)p[1]));
     }
     if constexpr (L >= 64) {
         auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-        v = min(v, min((uint32_t)p[0], (uint32_t)p[1]));
+        v = op(v, op((uint32_t)p[0], (uint32_t)p[1]));
     }
     return v;
+}
+
+template <int L>
+__device__ __forceinline__ uint32_t group_min(uint32_t v) {
+    return group_reduce<L>(v, [](uint32_t a, uint32_t b) { return min(a, b); });
+}
+
+template <int L>
+__device__ __forceinline__ uint32_t group_or(uint32_t v) {
+    return group_reduce<L>(v, [](uint32_t a, uint32_t b) { return a | b; });
 }
 
 // Value held by lane sub-1 / sub+1 of the same group (lanes at the group
@@ -314,12 +326,54 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
 #pragma unroll
         for (int i = 0; i < K; ++i) prev[i] = agg[i];
 
-        if (act && lane_store) {
-            uint32_t a32[K];
+        uint32_t a32[K];
 #pragma unroll
-            for (int i = 0; i < K; ++i) a32[i] = agg[i];
+        for (int i = 0; i < K; ++i) a32[i] = agg[i];
+        if (p.wta_out == nullptr) {
+            if (act && lane_store) {
+                cur.sv.set(a32, accumulate);
+                cur.sv.store(p.vol + (uint32_t)pix * (uint32_t)Dp + d0);
+            }
+        } else {
+            // Last path: the sums are final here, so pick the winner from
+            // registers (SGMStereo.metal:190-211) instead of storing them.
             cur.sv.set(a32, accumulate);
-            cur.sv.store(p.vol + (uint32_t)pix * (uint32_t)Dp + d0);
+            uint32_t v[K];
+            if constexpr (K == 1) {
+                v[0] = cur.sv.w[0];
+            } else {
+#pragma unroll
+                for (int i = 0; i < K; ++i) v[i] = (cur.sv.w[i / 2] >> (16 * (i & 1))) & 0xFFFFu;
+            }
+            // First strict minimum: key = value << 16 | d, smallest key wins.
+            uint32_t key = 0xFFFFFFFFu;
+#pragma unroll
+            for (int i = 0; i < K; ++i)
+                if (d0 + i < D) key = min(key, (v[i] << 16) | (uint32_t)(d0 + i));
+            key = group_min<L>(key);
+            const int bd = (int)(key & 0xFFFFu);
+            const uint32_t bc = key >> 16;
+            // Neighbours bd-1 and bd+1, each held by exactly one lane.
+            uint32_t nb2 = 0;
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                if (d0 + i == bd - 1) nb2 |= v[i] << 16;
+                if (d0 + i == bd + 1) nb2 |= v[i];
+            }
+            nb2 = group_or<L>(nb2);
+            if (act && sub == 0) {
+                // All-0xFFFF slices keep d = 0 (strict compare from 0xFFFF).
+                const int bdw = bc == 0xFFFFu ? 0 : bd;
+                float disp = (float)bdw;
+                if (bdw > 0 && bdw < D - 1) {
+                    const float c0 = (float)(nb2 >> 16);
+                    const float c1 = (float)bc;
+                    const float c2 = (float)(nb2 & 0xFFFFu);
+                    const float den = 2.0f * (c0 + c2 - 2.0f * c1);
+                    if (fabsf(den) > 1e-6f) disp = disp + (c0 - c2) / den;
+                }
+                p.wta_out[pix] = disp;
+            }
         }
     };
 

@@ -42,6 +42,8 @@ struct AggParams {
     int dir;
     int nscan;
     int accumulate;        // 0: store (first path), 1: add mod 2^16
+    float *wta_out;        // non-null on the last path: winner-take-all is
+                           // computed from the final sums instead of storing them
 };
 
 hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,

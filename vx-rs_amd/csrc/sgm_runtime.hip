@@ -281,9 +281,13 @@ size_t vxh_sgm_workspace_bytes(uint32_t w, uint32_t h, const vxh_sgm_config *c) 
     return workspace_layout(w, h, c->num_disparities).total;
 }
 
+// Runs every path into `vol`.  With `wta_out`, the last path also picks the
+// winner from its final sums and writes the disparity map instead of storing
+// its sums.
 static vxh_status aggregate_impl(const uint2 *lc, const uint2 *rc, uint32_t w,
                                  uint32_t h, const vxh_sgm_config *c, uint16_t *vol,
-                                 vxh::Layout lay, hipStream_t st) {
+                                 vxh::Layout lay, hipStream_t st,
+                                 float *wta_out = nullptr) {
     const int D = (int)c->num_disparities;
     if (!vxh::layout_supported(lay, D))
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
@@ -301,6 +305,7 @@ static vxh_status aggregate_impl(const uint2 *lc, const uint2 *rc, uint32_t w,
         p.dir = (int)i;
         p.nscan = vxh::num_scanlines((int)i, (int)w, (int)h);
         p.accumulate = i > 0;
+        p.wta_out = (i + 1 == c->num_paths) ? wta_out : nullptr;
         VXH_HIP(vxh::launch_aggregate(p, lay, st), "aggregate");
     }
     return VXH_OK;
@@ -331,15 +336,16 @@ static vxh_status enqueue_impl(vxh_sgm *s, const uint8_t *d_left, const uint8_t 
             "census");
     VXH_HIP(vxh::launch_census(d_right, w, h, c->census_radius_x, c->census_radius_y, rc, st),
             "census");
+    if (!c->lr_check)
+        return aggregate_impl(lc, rc, w, h, c, vol, layout_for(s, D), st, d_out);
+    // The LR check reads sums of other pixels, so the volume is completed
+    // first and the winner taken by separate passes.
     vxh_status v2 = aggregate_impl(lc, rc, w, h, c, vol, layout_for(s, D), st);
     if (v2 != VXH_OK) return v2;
     const int Dp = vxh::volume_stride(D);
-    VXH_HIP(vxh::launch_wta(vol, w, h, D, Dp, d_out, c->lr_check ? best : nullptr, st),
-            "wta");
-    if (c->lr_check)
-        VXH_HIP(vxh::launch_lr_check(vol, w, h, D, Dp, (int)c->lr_tolerance, best, d_out,
-                                     st),
-                "lr_check");
+    VXH_HIP(vxh::launch_wta(vol, w, h, D, Dp, d_out, best, st), "wta");
+    VXH_HIP(vxh::launch_lr_check(vol, w, h, D, Dp, (int)c->lr_tolerance, best, d_out, st),
+            "lr_check");
     return VXH_OK;
 }
 

@@ -20,6 +20,7 @@ import os
 import statistics
 
 import vxsgm
+from vxsgm import hip
 
 LAYOUTS = [(64, 1), (64, 2), (64, 4), (32, 2), (32, 4), (32, 8), (16, 4), (16, 8),
            (16, 16)]
@@ -56,26 +57,23 @@ class Tuner:
         key = _key("batch", w, h, D, paths)
         if not force and key in self.table:
             return tuple(self.table[key]["schedule"])
-        import torch
         left, right, cfg = vxsgm.mixture_pair(w, h, D, 0x5451)
         cfg.num_paths = paths
-        dl = torch.from_numpy(left).cuda()
-        dr = torch.from_numpy(right).cuda()
-        outs = [torch.empty(w * h, dtype=torch.float32, device="cuda") for _ in range(copies)]
-        stream = torch.cuda.current_stream()
+        dl, dr = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
+        outs = [hip.DeviceBuffer(w * h * 4) for _ in range(copies)]
+        stream = hip.Stream()
         times = {}
         for sched in candidates(D):
-            b = vxsgm.SGMBatch([(dl.data_ptr(), dr.data_ptr(), o.data_ptr(), w, h, cfg, sched)
-                                for o in outs])
-            b.launch(stream.cuda_stream)
+            b = vxsgm.SGMBatch([(dl.ptr, dr.ptr, o.ptr, w, h, cfg, sched) for o in outs])
+            b.launch(stream.handle)
             ms = []
             for _ in range(reps):
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0, e1 = hip.Event(), hip.Event()
                 e0.record(stream)
-                b.launch(stream.cuda_stream)
+                b.launch(stream.handle)
                 e1.record(stream)
                 e1.synchronize()
-                ms.append(e0.elapsed_time(e1))
+                ms.append(e0.elapsed_ms(e1))
             b.close()
             times[f"{sched[0]}x{sched[1]}"] = statistics.median(ms)
         best = min(times, key=times.get)
@@ -87,29 +85,25 @@ class Tuner:
         key = _key("plain", w, h, D, paths)
         if not force and key in self.table:
             return tuple(self.table[key]["schedule"])
-        import torch
         left, right, cfg = vxsgm.mixture_pair(w, h, D, 0x5452)
         cfg.num_paths = paths
-        dl = torch.from_numpy(left).cuda()
-        dr = torch.from_numpy(right).cuda()
-        out = torch.empty(w * h, dtype=torch.float32, device="cuda")
+        dl, dr = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
+        out = hip.DeviceBuffer(w * h * 4)
         wsb = vxsgm.workspace_bytes(w, h, cfg)
-        ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
-        stream = torch.cuda.current_stream()
+        ws = hip.DeviceBuffer(wsb)
+        stream = hip.Stream()
         times = {}
         for sched in candidates(D):
             sgm.set_layout(D, sched)
-            sgm.enqueue(dl.data_ptr(), dr.data_ptr(), w, h, cfg, out.data_ptr(),
-                        ws.data_ptr(), wsb, stream.cuda_stream)
+            sgm.enqueue(dl.ptr, dr.ptr, w, h, cfg, out.ptr, ws.ptr, wsb, stream.handle)
             ms = []
             for _ in range(reps):
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0, e1 = hip.Event(), hip.Event()
                 e0.record(stream)
-                sgm.enqueue(dl.data_ptr(), dr.data_ptr(), w, h, cfg, out.data_ptr(),
-                            ws.data_ptr(), wsb, stream.cuda_stream)
+                sgm.enqueue(dl.ptr, dr.ptr, w, h, cfg, out.ptr, ws.ptr, wsb, stream.handle)
                 e1.record(stream)
                 e1.synchronize()
-                ms.append(e0.elapsed_time(e1))
+                ms.append(e0.elapsed_ms(e1))
             times[f"{sched[0]}x{sched[1]}"] = statistics.median(ms)
         best = min(times, key=times.get)
         sched = tuple(int(v) for v in best.split("x"))

@@ -186,6 +186,13 @@ vxh_status vxh_sgm_wta(const uint16_t *d_volume, uint32_t w, uint32_t h,
 vxh_status vxh_sgm_set_layout(vxh_sgm *sgm, uint32_t num_disparities,
                               uint32_t lanes, uint32_t per_lane);
 
+/* Plain device-to-device copy kernel (16-byte vectors, grid-stride), used to
+ * calibrate HBM traffic counters against a kernel of known traffic.  bytes
+ * must be a multiple of 16 and both pointers 16-byte aligned.  No reference
+ * counterpart. */
+vxh_status vxh_util_copy(void *d_dst, const void *d_src, size_t bytes,
+                         void *stream);
+
 /* Thread-local message describing the last failure on this thread. */
 const char *vxh_last_error(void);
 /* Library version string. */

@@ -52,8 +52,11 @@ vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs,
                                 uint32_t n, vxh_sgm_batch **out);
 /* Replays the graph on `stream` (NULL = the batch's own stream). */
 vxh_status vxh_sgm_batch_launch(vxh_sgm_batch *batch, void *stream);
-/* Launch on the batch's own stream and wait. */
+/* Launch on the batch's own stream, wait, and check the error flag. */
 vxh_status vxh_sgm_batch_run(vxh_sgm_batch *batch);
+/* After a launch has completed: VXH_ERR_GPU if any item's bounded dependency
+ * wait expired (the batch's maps are then incomplete), else VXH_OK. */
+vxh_status vxh_sgm_batch_check(vxh_sgm_batch *batch);
 vxh_status vxh_sgm_batch_stats_get(const vxh_sgm_batch *batch, uint32_t pair,
                                    vxh_sgm_batch_stats *out);
 uint32_t vxh_sgm_batch_items(const vxh_sgm_batch *batch);

@@ -77,27 +77,47 @@ __global__ void batch_reset_kernel(uint32_t *counters, int n) {
     if (i < n) counters[i] = 0;
 }
 
+// Every item-level step below is wave-uniform: all 64 lanes issue the
+// counter atomics (only lane 0 adds 1) and all poll the same counter, so the
+// item loop has a single, uniform exit.  (Lane-0-only branches around these
+// steps let the compiler build a loop in which only some lanes leave after an
+// item while the rest re-enter with a stale index.)
+__device__ __forceinline__ uint32_t wave_take(uint32_t *ctr, int lane) {
+    const uint32_t v = __hip_atomic_fetch_add(ctr, lane == 0 ? 1u : 0u, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// Waits until *ctr >= target.  Bounded: after kSpinLimit polls (seconds) the
+// wave raises *error and returns, so a planning mistake ends the kernel with
+// an error instead of hanging the device.
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ bool wave_wait(const uint32_t *ctr, uint32_t target,
+                                          uint32_t *error) {
+    for (uint32_t n = 0; n < kSpinLimit; ++n) {
+        const uint32_t v = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((uint32_t)__builtin_amdgcn_readfirstlane((int)v) >= target) return true;
+        const uint32_t e = __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__builtin_amdgcn_readfirstlane((int)e) != 0) return false;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __hip_atomic_store(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return false;
+}
+
 __global__ void __launch_bounds__(256)
 batch_kernel(const PairDesc *__restrict__ pairs, const Item *__restrict__ items,
              int nitems, uint32_t *next, uint32_t *done,
-             const uint32_t *__restrict__ need) {
+             const uint32_t *__restrict__ need, uint32_t *error) {
     const int lane = threadIdx.x & 63;
-    while (true) {
-        uint32_t idx = 0;
-        if (lane == 0) idx = __hip_atomic_fetch_add(next, 1u, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
-        idx = (uint32_t)__builtin_amdgcn_readfirstlane((int)idx);
-        if (idx >= (uint32_t)nitems) break;
+    for (uint32_t idx = wave_take(next, lane); idx < (uint32_t)nitems;
+         idx = wave_take(next, lane)) {
         const Item it = items[idx];
         const PairDesc pd = pairs[it.pair];
         if (it.phase > 0) {
             const int c = it.pair * kMaxPhase + it.phase - 1;
-            const uint32_t target = need[c];
-            if (lane == 0) {
-                while (__hip_atomic_load(done + c, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT) < target)
-                    __builtin_amdgcn_s_sleep(2);
-            }
+            if (!wave_wait(done + c, need[c], error)) continue;
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         if (it.kind == kCensusL || it.kind == kCensusR) {
@@ -141,9 +161,8 @@ batch_kernel(const PairDesc *__restrict__ pairs, const Item *__restrict__ items,
                 lr_pixel(pd.vol, pd.w, pd.D, pd.Dp, it.dir, (size_t)i, pd.best, pd.out);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (lane == 0)
-            __hip_atomic_fetch_add(done + it.pair * kMaxPhase + it.phase, 1u,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(done + it.pair * kMaxPhase + it.phase, lane == 0 ? 1u : 0u,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -168,7 +187,7 @@ struct vxh_sgm_batch {
     vxh::PairDesc *d_pairs = nullptr;
     vxh::Item *d_items = nullptr;
     uint32_t *d_need = nullptr;
-    uint32_t *d_counters = nullptr;   // [0] = next, [1..] = done
+    uint32_t *d_counters = nullptr;   // [0] = next, [1] = error, [2..] = done
     int ncounters = 0;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -305,7 +324,7 @@ vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs, uint
         b->stats[i].per_lane = (uint32_t)kBatchLayouts[pd[i].layout][1];
     }
     b->nitems = (int)items.size();
-    b->ncounters = 1 + (int)n * kMaxPhase;
+    b->ncounters = 2 + (int)n * kMaxPhase;
     const size_t meta = align256(n * sizeof(PairDesc)) + align256(items.size() * sizeof(Item)) +
                         align256(need.size() * 4) + align256((size_t)b->ncounters * 4);
     if (hipMalloc(&b->meta, meta) != hipSuccess) {
@@ -342,7 +361,8 @@ vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs, uint
         hipLaunchKernelGGL(batch_reset_kernel, dim3((b->ncounters + 255) / 256), dim3(256), 0,
                            b->stream, b->d_counters, b->ncounters);
         hipLaunchKernelGGL(batch_kernel, dim3(b->grid), dim3(256), 0, b->stream, b->d_pairs,
-                           b->d_items, b->nitems, b->d_counters, b->d_counters + 1, b->d_need);
+                           b->d_items, b->nitems, b->d_counters, b->d_counters + 2, b->d_need,
+                           b->d_counters + 1);
         e = hipStreamEndCapture(b->stream, &b->graph);
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
@@ -369,6 +389,17 @@ vxh_status vxh_sgm_batch_run(vxh_sgm_batch *b) {
     hipError_t e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess)
         return bfail(VXH_ERR_GPU, std::string("GPU error: batch sync: ") + hipGetErrorString(e));
+    return vxh_sgm_batch_check(b);
+}
+
+vxh_status vxh_sgm_batch_check(vxh_sgm_batch *b) {
+    if (!b) return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: null batch");
+    uint32_t err = 0;
+    hipError_t e = hipMemcpy(&err, b->d_counters + 1, 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+        return bfail(VXH_ERR_GPU, std::string("GPU error: batch check: ") + hipGetErrorString(e));
+    if (err)
+        return bfail(VXH_ERR_GPU, "GPU error: a batch item's dependency wait timed out");
     return VXH_OK;
 }
 

@@ -52,6 +52,7 @@ hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
 Layout default_layout(int D);
 bool layout_supported(Layout l, int D);
 hipError_t launch_aggregate(const AggParams &p, Layout l, hipStream_t s);
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_wta(const uint16_t *vol, int w, int h, int D, int Dp,
                       float *out, int16_t *best, hipStream_t s);
 hipError_t launch_lr_check(const uint16_t *vol, int w, int h, int D, int Dp,

@@ -39,6 +39,13 @@ lr_kernel(const uint16_t *__restrict__ vol, int w, int h, int D, int Dp, int tol
     lr_pixel(vol, w, D, Dp, tol, i, best, out);
 }
 
+__global__ void __launch_bounds__(256)
+copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 template <int L, int K>
 hipError_t launch_agg_lk(const AggParams &p, hipStream_t s) {
     constexpr int G = 64 / L;
@@ -87,6 +94,14 @@ hipError_t launch_aggregate(const AggParams &p, Layout l, hipStream_t s) {
     VXH_AGG_CASE(16, 4) VXH_AGG_CASE(16, 8) VXH_AGG_CASE(16, 16)
 #undef VXH_AGG_CASE
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / 16;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(copy_kernel, dim3(grid ? grid : 1), dim3(256), 0, s, (uint4 *)dst,
+                       (const uint4 *)src, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_wta(const uint16_t *vol, int w, int h, int D, int Dp, float *out,

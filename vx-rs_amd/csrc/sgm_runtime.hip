@@ -489,6 +489,13 @@ vxh_status vxh_sgm_set_layout(vxh_sgm *s, uint32_t nd, uint32_t lanes, uint32_t 
     return VXH_OK;
 }
 
+vxh_status vxh_util_copy(void *d_dst, const void *d_src, size_t bytes, void *stream) {
+    if (!d_dst || !d_src || bytes % 16 || ((uintptr_t)d_dst | (uintptr_t)d_src) % 16)
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: copy needs 16-byte multiples");
+    VXH_HIP(vxh::launch_copy(d_dst, d_src, bytes, (hipStream_t)stream), "copy");
+    return VXH_OK;
+}
+
 uint32_t vxh_sgm_volume_stride(uint32_t nd) { return (uint32_t)vxh::volume_stride((int)nd); }
 
 vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,

@@ -91,6 +91,7 @@ _sig("vxh_sgm_aggregate", ctypes.c_int, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp)
 _sig("vxh_sgm_wta", ctypes.c_int, _vp, _u32, _u32, _cfgp, _vp, _vp)
 _sig("vxh_sgm_volume_stride", _u32, _u32)
 _sig("vxh_sgm_set_layout", ctypes.c_int, _vp, _u32, _u32, _u32)
+_sig("vxh_util_copy", ctypes.c_int, _vp, _vp, _sz, _vp)
 _sig("vxh_sgm_aggregate_layout", ctypes.c_int, _vp, _vp, _u32, _u32, _cfgp, _u32, _u32,
      _vp, _vp)
 _sig("vxh_synth_pair", ctypes.c_int, _u32, _u32, _u32, _u32, _u32,

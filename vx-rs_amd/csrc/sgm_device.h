@@ -247,18 +247,29 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
     auto issue = [&](StepBuf<K> &b, int t) {
         const bool a = t < len;
         const int pix = a ? pix0 + t * pstep : 0;
+        const int x = sx + t * dx;
         b.lv = load_census(rl, pix * 8);
-        const int base = (pix - d0 - (K - 1)) * 8;
+        if (__builtin_amdgcn_ballot_w64(a && x < d0 + K - 1) == 0) {
+            // Every element x - d of every active lane is in the row: one base
+            // and constant offsets (the compiler may merge these loads into
+            // wider ones, which is only safe when all of them are in range).
+            const int base = (pix - d0 - (K - 1)) * 8;
 #pragma unroll
-        for (int i = 0; i < K; ++i) b.rv[i] = load_census(rr, base + 8 * (K - 1 - i));
+            for (int i = 0; i < K; ++i) b.rv[i] = load_census(rr, base + 8 * (K - 1 - i));
+        } else {
+            // Left edge: each element gets its own in-range offset; those with
+            // x - d < 0 read the pixel itself and are replaced by 64 later.
+#pragma unroll
+            for (int i = 0; i < K; ++i)
+                b.rv[i] = load_census(rr, (x >= d0 + i ? pix - d0 - i : pix) * 8);
+        }
         if (accumulate && a && lane_store) b.sv.load(p.vol + (uint32_t)pix * (uint32_t)Dp + d0);
     };
 
-    auto step = [&](StepBuf<K> &cur, StepBuf<K> &nxt, int t) {
+    auto step = [&](StepBuf<K> &cur, int t) {
         const bool act = t < len;
         const int pix = act ? pix0 + t * pstep : 0;
         const int x = sx + t * dx;
-        if (t + 1 < maxlen) issue(nxt, t + 1);
 
         uint16_t C[K];
         const uint2 lv = cur.lv;
@@ -377,11 +388,22 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
         }
     };
 
-    StepBuf<K> b0, b1;
-    issue(b0, 0);
-    for (int t = 0; t < maxlen; t += 2) {
-        step(b0, b1, t);
-        if (t + 1 < maxlen) step(b1, b0, t + 1);
+    // Loads run PF steps ahead through a ring of PF + 1 buffers; the loop is
+    // unrolled by the ring size so every buffer stays in fixed registers.
+    constexpr int PF = K <= 4 ? 3 : (K <= 8 ? 2 : 1);
+    constexpr int NB = PF + 1;
+    StepBuf<K> buf[NB];
+#pragma unroll
+    for (int j = 0; j < PF; ++j)
+        if (j < maxlen) issue(buf[j], j);
+    for (int t = 0; t < maxlen; t += NB) {
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            if (t + u < maxlen) {
+                if (t + u + PF < maxlen) issue(buf[(u + PF) % NB], t + u + PF);
+                step(buf[u], t + u);
+            }
+        }
     }
 }
 
@@ -393,11 +415,18 @@ __device__ __forceinline__ void wta_pixel(const uint16_t *__restrict__ vol, int 
     const uint16_t *v = vol + i * Dp;
     uint32_t bc = 0xFFFFu;
     int bd = 0;
-    for (int d = 0; d < D; ++d) {
-        const uint32_t c = v[d];
-        if (c < bc) {
-            bc = c;
-            bd = d;
+    // The slice is 32-byte aligned (Dp is a multiple of 16): read 8 entries
+    // per 16-byte load, keeping the first strict minimum.
+    for (int d8 = 0; d8 < D; d8 += 8) {
+        const uint4 q = *(const uint4 *)(v + d8);
+        const uint32_t wv[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t c = (wv[j / 2] >> (16 * (j & 1))) & 0xFFFFu;
+            if (d8 + j < D && c < bc) {
+                bc = c;
+                bd = d8 + j;
+            }
         }
     }
     float disp = (float)bd;

@@ -50,6 +50,11 @@ def parse():
     ap.add_argument("--rates", default="200,500,1000")
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB MiB five times (the "
+                         "known-traffic reference for HBM byte counters) and exit")
+    ap.add_argument("--groups", type=int, default=0,
+                    help="concurrent path groups for the plain path (0 = tuned or default)")
     return ap.parse_args()
 
 
@@ -84,7 +89,7 @@ def stage_times(vx, hip, d_left, d_right, w, h, cfg, sched, d_out, ws, stream, r
             "wta": acc[2] / reps}
 
 
-def batched_leg(vx, hip, np, args, sgm, tuner):
+def batched_leg(vx, hip, np, args, sgm, tuner, plain_groups):
     """The 64-pair mixture in one persistent graph launch vs a loop of plain
     per-pair calls on one stream; every map compared bit for bit."""
     mix = vx.mixture(args.batch_pairs)
@@ -110,7 +115,10 @@ def batched_leg(vx, hip, np, args, sgm, tuner):
     e1.synchronize()
     ms_batch = e0.elapsed_ms(e1) / args.batch_reps
     log(f"batched: {ms_batch:.3f} ms per batch")
-    wsb = max(vx.workspace_bytes(w, h, cfg) for w, h, _, cfg in host)
+    # The plain loop runs with the path groups the plain path was tuned to.
+    for d in sorted({d for _, _, d, _ in host}):
+        sgm.set_groups(d, plain_groups)
+    wsb = max(vx.workspace_bytes(w, h, cfg, groups=plain_groups) for w, h, _, cfg in host)
     ws = hip.DeviceBuffer(wsb)
 
     def plain_loop():
@@ -209,23 +217,37 @@ def main():
     if vx.device_count() < 1:
         print("bench.py needs a visible HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        n = args.copy_calibration << 20
+        src, dst = hip.DeviceBuffer(n), hip.DeviceBuffer(n)
+        src.fill(1)
+        for _ in range(5):
+            vx.check(vx.lib.vxh_util_copy(dst.ptr, src.ptr, n, None))
+        hip.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n, "launches": 5,
+                          "read_bytes_per_launch": n, "write_bytes_per_launch": n}))
+        return
     w, h, D, paths, _, _, _ = vx.WORKLOADS[args.workload]
     left, right, _, cfg = vx.workload_pair(args.workload)
     ctx = vx.Context(0)
     sgm = vx.SGMStereo(ctx)
     d_left, d_right = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
     d_out = hip.DeviceBuffer(w * h * 4)
-    ws_bytes = vx.workspace_bytes(w, h, cfg)
+    # Room for eight path-group volumes; the handle uses as many as it is set to.
+    ws_bytes = vx.workspace_bytes(w, h, cfg, groups=8)
     ws = hip.DeviceBuffer(ws_bytes)
     stream = hip.Stream()
 
     tuner = None
-    plain_sched = None
+    plain_sched, plain_groups = None, 1
+    if args.groups:
+        sgm.set_groups(D, args.groups)
+        plain_groups = args.groups
     if not args.no_tune:
         from vxsgm.tuner import Tuner
         tuner = Tuner(path=None)
-        plain_sched = tuner.tune_plain(sgm, w, h, D, paths)
-        log(f"plain schedule {plain_sched}: {tuner.table}")
+        plain_sched, plain_groups = tuner.tune_plain(sgm, w, h, D, paths)
+        log(f"plain schedule {plain_sched} groups {plain_groups}: {tuner.table}")
 
     def step():
         sgm.enqueue(d_left.ptr, d_right.ptr, w, h, cfg, d_out.ptr, ws.ptr, ws_bytes,
@@ -268,7 +290,8 @@ def main():
         "config": {"workload": args.workload, "width": w, "height": h,
                    "num_disparities": D, "num_paths": paths, "p1": cfg.p1,
                    "p2": cfg.p2, "census": "9x7", "form": "plain per-pair",
-                   "schedule": list(plain_sched) if plain_sched else "default"},
+                   "schedule": list(plain_sched) if plain_sched else "default",
+                   "path_groups": plain_groups},
         "roofline": {"bound": "hbm", "kernel": "per-pair pipeline (census x2, "
                      "aggregate per path, wta)", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -296,7 +319,7 @@ def main():
     forms = set(args.forms.split(","))
     if "batched" in forms:
         log(f"batched leg ({args.batch_pairs} pairs)")
-        result["batched"] = batched_leg(vx, hip, np, args, sgm, tuner)
+        result["batched"] = batched_leg(vx, hip, np, args, sgm, tuner, plain_groups)
     if "served" in forms:
         log(f"served leg (rates {args.rates})")
         result["served"] = served_leg(vx, np, args)

@@ -146,9 +146,15 @@ vxh_status vxh_sgm_compute_host(vxh_sgm *sgm, const uint8_t *left,
 /* Scratch bytes one device-resident call needs for a w x h pair. */
 size_t vxh_sgm_workspace_bytes(uint32_t w, uint32_t h,
                                const vxh_sgm_config *cfg);
+/* Scratch bytes for a call that runs its paths in `groups` concurrent path
+ * groups (1, 2, 4 or 8; one volume each); 0 for other values. */
+size_t vxh_sgm_workspace_bytes_groups(uint32_t w, uint32_t h,
+                                      const vxh_sgm_config *cfg, uint32_t groups);
 /* Enqueues the whole pipeline on `stream` (a hipStream_t; NULL = the null
  * stream) with caller-owned device buffers: left/right w*h bytes, out w*h
- * floats, workspace >= vxh_sgm_workspace_bytes.  Does not synchronize. */
+ * floats, workspace >= vxh_sgm_workspace_bytes.  Does not synchronize.  The
+ * path groups used are the handle's setting (vxh_sgm_set_groups) reduced
+ * until their volumes fit `workspace_bytes`. */
 vxh_status vxh_sgm_enqueue(vxh_sgm *sgm, const uint8_t *d_left,
                            const uint8_t *d_right, uint32_t w, uint32_t h,
                            const vxh_sgm_config *cfg, float *d_out,
@@ -185,6 +191,17 @@ vxh_status vxh_sgm_wta(const uint16_t *d_volume, uint32_t w, uint32_t h,
  * depend on the layout, only the speed does. */
 vxh_status vxh_sgm_set_layout(vxh_sgm *sgm, uint32_t num_disparities,
                               uint32_t lanes, uint32_t per_lane);
+
+/* Selects how many direction passes (1, 2, 4 or 8; 0 = the default, 1) this
+ * handle runs concurrently for pairs with exactly `num_disparities`
+ * disparities.  With g > 1 the paths run as num_paths / g launches of g
+ * passes each, pass i accumulating into volume i % g, and a summing
+ * winner-take-all adds the g volumes.  The result is bit-identical for every
+ * setting (sums are mod 2^16); the workspace grows by one volume per extra
+ * group (vxh_sgm_workspace_bytes_groups).  No reference counterpart: the
+ * reference encodes its paths one after another. */
+vxh_status vxh_sgm_set_groups(vxh_sgm *sgm, uint32_t num_disparities,
+                              uint32_t groups);
 
 /* Plain device-to-device copy kernel (16-byte vectors, grid-stride), used to
  * calibrate HBM traffic counters against a kernel of known traffic.  bytes

@@ -213,19 +213,113 @@ def test_concurrent_calls_match_serial(sgm):
         assert_same_disp(a, b)
 
 
-def test_720p_d128_8path_bit_exact(sgm):
-    """Workload 720p_d128_8path at full size against the threaded oracle."""
-    left, right, _, cfg = vxsgm.workload_pair("720p_d128_8path")
-    got = sgm.compute_host(left, right, 1280, 720, cfg)
-    want = O.sgm(left, right, 128, num_paths=8, nthreads=16)
+@pytest.mark.parametrize("groups", [2, 4, 8])
+@pytest.mark.parametrize("paths", [2, 4, 8])
+@pytest.mark.parametrize("lr", [False, True])
+def test_path_groups_match_oracle(ctx, groups, paths, lr):
+    """Paths run as concurrent passes (one volume each, summed by the
+    winner-take-all) give the same map as the oracle, with and without the LR
+    check."""
+    D = 40
+    left, right, _ = vxsgm.synth_pair(150, 70, D, 5, 1, 700 + 10 * groups + paths)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=paths, lr_check=lr)
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        s.set_groups(D, groups)
+        got = s.compute_host(left, right, 150, 70, cfg)
+    finally:
+        s.close()
+    want = O.sgm(left, right, D, num_paths=paths, lr_check=lr)
     assert_same_disp(got, want)
 
 
-def test_1080p_d256_8path_matches_golden_hash(sgm):
+@pytest.mark.parametrize("D,w,h", [(1, 9, 5), (7, 40, 9), (130, 150, 12), (200, 230, 20),
+                                   (256, 260, 9)])
+def test_path_groups_disparity_ranges(ctx, D, w, h):
+    """The summing winner-take-all at slice widths below one chunk, straddling
+    two chunks and at the maximum D, for every group count."""
+    left, right, _ = vxsgm.synth_pair(w, h, D, 3, 1, 900 + D)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=8)
+    want = O.sgm(left, right, D, num_paths=8)
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        for g in (1, 2, 4, 8):
+            s.set_groups(D, g)
+            assert_same_disp(s.compute_host(left, right, w, h, cfg), want)
+    finally:
+        s.close()
+
+
+def test_path_groups_enqueue_workspace(ctx):
+    """enqueue uses as many groups as the workspace holds: a one-volume
+    workspace runs ungrouped, a four-volume one grouped; both match."""
+    w, h, D = 128, 64, 48
+    left, right, _ = vxsgm.synth_pair(w, h, D, 4, 1, 811)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=8)
+    want = O.sgm(left, right, D, num_paths=8)
+    ws1 = vxsgm.workspace_bytes(w, h, cfg)
+    ws4 = vxsgm.workspace_bytes(w, h, cfg, groups=4)
+    assert ws4 > ws1
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        s.set_groups(D, 4)
+        d_l = hip.DeviceBuffer.from_array(left)
+        d_r = hip.DeviceBuffer.from_array(right)
+        for ws in (ws1, ws4):
+            out = hip.DeviceBuffer(w * h * 4)
+            out.fill(0xFF)
+            d_ws = hip.DeviceBuffer(ws)
+            st = hip.Stream()
+            s.enqueue(d_l.ptr, d_r.ptr, w, h, cfg, out.ptr, d_ws.ptr, ws, st.handle)
+            st.synchronize()
+            st.close()
+            assert_same_disp(out.download(np.float32), want)
+    finally:
+        s.close()
+
+
+def test_path_groups_setting_validated(ctx):
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        with pytest.raises(vxsgm.InvalidConfig):
+            s.set_groups(64, 3)
+        with pytest.raises(vxsgm.InvalidConfig):
+            s.set_groups(64, 16)
+        with pytest.raises(vxsgm.InvalidConfig):
+            s.set_groups(0, 2)
+        s.set_groups(64, 0)
+    finally:
+        s.close()
+
+
+def test_720p_d128_8path_bit_exact(ctx, sgm):
+    """Workload 720p_d128_8path at full size against the threaded oracle, with
+    the paths run one after another and as concurrent passes."""
+    left, right, _, cfg = vxsgm.workload_pair("720p_d128_8path")
+    want = O.sgm(left, right, 128, num_paths=8, nthreads=16)
+    got = sgm.compute_host(left, right, 1280, 720, cfg)
+    assert_same_disp(got, want)
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        for g in (2, 4, 8):
+            s.set_groups(128, g)
+            assert_same_disp(s.compute_host(left, right, 1280, 720, cfg), want)
+    finally:
+        s.close()
+
+
+def test_1080p_d256_8path_matches_golden_hash(ctx, sgm):
     """Workload 1080p_d256_8path: the GPU map's SHA-256 equals the oracle's,
     recorded in tests/golden/workload_hashes.json."""
     with open(os.path.join(GOLDEN, "workload_hashes.json")) as f:
         golden = json.load(f)
     left, right, _, cfg = vxsgm.workload_pair("1080p_d256_8path")
     got = sgm.compute_host(left, right, 1920, 1080, cfg)
+    assert hashlib.sha256(bits(got).tobytes()).hexdigest() == golden["1080p_d256_8path"]
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        s.set_groups(256, 8)
+        got = s.compute_host(left, right, 1920, 1080, cfg)
+    finally:
+        s.close()
     assert hashlib.sha256(bits(got).tobytes()).hexdigest() == golden["1080p_d256_8path"]

@@ -185,17 +185,24 @@ __device__ __forceinline__ uint2 census_pixel(const uint8_t *__restrict__ img, i
     return make_uint2(lo, hi);
 }
 
-// Raw buffer resource over `bytes` bytes at `base`: loads outside
-// [0, bytes) return 0 instead of faulting, so census reads at x - d < 0 need no
-// guard (their cost is replaced by 64 afterwards).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
-                                             0x00020000);
+// Target of the stores of lanes with nothing to store (past the end of their
+// scanline, or padding lanes past the volume stride).  Every memory operation
+// of the scan loop is issued unconditionally, so each step issues the same
+// number of them on every path and the compiler can wait for exactly the
+// loads a step consumes instead of draining the prefetch ring.  32 bytes per
+// lane (the widest segment store).
+static __device__ uint4 g_store_sink[64 * 2];
+
+template <typename T>
+__device__ __forceinline__ T *store_sink() {
+    return (T *)&g_store_sink[(threadIdx.x & 63) * 2];
 }
 
-__device__ __forceinline__ uint2 load_census(__amdgpu_buffer_rsrc_t r, int byte_off) {
-    auto v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
-    return make_uint2(v[0], v[1]);
+// Census word of pixel `i` (every caller passes an index inside the image).
+// Plain global loads, not buffer loads: the compiler counts these exactly when
+// placing its waits, so a step waits only for its own prefetched loads.
+__device__ __forceinline__ uint2 load_census(const uint2 *__restrict__ c, int i) {
+    return c[i];
 }
 
 // Loads of one step: left census of the pixel, right census at x-d for the
@@ -221,9 +228,9 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
     const int d0 = sub * K;
     const bool lane_real = d0 < D;
     const bool lane_store = d0 < Dp;
-    const uint32_t npx = (uint32_t)w * (uint32_t)h;
-    const __amdgpu_buffer_rsrc_t rl = make_rsrc(p.lc, npx * 8u);
-    const __amdgpu_buffer_rsrc_t rr = make_rsrc(p.rc, npx * 8u);
+    const int npx = w * h;
+    const uint2 *__restrict__ lc = p.lc;
+    const uint2 *__restrict__ rc = p.rc;
 
     int sx = 0, sy = 0, len = 0;
     if (s < p.nscan) scan_start(s, dx, dy, w, h, sx, sy, len);
@@ -233,6 +240,10 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
     if (maxlen == 0) return;
     const int pstep = dy * w + dx;
     const int pix0 = sy * w + sx;
+    // Volume element of this lane at step t: voff0 + t * vstep, kept in 32-bit
+    // arithmetic (mod 2^32; the volume has fewer than 2^32 entries).
+    const uint32_t voff0 = (uint32_t)pix0 * (uint32_t)Dp + (uint32_t)d0;
+    const uint32_t vstep = (uint32_t)pstep * (uint32_t)Dp;
 
     // o2 is 0xFFFF where d + 1 >= D (SGMStereo.metal:156).
     uint16_t om[K];
@@ -248,22 +259,25 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
         const bool a = t < len;
         const int pix = a ? pix0 + t * pstep : 0;
         const int x = sx + t * dx;
-        b.lv = load_census(rl, pix * 8);
-        if (__builtin_amdgcn_ballot_w64(a && x < d0 + K - 1) == 0) {
+        b.lv = load_census(lc, pix);
+        if (__builtin_amdgcn_ballot_w64(a && x < d0 + K - 1) == 0 && npx >= K) {
             // Every element x - d of every active lane is in the row: one base
             // and constant offsets (the compiler may merge these loads into
-            // wider ones, which is only safe when all of them are in range).
-            const int base = (pix - d0 - (K - 1)) * 8;
+            // wider ones).  Inactive lanes read pixels 0 .. K-1.
+            const int base = a ? pix - d0 - (K - 1) : 0;
 #pragma unroll
-            for (int i = 0; i < K; ++i) b.rv[i] = load_census(rr, base + 8 * (K - 1 - i));
+            for (int i = 0; i < K; ++i) b.rv[i] = load_census(rc, base + (K - 1 - i));
         } else {
-            // Left edge: each element gets its own in-range offset; those with
+            // Left edge: each element gets its own in-range index; those with
             // x - d < 0 read the pixel itself and are replaced by 64 later.
 #pragma unroll
             for (int i = 0; i < K; ++i)
-                b.rv[i] = load_census(rr, (x >= d0 + i ? pix - d0 - i : pix) * 8);
+                b.rv[i] = load_census(rc, (a && x >= d0 + i) ? pix - d0 - i : pix);
         }
-        if (accumulate && a && lane_store) b.sv.load(p.vol + (uint32_t)pix * (uint32_t)Dp + d0);
+        // Lanes without a volume entry here read entry 0 (always in range)
+        // and discard it.
+        const uint32_t off = (a && lane_store) ? voff0 + (uint32_t)t * vstep : 0u;
+        if (accumulate) b.sv.load(p.vol + off);
     };
 
     auto step = [&](StepBuf<K> &cur, int t) {
@@ -341,10 +355,10 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
 #pragma unroll
         for (int i = 0; i < K; ++i) a32[i] = agg[i];
         if (p.wta_out == nullptr) {
-            if (act && lane_store) {
-                cur.sv.set(a32, accumulate);
-                cur.sv.store(p.vol + (uint32_t)pix * (uint32_t)Dp + d0);
-            }
+            cur.sv.set(a32, accumulate);
+            uint16_t *dst = (act && lane_store) ? p.vol + (voff0 + (uint32_t)t * vstep)
+                                                : store_sink<uint16_t>();
+            cur.sv.store(dst);
         } else {
             // Last path: the sums are final here, so pick the winner from
             // registers (SGMStereo.metal:190-211) instead of storing them.
@@ -372,37 +386,39 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
                 if (d0 + i == bd + 1) nb2 |= v[i];
             }
             nb2 = group_or<L>(nb2);
-            if (act && sub == 0) {
-                // All-0xFFFF slices keep d = 0 (strict compare from 0xFFFF).
-                const int bdw = bc == 0xFFFFu ? 0 : bd;
-                float disp = (float)bdw;
-                if (bdw > 0 && bdw < D - 1) {
-                    const float c0 = (float)(nb2 >> 16);
-                    const float c1 = (float)bc;
-                    const float c2 = (float)(nb2 & 0xFFFFu);
-                    const float den = 2.0f * (c0 + c2 - 2.0f * c1);
-                    if (fabsf(den) > 1e-6f) disp = disp + (c0 - c2) / den;
-                }
-                p.wta_out[pix] = disp;
+            // All-0xFFFF slices keep d = 0 (strict compare from 0xFFFF).
+            const int bdw = bc == 0xFFFFu ? 0 : bd;
+            float disp = (float)bdw;
+            if (bdw > 0 && bdw < D - 1) {
+                const float c0 = (float)(nb2 >> 16);
+                const float c1 = (float)bc;
+                const float c2 = (float)(nb2 & 0xFFFFu);
+                const float den = 2.0f * (c0 + c2 - 2.0f * c1);
+                if (fabsf(den) > 1e-6f) disp = disp + (c0 - c2) / den;
             }
+            float *dst = (act && sub == 0) ? p.wta_out + pix : store_sink<float>();
+            *dst = disp;
         }
     };
 
     // Loads run PF steps ahead through a ring of PF + 1 buffers; the loop is
-    // unrolled by the ring size so every buffer stays in fixed registers.
-    constexpr int PF = K <= 4 ? 3 : (K <= 8 ? 2 : 1);
+    // unrolled by the ring size so every buffer stays in fixed registers, and
+    // runs a whole number of rounds (the steps past maxlen are inactive on
+    // every lane: their loads read in-range entries, their stores go to the
+    // sink).  With no branch around any load or store, every step issues the
+    // same memory operations and waits only for its own PF-old loads.
+    constexpr int PF = K <= 2 ? 8 : (K <= 4 ? 6 : (K <= 8 ? 3 : 1));
     constexpr int NB = PF + 1;
+    const int rounds = __builtin_amdgcn_readfirstlane((maxlen + NB - 1) / NB);
     StepBuf<K> buf[NB];
 #pragma unroll
-    for (int j = 0; j < PF; ++j)
-        if (j < maxlen) issue(buf[j], j);
-    for (int t = 0; t < maxlen; t += NB) {
+    for (int j = 0; j < PF; ++j) issue(buf[j], j);
+    for (int r = 0; r < rounds; ++r) {
+        const int t = r * NB;
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
-            if (t + u < maxlen) {
-                if (t + u + PF < maxlen) issue(buf[(u + PF) % NB], t + u + PF);
-                step(buf[u], t + u);
-            }
+            issue(buf[(u + PF) % NB], t + u + PF);
+            step(buf[u], t + u);
         }
     }
 }

@@ -46,12 +46,41 @@ struct AggParams {
                            // computed from the final sums instead of storing them
 };
 
+// Several independent direction passes in one launch (each into its own
+// volume).  Blocks [first_block, first_block + nblocks) belong to a pass;
+// nblocks is a multiple of 8 so that the hardware's round-robin placement of
+// workgroups over the 8 XCDs can be undone (each XCD gets a contiguous range
+// of the pass's scanlines and shares their census rows in its L2).
+constexpr int kMaxPasses = 8;
+struct PassDesc {
+    uint16_t *vol;
+    int dir, dx, dy, nscan;
+    int accumulate;
+    int first_block, nblocks;
+};
+struct MultiAgg {
+    AggParams base;        // shared fields; vol/dir/nscan/accumulate come per pass
+    int npass;
+    PassDesc pass[kMaxPasses];
+};
+
 hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
                          uint2 *out, hipStream_t s);
 // Default lane layout for D disparities.
 Layout default_layout(int D);
 bool layout_supported(Layout l, int D);
 hipError_t launch_aggregate(const AggParams &p, Layout l, hipStream_t s);
+// Passes `dirs[0..n)` at once, pass j into `vols[j]` (accumulating when
+// `accumulate[j]`); no winner-take-all.  base supplies the shared fields.
+hipError_t launch_aggregate_multi(const AggParams &base, const int *dirs,
+                                  uint16_t *const *vols, const int *accumulate, int n,
+                                  Layout l, hipStream_t s);
+// Sums `nvol` volumes (vol0 + k * vol_stride, k < nvol) mod 2^16 and picks
+// each pixel's winner with parabola refinement; with `sum_out` the summed
+// volume is also stored there (may equal vol0), with `best` the integer winner.
+hipError_t launch_wta_sum(const uint16_t *vol0, size_t vol_stride, int nvol, int npx,
+                          int D, int Dp, float *out, int16_t *best, uint16_t *sum_out,
+                          hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_wta(const uint16_t *vol, int w, int h, int D, int Dp,
                       float *out, int16_t *best, hipStream_t s);

@@ -58,16 +58,19 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 enum TexFormat { kGray8 = 1, kR32F = 2 };
 
 struct Layout {
-    size_t lc, rc, vol, best, total;
+    size_t lc, rc, vol, vol_bytes, best, total;
 };
 
-Layout workspace_layout(uint32_t w, uint32_t h, uint32_t D) {
+// Scratch of one pair: census of both images, `groups` summed volumes (one
+// per path group) and the winner index used by the LR check.
+Layout workspace_layout(uint32_t w, uint32_t h, uint32_t D, int groups = 1) {
     Layout l;
     size_t n = (size_t)w * h;
     l.lc = 0;
     l.rc = l.lc + align_up(n * 8);
     l.vol = l.rc + align_up(n * 8);
-    l.best = l.vol + align_up(n * (size_t)vxh::volume_stride((int)D) * 2);
+    l.vol_bytes = align_up(n * (size_t)vxh::volume_stride((int)D) * 2);
+    l.best = l.vol + l.vol_bytes * (size_t)groups;
     l.total = l.best + align_up(n * 2);
     return l;
 }
@@ -77,6 +80,8 @@ struct Slot {
     void *ws = nullptr;
     size_t ws_bytes = 0;
 };
+
+bool groups_valid(uint32_t g) { return g == 1 || g == 2 || g == 4 || g == 8; }
 
 }  // namespace
 
@@ -98,12 +103,23 @@ struct vxh_sgm {
     std::vector<Slot *> free_slots;
     std::vector<Slot *> all_slots;
     vxh::Layout layouts[257] = {};   // per D; L == 0 selects the default
+    uint8_t groups[257] = {};        // per D; 0 selects the default
 };
 
 static vxh::Layout layout_for(vxh_sgm *s, int D) {
     std::lock_guard<std::mutex> g(s->mu);
     vxh::Layout l = s->layouts[D];
     return l.L == 0 ? vxh::default_layout(D) : l;
+}
+
+static int groups_for(vxh_sgm *s, int D, int P) {
+    int g;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        g = s->groups[D] == 0 ? 1 : s->groups[D];
+    }
+    while (g > P) g /= 2;
+    return g;
 }
 
 extern "C" {
@@ -158,6 +174,16 @@ vxh_status vxh_sgm_config_validate(const vxh_sgm_config *c, uint32_t w,
     if (c->num_paths != 2 && c->num_paths != 4 && c->num_paths != 8) {
         snprintf(buf, sizeof buf, "invalid config: num_paths %u not in {2,4,8}",
                  c->num_paths);
+        return fail(VXH_ERR_INVALID_CONFIG, buf);
+    }
+    // Volume entries are addressed with 32-bit indices (as the reference's
+    // shader does, SGMStereo.metal:164), so the padded volume must stay below
+    // 2^32 entries.
+    if ((uint64_t)w * h * (uint64_t)vxh::volume_stride((int)c->num_disparities) >
+        0xFFFFFFFFull) {
+        snprintf(buf, sizeof buf,
+                 "invalid config: %ux%u with %u disparities exceeds 2^32 volume entries", w,
+                 h, c->num_disparities);
         return fail(VXH_ERR_INVALID_CONFIG, buf);
     }
     return VXH_OK;
@@ -281,6 +307,12 @@ size_t vxh_sgm_workspace_bytes(uint32_t w, uint32_t h, const vxh_sgm_config *c) 
     return workspace_layout(w, h, c->num_disparities).total;
 }
 
+size_t vxh_sgm_workspace_bytes_groups(uint32_t w, uint32_t h, const vxh_sgm_config *c,
+                                      uint32_t groups) {
+    if (!c || !groups_valid(groups)) return 0;
+    return workspace_layout(w, h, c->num_disparities, (int)groups).total;
+}
+
 // Runs every path into `vol`.  With `wta_out`, the last path also picks the
 // winner from its final sums and writes the disparity map instead of storing
 // its sums.
@@ -311,13 +343,55 @@ static vxh_status aggregate_impl(const uint2 *lc, const uint2 *rc, uint32_t w,
     return VXH_OK;
 }
 
+// Runs the P paths as P / groups launches of `groups` concurrent passes:
+// launch k runs directions k*groups .. k*groups + groups - 1, direction i
+// accumulating into volume i % groups (the slices of `vol_bytes` after
+// `vol`), and the summing winner-take-all adds the volumes.  Sums are
+// mod 2^16 and therefore independent of the grouping.  With `sum_out` the
+// summed volume is stored into the first slice and `best` receives the
+// integer winners (for the LR check).
+static vxh_status run_path_groups(const uint2 *lc, const uint2 *rc, uint32_t w, uint32_t h,
+                                  const vxh_sgm_config *c, uint16_t *vol, size_t vol_bytes,
+                                  int groups, vxh::Layout lay, hipStream_t st, float *out,
+                                  int16_t *best, bool store_sum) {
+    const int D = (int)c->num_disparities;
+    if (!vxh::layout_supported(lay, D))
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
+    const int P = (int)c->num_paths;
+    vxh::AggParams base{};
+    base.lc = lc;
+    base.rc = rc;
+    base.w = (int)w;
+    base.h = (int)h;
+    base.D = D;
+    base.Dp = vxh::volume_stride(D);
+    base.p1 = (uint32_t)c->p1 & 0xFFFFu;
+    base.p2 = (uint32_t)c->p2 & 0xFFFFu;
+    for (int k = 0; k < P / groups; ++k) {
+        int dirs[vxh::kMaxPasses], acc[vxh::kMaxPasses];
+        uint16_t *vols[vxh::kMaxPasses];
+        for (int g = 0; g < groups; ++g) {
+            dirs[g] = k * groups + g;
+            vols[g] = (uint16_t *)((char *)vol + (size_t)g * vol_bytes);
+            acc[g] = k > 0;
+        }
+        VXH_HIP(vxh::launch_aggregate_multi(base, dirs, vols, acc, groups, lay, st),
+                "aggregate");
+    }
+    VXH_HIP(vxh::launch_wta_sum(vol, vol_bytes / 2, groups, (int)(w * h), D, base.Dp, out,
+                                best, store_sum ? vol : nullptr, st),
+            "wta");
+    return VXH_OK;
+}
+
 static vxh_status enqueue_impl(vxh_sgm *s, const uint8_t *d_left, const uint8_t *d_right,
                                uint32_t w, uint32_t h, const vxh_sgm_config *c,
                                float *d_out, void *d_ws, size_t ws_bytes,
                                hipStream_t st) {
     vxh_status v = vxh_sgm_config_validate(c, w, h);
     if (v != VXH_OK) return v;
-    Layout L = workspace_layout(w, h, c->num_disparities);
+    const int D = (int)c->num_disparities;
+    Layout L = workspace_layout(w, h, D);
     if (!d_left || !d_right || !d_out || !d_ws)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null device buffer");
     if (ws_bytes < L.total) {
@@ -326,24 +400,35 @@ static vxh_status enqueue_impl(vxh_sgm *s, const uint8_t *d_left, const uint8_t 
                  ws_bytes, L.total);
         return fail(VXH_ERR_INVALID_CONFIG, buf);
     }
+    // The largest path-group count up to the handle's setting whose volumes
+    // fit the workspace given.
+    int groups = groups_for(s, D, (int)c->num_paths);
+    while (groups > 1 && workspace_layout(w, h, D, groups).total > ws_bytes) groups /= 2;
+    L = workspace_layout(w, h, D, groups);
     char *base = (char *)d_ws;
     uint2 *lc = (uint2 *)(base + L.lc);
     uint2 *rc = (uint2 *)(base + L.rc);
     uint16_t *vol = (uint16_t *)(base + L.vol);
     int16_t *best = (int16_t *)(base + L.best);
-    const int D = (int)c->num_disparities;
     VXH_HIP(vxh::launch_census(d_left, w, h, c->census_radius_x, c->census_radius_y, lc, st),
             "census");
     VXH_HIP(vxh::launch_census(d_right, w, h, c->census_radius_x, c->census_radius_y, rc, st),
             "census");
-    if (!c->lr_check)
-        return aggregate_impl(lc, rc, w, h, c, vol, layout_for(s, D), st, d_out);
-    // The LR check reads sums of other pixels, so the volume is completed
-    // first and the winner taken by separate passes.
-    vxh_status v2 = aggregate_impl(lc, rc, w, h, c, vol, layout_for(s, D), st);
-    if (v2 != VXH_OK) return v2;
+    const vxh::Layout lay = layout_for(s, D);
     const int Dp = vxh::volume_stride(D);
-    VXH_HIP(vxh::launch_wta(vol, w, h, D, Dp, d_out, best, st), "wta");
+    if (groups > 1) {
+        vxh_status v2 = run_path_groups(lc, rc, w, h, c, vol, L.vol_bytes, groups, lay, st,
+                                        d_out, c->lr_check ? best : nullptr, c->lr_check);
+        if (v2 != VXH_OK || !c->lr_check) return v2;
+    } else if (!c->lr_check) {
+        return aggregate_impl(lc, rc, w, h, c, vol, lay, st, d_out);
+    } else {
+        // The LR check reads sums of other pixels, so the volume is completed
+        // first and the winner taken by separate passes.
+        vxh_status v2 = aggregate_impl(lc, rc, w, h, c, vol, lay, st);
+        if (v2 != VXH_OK) return v2;
+        VXH_HIP(vxh::launch_wta(vol, w, h, D, Dp, d_out, best, st), "wta");
+    }
     VXH_HIP(vxh::launch_lr_check(vol, w, h, D, Dp, (int)c->lr_tolerance, best, d_out, st),
             "lr_check");
     return VXH_OK;
@@ -415,7 +500,8 @@ vxh_status vxh_sgm_compute_disparity(vxh_sgm *s, vxh_context *ctx,
     if (v != VXH_OK) return v;
     VXH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
     Slot *sl = nullptr;
-    v = acquire_slot(s, workspace_layout(w, h, c->num_disparities).total, &sl);
+    const int groups = groups_for(s, (int)c->num_disparities, (int)c->num_paths);
+    v = acquire_slot(s, workspace_layout(w, h, c->num_disparities, groups).total, &sl);
     if (v != VXH_OK) return v;
     v = enqueue_impl(s, (const uint8_t *)left->dptr, (const uint8_t *)right->dptr, w, h,
                      c, (float *)out->dptr, sl->ws, sl->ws_bytes, sl->stream);
@@ -486,6 +572,16 @@ vxh_status vxh_sgm_set_layout(vxh_sgm *s, uint32_t nd, uint32_t lanes, uint32_t 
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
     std::lock_guard<std::mutex> g(s->mu);
     s->layouts[nd] = l;
+    return VXH_OK;
+}
+
+vxh_status vxh_sgm_set_groups(vxh_sgm *s, uint32_t nd, uint32_t groups) {
+    if (!s || nd < 1 || nd > 256)
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: groups target");
+    if (groups != 0 && !groups_valid(groups))
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: path groups not in {1,2,4,8}");
+    std::lock_guard<std::mutex> g(s->mu);
+    s->groups[nd] = (uint8_t)groups;
     return VXH_OK;
 }
 

@@ -91,6 +91,8 @@ _sig("vxh_sgm_aggregate", ctypes.c_int, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp)
 _sig("vxh_sgm_wta", ctypes.c_int, _vp, _u32, _u32, _cfgp, _vp, _vp)
 _sig("vxh_sgm_volume_stride", _u32, _u32)
 _sig("vxh_sgm_set_layout", ctypes.c_int, _vp, _u32, _u32, _u32)
+_sig("vxh_sgm_set_groups", ctypes.c_int, _vp, _u32, _u32)
+_sig("vxh_sgm_workspace_bytes_groups", _sz, _u32, _u32, _cfgp, _u32)
 _sig("vxh_util_copy", ctypes.c_int, _vp, _vp, _sz, _vp)
 _sig("vxh_sgm_aggregate_layout", ctypes.c_int, _vp, _vp, _u32, _u32, _cfgp, _u32, _u32,
      _vp, _vp)
@@ -301,6 +303,10 @@ class SGMStereo:
         lanes, per = schedule if schedule else (0, 0)
         check(lib.vxh_sgm_set_layout(self._h, num_disparities, lanes, per))
 
+    def set_groups(self, num_disparities, groups):
+        """Concurrent direction passes (1, 2, 4, 8) for pairs with this D; 0/None = default."""
+        check(lib.vxh_sgm_set_groups(self._h, num_disparities, groups or 0))
+
     def enqueue(self, d_left, d_right, width, height, config, d_out, d_workspace,
                 workspace_bytes, stream=None):
         c = config.to_c()
@@ -501,9 +507,11 @@ def mixture_pair(w, h, d, pair_seed):
     return left, right, cfg
 
 
-def workspace_bytes(width, height, config):
+def workspace_bytes(width, height, config, groups=1):
     c = config.to_c()
-    return lib.vxh_sgm_workspace_bytes(width, height, ctypes.byref(c))
+    if groups == 1:
+        return lib.vxh_sgm_workspace_bytes(width, height, ctypes.byref(c))
+    return lib.vxh_sgm_workspace_bytes_groups(width, height, ctypes.byref(c), groups)
 
 
 def synth_pair(width, height, num_disparities, n_rects, n_planes, seed):

@@ -9,7 +9,8 @@ times every candidate in the form it will serve:
 
   tune_batch(w, h, D, paths)  a batch of copies of one class, persistent
                               kernel replayed from its hipGraph;
-  tune_plain(w, h, D, paths)  the per-pair pipeline of one pair.
+  tune_plain(w, h, D, paths)  the per-pair pipeline of one pair, also over
+                              the number of concurrent path groups.
 
 Timings use HIP events on the launch stream; the median of `reps` replays
 after one warm-up wins.  Winners are cached in a JSON file keyed by form and
@@ -82,31 +83,43 @@ class Tuner:
         return sched
 
     def tune_plain(self, sgm, w, h, D, paths, reps=5, force=False):
+        """Sweeps lane layout x concurrent path groups (1, 2, 4, 8) for one pair
+        and leaves the winner set on `sgm`; returns (schedule, groups)."""
         key = _key("plain", w, h, D, paths)
         if not force and key in self.table:
-            return tuple(self.table[key]["schedule"])
+            e = self.table[key]
+            sched, groups = tuple(e["schedule"]), e.get("groups", 1)
+            sgm.set_layout(D, sched)
+            sgm.set_groups(D, groups)
+            return sched, groups
         left, right, cfg = vxsgm.mixture_pair(w, h, D, 0x5452)
         cfg.num_paths = paths
         dl, dr = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
         out = hip.DeviceBuffer(w * h * 4)
-        wsb = vxsgm.workspace_bytes(w, h, cfg)
+        group_opts = [g for g in (1, 2, 4, 8) if g <= paths]
+        wsb = vxsgm.workspace_bytes(w, h, cfg, groups=max(group_opts))
         ws = hip.DeviceBuffer(wsb)
         stream = hip.Stream()
         times = {}
         for sched in candidates(D):
             sgm.set_layout(D, sched)
-            sgm.enqueue(dl.ptr, dr.ptr, w, h, cfg, out.ptr, ws.ptr, wsb, stream.handle)
-            ms = []
-            for _ in range(reps):
-                e0, e1 = hip.Event(), hip.Event()
-                e0.record(stream)
+            for g in group_opts:
+                sgm.set_groups(D, g)
                 sgm.enqueue(dl.ptr, dr.ptr, w, h, cfg, out.ptr, ws.ptr, wsb, stream.handle)
-                e1.record(stream)
-                e1.synchronize()
-                ms.append(e0.elapsed_ms(e1))
-            times[f"{sched[0]}x{sched[1]}"] = statistics.median(ms)
+                ms = []
+                for _ in range(reps):
+                    e0, e1 = hip.Event(), hip.Event()
+                    e0.record(stream)
+                    sgm.enqueue(dl.ptr, dr.ptr, w, h, cfg, out.ptr, ws.ptr, wsb,
+                                stream.handle)
+                    e1.record(stream)
+                    e1.synchronize()
+                    ms.append(e0.elapsed_ms(e1))
+                times[f"{sched[0]}x{sched[1]}/g{g}"] = statistics.median(ms)
         best = min(times, key=times.get)
-        sched = tuple(int(v) for v in best.split("x"))
+        lk, g = best.split("/g")
+        sched, groups = tuple(int(v) for v in lk.split("x")), int(g)
         sgm.set_layout(D, sched)
-        self.table[key] = {"schedule": list(sched), "ms": times}
-        return sched
+        sgm.set_groups(D, groups)
+        self.table[key] = {"schedule": list(sched), "groups": groups, "ms": times}
+        return sched, groups

@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB five times (the "
                          "known-traffic reference for HBM byte counters) and exit")
+    ap.add_argument("--schedule", default="",
+                    help="lane layout LxK for the plain path (e.g. 32x4; default: tuned "
+                         "or the built-in default)")
     ap.add_argument("--groups", type=int, default=0,
                     help="concurrent path groups for the plain path (0 = tuned or default)")
     return ap.parse_args()
@@ -243,6 +246,9 @@ def main():
     if args.groups:
         sgm.set_groups(D, args.groups)
         plain_groups = args.groups
+    if args.schedule:
+        plain_sched = tuple(int(v) for v in args.schedule.lower().split("x"))
+        sgm.set_layout(D, plain_sched)
     if not args.no_tune:
         from vxsgm.tuner import Tuner
         tuner = Tuner(path=None)

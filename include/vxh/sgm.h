@@ -176,7 +176,9 @@ vxh_status vxh_sgm_aggregate(const uint32_t *d_lc, const uint32_t *d_rc,
 /* vxh_sgm_aggregate with an explicit lane layout: `lanes` lanes of a wave
  * own one scanline (16, 32 or 64) and each owns `per_lane` consecutive
  * disparities (lanes * per_lane >= D); 0, 0 selects the default layout.
- * Unsupported layouts return VXH_ERR_INVALID_CONFIG. */
+ * Unsupported layouts return VXH_ERR_INVALID_CONFIG.  Both aggregate entry
+ * points take a stream-ordered scratch copy of d_rc (hipMallocAsync on
+ * `stream`, freed in stream order). */
 vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,
                                     uint32_t w, uint32_t h,
                                     const vxh_sgm_config *cfg, uint32_t lanes,

@@ -235,7 +235,8 @@ vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs, uint
         const size_t npx = (size_t)q.width * q.height;
         const int Dp = volume_stride(D);
         off[i] = slab;
-        slab += align256(npx * 8) * 2 + align256(npx * (size_t)Dp * 2) + align256(npx * 2);
+        slab += census_span(npx) + align256(npx * 8) + align256(npx * (size_t)Dp * 2) +
+                align256(npx * 2);
         PairDesc &d = pd[i];
         d.left = q.d_left;
         d.right = q.d_right;
@@ -262,11 +263,11 @@ vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs, uint
     for (uint32_t i = 0; i < n; ++i) {
         char *base = (char *)b->slab + off[i];
         const size_t npx = (size_t)pd[i].w * pd[i].h;
+        const size_t cs = census_span(npx) + align256(npx * 8);
         pd[i].lc = (uint2 *)base;
-        pd[i].rc = (uint2 *)(base + align256(npx * 8));
-        pd[i].vol = (uint16_t *)(base + 2 * align256(npx * 8));
-        pd[i].best = (int16_t *)(base + 2 * align256(npx * 8) +
-                                 align256(npx * (size_t)pd[i].Dp * 2));
+        pd[i].rc = (uint2 *)(base + census_span(npx));
+        pd[i].vol = (uint16_t *)(base + cs);
+        pd[i].best = (int16_t *)(base + cs + align256(npx * (size_t)pd[i].Dp * 2));
     }
 
     // Work list, phase-major; within a phase, larger pairs first.

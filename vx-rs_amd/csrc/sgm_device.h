@@ -29,6 +29,8 @@
 // sequential result including every wrap.
 #pragma once
 
+#include <type_traits>
+
 #include "sgm_internal.h"
 
 namespace vxh {
@@ -46,9 +48,30 @@ constexpr int kWaveShr1 = 0x138;     // lane i <- lane i-1 across the wave
 constexpr int kRowMirror = 0x140;    // lane i <- lane 15-i within a row
 constexpr int kRowHalfMirror = 0x141;// lane i <- lane 7-i within 8 lanes
 
+// Calls f(std::integral_constant<int, i>) for i = 0 .. N-1.
+template <int N, typename F, int I = 0>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<N, F, I + 1>(static_cast<F &&>(f));
+    }
+}
+
+// Lanes whose DPP source lies outside the wave (the ends of the wave shifts)
+// read 0 (bound_ctrl); every caller masks those lanes.  With no `old` operand
+// the compiler folds the move into the consuming ALU op (v_min_u32_dpp ...).
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_mov(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
+
+// Wave mask of lanes with a < b (signed) / a != b: one v_cmp whose result the
+// branch reads directly.
+__device__ __forceinline__ uint64_t lanes_lt(int a, int b) {
+    return __builtin_amdgcn_sicmp(a, b, 40);
+}
+__device__ __forceinline__ uint64_t lanes_ne(uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_uicmp(a, b, 33);
 }
 
 // Reduction with `op` over the L lanes of a group, broadcast to all of them.
@@ -198,11 +221,14 @@ __device__ __forceinline__ T *store_sink() {
     return (T *)&g_store_sink[(threadIdx.x & 63) * 2];
 }
 
-// Census word of pixel `i` (every caller passes an index inside the image).
-// Plain global loads, not buffer loads: the compiler counts these exactly when
-// placing its waits, so a step waits only for its own prefetched loads.
-__device__ __forceinline__ uint2 load_census(const uint2 *__restrict__ c, int i) {
-    return c[i];
+// Census word at byte offset `off` (a pixel index inside the image times 8;
+// images have fewer than 2^29 pixels, so the offset fits 32 bits and the load
+// takes the scalar-base + 32-bit-offset form).  Plain global loads, not buffer
+// loads: the compiler counts these exactly when placing its waits, so a step
+// waits only for its own prefetched loads.
+template <int IMM = 0>
+__device__ __forceinline__ uint2 load_census(const uint2 *__restrict__ c, uint32_t off) {
+    return *(const uint2 *)((const char *)c + off + IMM);
 }
 
 // Loads of one step: left census of the pixel, right census at x-d for the
@@ -212,25 +238,28 @@ struct StepBuf {
     uint2 lv;
     uint2 rv[K];
     Seg<K> sv;
+    bool edge;   // some active lane has x - d < 0 for one of its d (wave-uniform)
 };
 
 // One wave aggregates scanlines s_first .. s_first + 64/L - 1 of direction
 // (dx, dy) (those below p.nscan), adding into the summed volume.  Wave-level:
 // every lane of the wave must call it with the same arguments.
-template <int L, int K>
+//
+// ACC: 1 = the path adds into the volume, 0 = it stores (first path), -1 =
+// read p.accumulate at run time.
+template <int L, int K, int ACC = -1>
 __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int s_first) {
     const int lane = threadIdx.x & 63;
     const int g = lane / L, sub = lane % L;
     const int s = s_first + g;
     const int w = p.w, h = p.h, D = p.D, Dp = p.Dp;
     const uint16_t P1 = (uint16_t)p.p1, P2 = (uint16_t)p.p2;
-    const bool accumulate = p.accumulate != 0;
+    const bool accumulate = ACC < 0 ? p.accumulate != 0 : ACC == 1;
     const int d0 = sub * K;
     const bool lane_real = d0 < D;
     const bool lane_store = d0 < Dp;
-    const int npx = w * h;
     const uint2 *__restrict__ lc = p.lc;
-    const uint2 *__restrict__ rc = p.rc;
+    const uint2 *__restrict__ rcg = (const uint2 *)((const char *)p.rc - kCensusGuard);
 
     int sx = 0, sy = 0, len = 0;
     if (s < p.nscan) scan_start(s, dx, dy, w, h, sx, sy, len);
@@ -257,23 +286,20 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
 
     auto issue = [&](StepBuf<K> &b, int t) {
         const bool a = t < len;
-        const int pix = a ? pix0 + t * pstep : 0;
+        const uint32_t pix = a ? (uint32_t)(pix0 + t * pstep) : 0u;
         const int x = sx + t * dx;
-        b.lv = load_census(lc, pix);
-        if (__builtin_amdgcn_ballot_w64(a && x < d0 + K - 1) == 0 && npx >= K) {
-            // Every element x - d of every active lane is in the row: one base
-            // and constant offsets (the compiler may merge these loads into
-            // wider ones).  Inactive lanes read pixels 0 .. K-1.
-            const int base = a ? pix - d0 - (K - 1) : 0;
-#pragma unroll
-            for (int i = 0; i < K; ++i) b.rv[i] = load_census(rc, base + (K - 1 - i));
-        } else {
-            // Left edge: each element gets its own in-range index; those with
-            // x - d < 0 read the pixel itself and are replaced by 64 later.
-#pragma unroll
-            for (int i = 0; i < K; ++i)
-                b.rv[i] = load_census(rc, (a && x >= d0 + i) ? pix - d0 - i : pix);
-        }
+        b.lv = load_census(lc, pix << 3);
+        // Right census at x - d for the lane's K disparities: one base, constant
+        // offsets.  Reads with x - d < 0 fall into the previous row or the
+        // guard bytes before rc and are replaced by 64 in step().  Inactive
+        // lanes read around pixel 0.
+        const uint32_t rb = ((pix - (uint32_t)(d0 + K - 1)) << 3) + (uint32_t)kCensusGuard;
+        static_for<K>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            b.rv[i] = load_census<8 * (K - 1 - i)>(rcg, rb);
+        });
+        // Inactive lanes count as far from the edge.
+        b.edge = lanes_lt(a ? x : 0x7FFFFFFF, d0 + K - 1) != 0;
         // Lanes without a volume entry here read entry 0 (always in range)
         // and discard it.
         const uint32_t off = (a && lane_store) ? voff0 + (uint32_t)t * vstep : 0u;
@@ -287,7 +313,7 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
 
         uint16_t C[K];
         const uint2 lv = cur.lv;
-        if (__builtin_amdgcn_ballot_w64(act && x < d0 + K - 1) != 0) {
+        if (cur.edge) {
 #pragma unroll
             for (int i = 0; i < K; ++i) {
                 const uint32_t c = __popc(lv.x ^ cur.rv[i].x) + __popc(lv.y ^ cur.rv[i].y);
@@ -328,10 +354,10 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
             uint16_t e_used = A[0];
             for (int r = 0; r < L; ++r) {
                 const uint16_t up = (uint16_t)from_lower_lane<L>((uint32_t)agg[K - 1]);
-                const uint16_t e = min(A[0], (uint16_t)(up + P1));
-                const bool ch = live && e != e_used;
-                if (__builtin_amdgcn_ballot_w64(ch) == 0) break;
-                if (ch) {
+                // Lanes that take no carry keep e == e_used.
+                const uint16_t e = live ? min(A[0], (uint16_t)(up + P1)) : e_used;
+                if (lanes_ne(e, e_used) == 0) break;
+                if (e != e_used) {
                     e_used = e;
                     agg[0] = (uint16_t)(B[0] + e);
                     uint16_t o1 = (uint16_t)(agg[0] + P1);
@@ -411,8 +437,13 @@ __device__ __forceinline__ void agg_wave(const AggParams &p, int dx, int dy, int
     constexpr int NB = PF + 1;
     const int rounds = __builtin_amdgcn_readfirstlane((maxlen + NB - 1) / NB);
     StepBuf<K> buf[NB];
+    // The prologue keeps its issue order (buffer 0 first): the waits at the
+    // top of the loop are derived from it.
 #pragma unroll
-    for (int j = 0; j < PF; ++j) issue(buf[j], j);
+    for (int j = 0; j < PF; ++j) {
+        issue(buf[j], j);
+        __builtin_amdgcn_sched_barrier(0);
+    }
     for (int r = 0; r < rounds; ++r) {
         const int t = r * NB;
 #pragma unroll

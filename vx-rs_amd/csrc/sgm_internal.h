@@ -33,9 +33,22 @@ struct Layout {
     int L, K;
 };
 
+// The aggregation reads the right census at x - d for every d of a lane
+// through one base kCensusGuard bytes before rc; the reads with x - d < 0
+// (whose cost is replaced by 64) then land in the kCensusGuard readable bytes
+// that every rc buffer must have in front of it (the left census, or padding).
+constexpr size_t kCensusGuard = 2048;
+
+// Bytes reserved for the left census in front of the right one, so that rc
+// has kCensusGuard readable bytes before it.
+inline size_t census_span(size_t npx) {
+    const size_t b = npx * 8 > kCensusGuard ? npx * 8 : kCensusGuard;
+    return (b + 255) & ~(size_t)255;
+}
+
 struct AggParams {
     const uint2 *lc;       // left census, w*h (lo, hi)
-    const uint2 *rc;       // right census
+    const uint2 *rc;       // right census, kCensusGuard readable bytes before it
     uint16_t *vol;         // summed volume, (y*w + x)*Dp + d
     int w, h, D, Dp;
     uint32_t p1, p2;       // penalties already truncated to u16

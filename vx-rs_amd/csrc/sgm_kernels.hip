@@ -34,7 +34,7 @@ census_kernel(const uint8_t *__restrict__ img, int w, int h, int rx, int ry,
 // block index is permuted so that the blocks the hardware places on one XCD
 // (b = x mod 8) take a contiguous range of scanlines, whose census rows then
 // stay in that XCD's L2.
-template <int L, int K>
+template <int L, int K, int ACC>
 __global__ void __launch_bounds__(256)
 aggregate_kernel(MultiAgg m) {
     const int b = blockIdx.x;
@@ -49,9 +49,10 @@ aggregate_kernel(MultiAgg m) {
     p.vol = VXH_PASS_FIELD(m, i, vol);
     p.dir = VXH_PASS_FIELD(m, i, dir);
     p.nscan = VXH_PASS_FIELD(m, i, nscan);
-    p.accumulate = VXH_PASS_FIELD(m, i, accumulate);
+    p.accumulate = ACC;
     const int wave = lb * 4 + (int)(threadIdx.x >> 6);
-    agg_wave<L, K>(p, VXH_PASS_FIELD(m, i, dx), VXH_PASS_FIELD(m, i, dy), wave * (64 / L));
+    agg_wave<L, K, ACC>(p, VXH_PASS_FIELD(m, i, dx), VXH_PASS_FIELD(m, i, dy),
+                        wave * (64 / L));
 }
 
 // Sum of `NV` volumes and winner-take-all with parabola refinement
@@ -161,7 +162,11 @@ hipError_t launch_multi_lk(MultiAgg &m, hipStream_t s) {
         blocks += nb;
     }
     for (int j = m.npass; j < kMaxPasses; ++j) m.pass[j] = m.pass[0];
-    hipLaunchKernelGGL((aggregate_kernel<L, K>), dim3(blocks), dim3(256), 0, s, m);
+    // Every pass of one launch either stores or accumulates.
+    if (m.pass[0].accumulate)
+        hipLaunchKernelGGL((aggregate_kernel<L, K, 1>), dim3(blocks), dim3(256), 0, s, m);
+    else
+        hipLaunchKernelGGL((aggregate_kernel<L, K, 0>), dim3(blocks), dim3(256), 0, s, m);
     return hipGetLastError();
 }
 
@@ -231,7 +236,8 @@ hipError_t launch_aggregate_multi(const AggParams &base, const int *dirs,
     m.npass = n;
     int order[kMaxPasses];
     for (int j = 0; j < n; ++j) {
-        if (dirs[j] < 0 || dirs[j] > 7) return hipErrorInvalidValue;
+        if (dirs[j] < 0 || dirs[j] > 7 || (accumulate[j] != 0) != (accumulate[0] != 0))
+            return hipErrorInvalidValue;
         order[j] = j;
     }
     std::stable_sort(order, order + n, [&](int a, int b) {

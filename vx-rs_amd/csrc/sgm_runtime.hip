@@ -67,7 +67,7 @@ Layout workspace_layout(uint32_t w, uint32_t h, uint32_t D, int groups = 1) {
     Layout l;
     size_t n = (size_t)w * h;
     l.lc = 0;
-    l.rc = l.lc + align_up(n * 8);
+    l.rc = l.lc + vxh::census_span(n);
     l.vol = l.rc + align_up(n * 8);
     l.vol_bytes = align_up(n * (size_t)vxh::volume_stride((int)D) * 2);
     l.best = l.vol + l.vol_bytes * (size_t)groups;
@@ -605,8 +605,26 @@ vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,
     vxh::Layout lay = (lanes == 0 && per_lane == 0)
                           ? vxh::default_layout((int)c->num_disparities)
                           : vxh::Layout{(int)lanes, (int)per_lane};
-    return aggregate_impl((const uint2 *)d_lc, (const uint2 *)d_rc, w, h, c, d_volume, lay,
-                          (hipStream_t)stream);
+    if (!vxh::layout_supported(lay, (int)c->num_disparities))
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
+    // The kernels read up to kCensusGuard bytes in front of the right census,
+    // which a caller's buffer need not have: copy it behind such a guard
+    // (stream-ordered allocation, so the call stays asynchronous).
+    hipStream_t st = (hipStream_t)stream;
+    const size_t cb = (size_t)w * h * 8;
+    char *g = nullptr;
+    hipError_t e = hipMallocAsync((void **)&g, vxh::kCensusGuard + cb, st);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync", vxh::kCensusGuard + cb);
+    e = hipMemcpyAsync(g + vxh::kCensusGuard, d_rc, cb, hipMemcpyDeviceToDevice, st);
+    vxh_status v2 = e == hipSuccess
+                        ? aggregate_impl((const uint2 *)d_lc,
+                                         (const uint2 *)(g + vxh::kCensusGuard), w, h, c,
+                                         d_volume, lay, st)
+                        : hip_fail(e, "hipMemcpyAsync");
+    hipError_t e2 = hipFreeAsync(g, st);
+    if (v2 != VXH_OK) return v2;
+    if (e2 != hipSuccess) return hip_fail(e2, "hipFreeAsync");
+    return VXH_OK;
 }
 
 vxh_status vxh_sgm_aggregate(const uint32_t *d_lc, const uint32_t *d_rc, uint32_t w,

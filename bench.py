@@ -8,12 +8,16 @@ winner-take-all) over one synthetic rectified pair already resident in HBM,
 on the workload 720p_d128_8path (1280x720, D=128, 8 paths, P1=10, P2=120,
 census 9x7).  value = W*H*D*K / t / 1e9 Gpx*disp/s over K timed steps.
 
-`roofline` prices the per-pair pipeline against HBM: algorithmic bytes per
-pair are W*H*(4*D + 6) (the u16 summed volume written once and read once,
-both gray8 inputs, the f32 output), divided by the pipeline's average HIP-event
-time on the launch stream.  `cpu_baseline` times the scalar C++ oracle (the
-checker) on the same pair with the host's threads and confirms that the timed
-GPU map is bit-identical to it.
+`roofline` prices the dominant kernel, the path aggregation (all directions
+in one launch), against HBM.  The pipeline's algorithmic bytes per pair,
+W*H*(4*D + 6) (the u16 summed volume written once and read once, both gray8
+inputs, the f32 output), split between its two kernels: the aggregation's
+share is W*H*(2*D + 2) (the summed volume written once, both inputs read), the
+winner-take-all's W*H*(2*D + 4).  The aggregation's share is divided by its
+average HIP-event time over the timed steps, recorded on the launch stream
+between the census and winner-take-all launches.  `cpu_baseline` times the
+scalar C++ oracle (the checker) on the same pair with the host's threads and
+confirms that the timed GPU map is bit-identical to it.
 
 `batched`: the 64-pair mixture of sizes {320x240, 640x480, 1280x720,
 1920x1080} x D {64, 128, 256}, 8 paths, as one persistent launch replayed from
@@ -54,10 +58,11 @@ def parse():
                     help="only run the plain copy kernel over MIB MiB five times (the "
                          "known-traffic reference for HBM byte counters) and exit")
     ap.add_argument("--schedule", default="",
-                    help="lane layout LxK for the plain path (e.g. 32x4; default: tuned "
-                         "or the built-in default)")
-    ap.add_argument("--groups", type=int, default=0,
-                    help="concurrent path groups for the plain path (0 = tuned or default)")
+                    help="lane layouts of the plain path, horizontal/other (e.g. "
+                         "64x2/16x8) or one LxK for all directions; default: tuned or "
+                         "the built-in default")
+    ap.add_argument("--volume-format", default="auto", choices=["auto", "u8", "u16"],
+                    help="per-direction volume storage of the plain path")
     return ap.parse_args()
 
 
@@ -65,34 +70,7 @@ def log(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def stage_times(vx, hip, d_left, d_right, w, h, cfg, sched, d_out, ws, stream, reps=10):
-    """Per-stage HIP-event times (ms) of the same pipeline, on the launch stream."""
-    csz = ((w * h * 8 + 255) // 256) * 256
-    lc, rc, vol = ws.ptr, ws.ptr + csz, ws.ptr + 2 * csz
-    c = cfg.to_c()
-    L, K = sched if sched else (0, 0)
-    ev = [hip.Event() for _ in range(4)]
-    acc = [0.0, 0.0, 0.0]
-    for _ in range(reps):
-        ev[0].record(stream)
-        vx.check(vx.lib.vxh_sgm_census(d_left.ptr, w, h, cfg.census_radius_x,
-                                       cfg.census_radius_y, lc, stream.handle))
-        vx.check(vx.lib.vxh_sgm_census(d_right.ptr, w, h, cfg.census_radius_x,
-                                       cfg.census_radius_y, rc, stream.handle))
-        ev[1].record(stream)
-        vx.check(vx.lib.vxh_sgm_aggregate_layout(lc, rc, w, h, ctypes.byref(c), L, K, vol,
-                                                 stream.handle))
-        ev[2].record(stream)
-        vx.check(vx.lib.vxh_sgm_wta(vol, w, h, ctypes.byref(c), d_out.ptr, stream.handle))
-        ev[3].record(stream)
-        ev[3].synchronize()
-        for i in range(3):
-            acc[i] += ev[i].elapsed_ms(ev[i + 1])
-    return {"census_x2": acc[0] / reps, "aggregate_all_paths": acc[1] / reps,
-            "wta": acc[2] / reps}
-
-
-def batched_leg(vx, hip, np, args, sgm, tuner, plain_groups):
+def batched_leg(vx, hip, np, args, sgm, tuner):
     """The 64-pair mixture in one persistent graph launch vs a loop of plain
     per-pair calls on one stream; every map compared bit for bit."""
     mix = vx.mixture(args.batch_pairs)
@@ -103,11 +81,11 @@ def batched_leg(vx, hip, np, args, sgm, tuner, plain_groups):
         dl, dr = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
         ob, op = hip.DeviceBuffer(w * h * 4), hip.DeviceBuffer(w * h * 4)
         bufs.append((dl, dr, ob, op))
-        sched = None if (args.no_tune or tuner is None) else tuner.tune_batch(w, h, d, 8)
-        descs.append((dl.ptr, dr.ptr, ob.ptr, w, h, cfg, sched))
-    log(f"batched: {len(host)} pairs generated and tuned")
+        descs.append((dl.ptr, dr.ptr, ob.ptr, w, h, cfg, None))
+    opts = None if tuner is None else tuner.tune_batch(descs)
+    log(f"batched: {len(host)} pairs, launch schedule {opts}")
     stream = hip.Stream()
-    batch = vx.SGMBatch(descs)
+    batch = vx.SGMBatch(descs, options=opts)
     batch.launch(stream.handle)
     stream.synchronize()
     e0, e1 = hip.Event(), hip.Event()
@@ -118,10 +96,8 @@ def batched_leg(vx, hip, np, args, sgm, tuner, plain_groups):
     e1.synchronize()
     ms_batch = e0.elapsed_ms(e1) / args.batch_reps
     log(f"batched: {ms_batch:.3f} ms per batch")
-    # The plain loop runs with the path groups the plain path was tuned to.
-    for d in sorted({d for _, _, d, _ in host}):
-        sgm.set_groups(d, plain_groups)
-    wsb = max(vx.workspace_bytes(w, h, cfg, groups=plain_groups) for w, h, _, cfg in host)
+    # The plain loop runs each pair with the handle's schedule for its D.
+    wsb = max(vx.workspace_bytes(w, h, cfg) for w, h, _, cfg in host)
     ws = hip.DeviceBuffer(wsb)
 
     def plain_loop():
@@ -152,6 +128,7 @@ def batched_leg(vx, hip, np, args, sgm, tuner, plain_groups):
         "roofline_frac_batch": round(alg / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "bit_exact_vs_plain": bool(exact),
         "form": "persistent kernel replayed from a hipGraph",
+        "launch_schedule": list(opts) if opts else "default",
     }
 
 
@@ -236,50 +213,47 @@ def main():
     sgm = vx.SGMStereo(ctx)
     d_left, d_right = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
     d_out = hip.DeviceBuffer(w * h * 4)
-    # Room for eight path-group volumes; the handle uses as many as it is set to.
-    ws_bytes = vx.workspace_bytes(w, h, cfg, groups=8)
+    ws_bytes = vx.workspace_bytes(w, h, cfg)
     ws = hip.DeviceBuffer(ws_bytes)
     stream = hip.Stream()
 
     tuner = None
-    plain_sched, plain_groups = None, 1
-    if args.groups:
-        sgm.set_groups(D, args.groups)
-        plain_groups = args.groups
+    sgm.set_volume_format(D, args.volume_format)
     if args.schedule:
-        plain_sched = tuple(int(v) for v in args.schedule.lower().split("x"))
-        sgm.set_layout(D, plain_sched)
-    if not args.no_tune:
+        parts = [tuple(int(v) for v in p.split("x")) for p in args.schedule.lower().split("/")]
+        sgm.set_schedule(D, (parts[0], parts[-1]))
+    elif not args.no_tune:
         from vxsgm.tuner import Tuner
         tuner = Tuner(path=None)
-        plain_sched, plain_groups = tuner.tune_plain(sgm, w, h, D, paths)
-        log(f"plain schedule {plain_sched} groups {plain_groups}: {tuner.table}")
-
-    def step():
-        sgm.enqueue(d_left.ptr, d_right.ptr, w, h, cfg, d_out.ptr, ws.ptr, ws_bytes,
-                    stream.handle)
+        tuner.tune_plain(sgm, w, h, D, paths)
+        log(f"plain schedules: {tuner.table}")
+    plain_sched = sgm.schedule(D)
 
     for _ in range(args.warmup):
-        step()
+        sgm.enqueue(d_left.ptr, d_right.ptr, w, h, cfg, d_out.ptr, ws.ptr, ws_bytes,
+                    stream.handle)
     stream.synchronize()
-    e0, e1 = hip.Event(), hip.Event()
+    # Per step: start, after census, after aggregation, end (stream order).
+    marks = [[hip.Event() for _ in range(4)] for _ in range(args.steps)]
     t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    e1.synchronize()
+    for m in marks:
+        m[0].record(stream)
+        sgm.enqueue_marked(d_left.ptr, d_right.ptr, w, h, cfg, d_out.ptr, ws.ptr, ws_bytes,
+                           stream.handle, m[1:])
+    marks[-1][3].synchronize()
     wall = time.perf_counter() - t0
-    ms_event = e0.elapsed_ms(e1) / args.steps
+    ms_event = marks[0][0].elapsed_ms(marks[-1][3]) / args.steps
+    stage = [sum(m[i].elapsed_ms(m[i + 1]) for m in marks) / args.steps for i in range(3)]
     ms_per_step = wall * 1e3 / args.steps
     gpu_disp = d_out.download(np.float32).reshape(h, w)
-    stages = stage_times(vx, hip, d_left, d_right, w, h, cfg, plain_sched, d_out, ws, stream)
+    stages = {"census_x2": stage[0], "aggregate_all_paths": stage[1], "sum_wta": stage[2]}
     log(f"plain: {ms_per_step:.3f} ms/step, stages {stages}")
 
     pxd = w * h * D
     value = pxd * args.steps / wall / 1e9
     alg_bytes = w * h * (4 * D + 6)
-    achieved = alg_bytes / (ms_event * 1e-3) / 1e9
+    agg_bytes = w * h * (2 * D + 2)
+    achieved = agg_bytes / (stage[1] * 1e-3) / 1e9
     result = {
         "metric": "Gpx*disp/s @1280x720 D128 8-path (bit-exact)",
         "value": round(value, 3),
@@ -296,14 +270,18 @@ def main():
         "config": {"workload": args.workload, "width": w, "height": h,
                    "num_disparities": D, "num_paths": paths, "p1": cfg.p1,
                    "p2": cfg.p2, "census": "9x7", "form": "plain per-pair",
-                   "schedule": list(plain_sched) if plain_sched else "default",
-                   "path_groups": plain_groups},
-        "roofline": {"bound": "hbm", "kernel": "per-pair pipeline (census x2, "
-                     "aggregate per path, wta)", "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "schedule": {"horizontal": list(plain_sched[0]),
+                                "other": list(plain_sched[1])},
+                   "volume_format": args.volume_format},
+        "roofline": {"bound": "hbm", "kernel": "paths_kernel (aggregation, all directions)",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "alg_bytes_per_launch": alg_bytes,
-                     "launch_ms_event": round(ms_event, 4)},
+                     "alg_bytes_per_launch": agg_bytes,
+                     "launch_ms_event": round(stage[1], 4),
+                     "pipeline_alg_bytes": alg_bytes,
+                     "pipeline_ms_event": round(ms_event, 4),
+                     "pipeline_frac": round(alg_bytes / (ms_event * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                            4)},
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
     }
     if not args.no_cpu_baseline:
@@ -325,7 +303,7 @@ def main():
     forms = set(args.forms.split(","))
     if "batched" in forms:
         log(f"batched leg ({args.batch_pairs} pairs)")
-        result["batched"] = batched_leg(vx, hip, np, args, sgm, tuner, plain_groups)
+        result["batched"] = batched_leg(vx, hip, np, args, sgm, tuner)
     if "served" in forms:
         log(f"served leg (rates {args.rates})")
         result["served"] = served_leg(vx, np, args)

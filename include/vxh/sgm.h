@@ -143,23 +143,29 @@ vxh_status vxh_sgm_compute_host(vxh_sgm *sgm, const uint8_t *left,
 
 /* ---- Device-resident, stream-ordered form (the encode() pattern the
  * reference's kernels follow, vx-vision/src/pipeline.rs:12-80) ---- */
-/* Scratch bytes one device-resident call needs for a w x h pair. */
+/* Scratch bytes one device-resident call needs for a w x h pair (enough for
+ * either volume format). */
 size_t vxh_sgm_workspace_bytes(uint32_t w, uint32_t h,
                                const vxh_sgm_config *cfg);
-/* Scratch bytes for a call that runs its paths in `groups` concurrent path
- * groups (1, 2, 4 or 8; one volume each); 0 for other values. */
-size_t vxh_sgm_workspace_bytes_groups(uint32_t w, uint32_t h,
-                                      const vxh_sgm_config *cfg, uint32_t groups);
 /* Enqueues the whole pipeline on `stream` (a hipStream_t; NULL = the null
  * stream) with caller-owned device buffers: left/right w*h bytes, out w*h
- * floats, workspace >= vxh_sgm_workspace_bytes.  Does not synchronize.  The
- * path groups used are the handle's setting (vxh_sgm_set_groups) reduced
- * until their volumes fit `workspace_bytes`. */
+ * floats, workspace >= vxh_sgm_workspace_bytes.  Does not synchronize. */
 vxh_status vxh_sgm_enqueue(vxh_sgm *sgm, const uint8_t *d_left,
                            const uint8_t *d_right, uint32_t w, uint32_t h,
                            const vxh_sgm_config *cfg, float *d_out,
                            void *d_workspace, size_t workspace_bytes,
                            void *stream);
+
+/* vxh_sgm_enqueue that also records three caller-created HIP events
+ * (hipEvent_t) on `stream`: events3[0] after the census pass, events3[1]
+ * after the path aggregation, events3[2] after the winner-take-all (and LR
+ * check), so a caller can time each stage in stream order.  No reference
+ * counterpart (the reference has no GPU timestamps). */
+vxh_status vxh_sgm_enqueue_marked(vxh_sgm *sgm, const uint8_t *d_left,
+                                  const uint8_t *d_right, uint32_t w, uint32_t h,
+                                  const vxh_sgm_config *cfg, float *d_out,
+                                  void *d_workspace, size_t workspace_bytes,
+                                  void *stream, void *const *events3);
 
 /* Stage-level entry points on device buffers, for parity tests of the
  * intermediate results.  d_census is 2*w*h uint32 (lo, hi per pixel);
@@ -173,12 +179,13 @@ vxh_status vxh_sgm_aggregate(const uint32_t *d_lc, const uint32_t *d_rc,
                              uint32_t w, uint32_t h,
                              const vxh_sgm_config *cfg, uint16_t *d_volume,
                              void *stream);
-/* vxh_sgm_aggregate with an explicit lane layout: `lanes` lanes of a wave
- * own one scanline (16, 32 or 64) and each owns `per_lane` consecutive
- * disparities (lanes * per_lane >= D); 0, 0 selects the default layout.
+/* vxh_sgm_aggregate with an explicit lane layout for every direction:
+ * `lanes` lanes of a wave own a pair of scanlines and each lane owns
+ * `per_lane` consecutive disparities, lanes * per_lane = 64, 128 or 256 for
+ * D <= 64, <= 128, <= 256 (per_lane 2..16); 0, 0 selects the default.
  * Unsupported layouts return VXH_ERR_INVALID_CONFIG.  Both aggregate entry
- * points take a stream-ordered scratch copy of d_rc (hipMallocAsync on
- * `stream`, freed in stream order). */
+ * points take stream-ordered scratch (hipMallocAsync on `stream`, freed in
+ * stream order) for a guarded copy of d_rc and the per-direction volumes. */
 vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,
                                     uint32_t w, uint32_t h,
                                     const vxh_sgm_config *cfg, uint32_t lanes,
@@ -187,23 +194,29 @@ vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,
 vxh_status vxh_sgm_wta(const uint16_t *d_volume, uint32_t w, uint32_t h,
                        const vxh_sgm_config *cfg, float *d_out, void *stream);
 
-/* Selects the aggregation lane layout this handle uses for pairs with exactly
- * `num_disparities` disparities (lanes 16/32/64, per_lane as for
- * vxh_sgm_aggregate_layout); 0, 0 restores the default.  Results do not
- * depend on the layout, only the speed does. */
-vxh_status vxh_sgm_set_layout(vxh_sgm *sgm, uint32_t num_disparities,
-                              uint32_t lanes, uint32_t per_lane);
+/* Selects the aggregation lane layouts this handle uses for pairs with
+ * exactly `num_disparities` disparities: (lanes_h, per_lane_h) for the
+ * horizontal directions, whose scanlines are the longest, and
+ * (lanes, per_lane) for the others, each as for vxh_sgm_aggregate_layout;
+ * all zero restores the default.  Results do not depend on the layouts, only
+ * the speed does.  No reference counterpart (the reference has one fixed
+ * thread per scanline, vx-vision/src/kernels/sgm.rs:346-357). */
+vxh_status vxh_sgm_set_schedule(vxh_sgm *sgm, uint32_t num_disparities,
+                                uint32_t lanes_h, uint32_t per_lane_h,
+                                uint32_t lanes, uint32_t per_lane);
+/* The layouts in effect for `num_disparities`: out4 = lanes_h, per_lane_h,
+ * lanes, per_lane. */
+vxh_status vxh_sgm_get_schedule(vxh_sgm *sgm, uint32_t num_disparities,
+                                uint32_t *out4);
 
-/* Selects how many direction passes (1, 2, 4 or 8; 0 = the default, 1) this
- * handle runs concurrently for pairs with exactly `num_disparities`
- * disparities.  With g > 1 the paths run as num_paths / g launches of g
- * passes each, pass i accumulating into volume i % g, and a summing
- * winner-take-all adds the g volumes.  The result is bit-identical for every
- * setting (sums are mod 2^16); the workspace grows by one volume per extra
- * group (vxh_sgm_workspace_bytes_groups).  No reference counterpart: the
- * reference encodes its paths one after another. */
-vxh_status vxh_sgm_set_groups(vxh_sgm *sgm, uint32_t num_disparities,
-                              uint32_t groups);
+/* Storage of the per-direction path volumes for pairs with exactly
+ * `num_disparities` disparities: 0 = automatic (u8 when 64 + P2 <= 255,
+ * else u16), 1 = u8 low bytes plus the high bytes of the rare slices that
+ * exceed 255, 2 = u16.  Results are identical for every setting (all sums
+ * are mod 2^16).  No reference counterpart (the reference accumulates one
+ * u16 volume, SGMStereo.metal:164-165). */
+vxh_status vxh_sgm_set_volume_format(vxh_sgm *sgm, uint32_t num_disparities,
+                                     uint32_t format);
 
 /* Plain device-to-device copy kernel (16-byte vectors, grid-stride), used to
  * calibrate HBM traffic counters against a kernel of known traffic.  bytes

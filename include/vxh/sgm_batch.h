@@ -22,9 +22,12 @@ extern "C" {
 #endif
 
 /* One pair of a batch: caller-owned device buffers (left/right w*h bytes,
- * out w*h floats, all resident on the batch's device), its config, and the
- * aggregation schedule (lanes per scanline and disparities per lane, as for
- * vxh_sgm_aggregate_layout; 0, 0 = the default for its D). */
+ * out w*h floats, all resident on the batch's device), its config, the
+ * aggregation layouts as for vxh_sgm_set_schedule ((lanes, per_lane) for the
+ * non-horizontal directions, (lanes_h, per_lane_h) for the horizontal ones,
+ * 0, 0 = default; the batched kernel runs 8 disparities per lane: 8x8,
+ * 16x8 or 32x8 for D <= 64, <= 128, <= 256, and rejects other layouts), and
+ * volume_format as for vxh_sgm_set_volume_format (0 = automatic). */
 typedef struct {
     const uint8_t *d_left;
     const uint8_t *d_right;
@@ -34,22 +37,41 @@ typedef struct {
     vxh_sgm_config cfg;
     uint32_t lanes;
     uint32_t per_lane;
+    uint32_t lanes_h;
+    uint32_t per_lane_h;
+    uint32_t volume_format;
 } vxh_sgm_pair_desc;
 
 typedef struct vxh_sgm_batch vxh_sgm_batch;
 
+/* Schedule of the persistent launch (the autotuner's knobs; 0 = default):
+ *   blocks_per_cu  resident 256-thread workgroups per CU, 1 or 2 (default 2);
+ *   wta_lag        pairs of aggregation items placed between a pair's last
+ *                  aggregation item and its sum + winner items (default 2);
+ *   wta_pixels     pixels per sum + winner item, a multiple of 4 (default
+ *                  1024). */
+typedef struct {
+    uint32_t blocks_per_cu;
+    uint32_t wta_lag;
+    uint32_t wta_pixels;
+} vxh_sgm_batch_options;
+
 /* Per-pair plan summary. */
 typedef struct {
     uint32_t items;      /* work items of this pair in the batch's list */
-    uint32_t lanes;      /* schedule used */
+    uint32_t lanes;      /* schedule used for the non-horizontal directions */
     uint32_t per_lane;
 } vxh_sgm_batch_stats;
 
-/* Validates every pair, allocates the per-pair scratch (census, summed
- * volume), builds the work list and captures the persistent launch in a
+/* Validates every pair, allocates the per-pair scratch (census, direction
+ * volumes; vxh_sgm_workspace_bytes per pair), builds the work list and captures the persistent launch in a
  * hipGraph.  Buffers named in `pairs` are bound at creation. */
 vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs,
                                 uint32_t n, vxh_sgm_batch **out);
+/* vxh_sgm_batch_create with an explicit launch schedule (NULL = defaults). */
+vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs,
+                                   uint32_t n, const vxh_sgm_batch_options *options,
+                                   vxh_sgm_batch **out);
 /* Replays the graph on `stream` (NULL = the batch's own stream). */
 vxh_status vxh_sgm_batch_launch(vxh_sgm_batch *batch, void *stream);
 /* Launch on the batch's own stream, wait, and check the error flag. */

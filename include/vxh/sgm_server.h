@@ -44,7 +44,9 @@ typedef struct {
 
 typedef struct vxh_server vxh_server;
 
-/* Cost-volume bytes a request is admitted against (W*H*stride(D)*2). */
+/* Device bytes a request is admitted against: its pipeline workspace
+ * (vxh_sgm_workspace_bytes: the per-direction cost volumes, their high-byte
+ * volumes and the census). */
 uint64_t vxh_server_request_bytes(uint32_t w, uint32_t h, const vxh_sgm_config *cfg);
 
 vxh_status vxh_server_create(int device, const vxh_server_config *cfg,

@@ -86,16 +86,23 @@ def test_volume_matches_oracle(w, h, D, paths):
         raise AssertionError(f"{len(bad)} volume entries differ; first {bad[:5].tolist()}")
 
 
-LAYOUTS = [(64, 1), (64, 2), (64, 4), (32, 1), (32, 2), (32, 4), (32, 8), (16, 4),
-           (16, 8), (16, 16)]
+# Lane layouts by class: D <= 64, <= 128, <= 256 (lanes own a scanline pair).
+CLASS_LAYOUTS = [[(32, 2), (16, 4), (8, 8)], [(64, 2), (32, 4), (16, 8)],
+                 [(64, 4), (32, 8)]]
 
 
-@pytest.mark.parametrize("L,K", LAYOUTS)
+def layout_class(D):
+    return 0 if D <= 64 else (1 if D <= 128 else 2)
+
+
+@pytest.mark.parametrize("li", [0, 1, 2])
 @pytest.mark.parametrize("D", [1, 5, 16, 32, 63, 64, 100, 128, 200, 256])
-def test_volume_every_layout(L, K, D):
+def test_volume_every_layout(li, D):
     """Every lane layout the autotuner may pick gives the oracle's volume."""
-    if L * K < D:
-        pytest.skip("layout covers fewer disparities than D")
+    lays = CLASS_LAYOUTS[layout_class(D)]
+    if li >= len(lays):
+        pytest.skip("class has two layouts")
+    L, K = lays[li]
     w, h = max(D, 48) + 7, 11
     left, right = random_pair(w, h, seed=L * 31 + K * 7 + D)
     cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=8)
@@ -107,8 +114,9 @@ def test_volume_every_layout(L, K, D):
 def test_unsupported_layout_rejected():
     left, right = random_pair(64, 4, seed=1)
     cfg = vxsgm.SGMStereoConfig(num_disparities=64, num_paths=2)
-    with pytest.raises(vxsgm.InvalidConfig):
-        gpu_volume(O.census(left), O.census(right), cfg, (16, 2))
+    for bad in ((16, 2), (64, 2), (16, 16), (32, 1), (4, 16)):
+        with pytest.raises(vxsgm.InvalidConfig):
+            gpu_volume(O.census(left), O.census(right), cfg, bad)
 
 
 @pytest.mark.parametrize("p1,p2", [(10, 120), (0, 0), (1, 2), (200, 1000),
@@ -213,19 +221,18 @@ def test_concurrent_calls_match_serial(sgm):
         assert_same_disp(a, b)
 
 
-@pytest.mark.parametrize("groups", [2, 4, 8])
+@pytest.mark.parametrize("fmt", ["u8", "u16"])
 @pytest.mark.parametrize("paths", [2, 4, 8])
 @pytest.mark.parametrize("lr", [False, True])
-def test_path_groups_match_oracle(ctx, groups, paths, lr):
-    """Paths run as concurrent passes (one volume each, summed by the
-    winner-take-all) give the same map as the oracle, with and without the LR
-    check."""
+def test_volume_formats_match_oracle(ctx, fmt, paths, lr):
+    """Both per-direction volume formats, forced, give the oracle's map, with
+    and without the LR check (which reads the summed volume)."""
     D = 40
-    left, right, _ = vxsgm.synth_pair(150, 70, D, 5, 1, 700 + 10 * groups + paths)
+    left, right, _ = vxsgm.synth_pair(150, 70, D, 5, 1, 700 + paths + (0 if fmt == "u8" else 50))
     cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=paths, lr_check=lr)
     s = vxsgm.SGMStereo(ctx)
     try:
-        s.set_groups(D, groups)
+        s.set_volume_format(D, fmt)
         got = s.compute_host(left, right, 150, 70, cfg)
     finally:
         s.close()
@@ -233,84 +240,128 @@ def test_path_groups_match_oracle(ctx, groups, paths, lr):
     assert_same_disp(got, want)
 
 
+@pytest.mark.parametrize("p1,p2", [(10, 300), (200, 1000), (3000, 60000), (65535, 65535)])
+def test_u8_volumes_with_large_penalties(ctx, p1, p2):
+    """u8 volumes forced with P2 far above 191: most slices exceed a byte and
+    take the high-byte path, which must still reproduce every u16 sum."""
+    D = 48
+    left, right, _ = vxsgm.synth_pair(120, 50, D, 4, 1, 4000 + p2)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, p1=p1, p2=p2, num_paths=8, lr_check=True)
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        s.set_volume_format(D, "u8")
+        got = s.compute_host(left, right, 120, 50, cfg)
+    finally:
+        s.close()
+    assert_same_disp(got, O.sgm(left, right, D, p1=p1, p2=p2, num_paths=8, lr_check=True))
+
+
+@pytest.mark.parametrize("hi", [0, 1, 2])
+@pytest.mark.parametrize("oi", [0, 1, 2])
+def test_schedules_match_oracle(ctx, hi, oi):
+    """Every (horizontal, other) layout pair of a class gives the oracle's map."""
+    for D, w, h in ((64, 90, 40), (128, 150, 30), (256, 270, 20)):
+        lay = CLASS_LAYOUTS[layout_class(D)]
+        if hi >= len(lay) or oi >= len(lay):
+            continue
+        left, right, _ = vxsgm.synth_pair(w, h, D, 3, 1, 1000 + 10 * hi + oi + D)
+        cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=8)
+        s = vxsgm.SGMStereo(ctx)
+        try:
+            s.set_schedule(D, (lay[hi], lay[oi]))
+            assert s.schedule(D) == (lay[hi], lay[oi])
+            got = s.compute_host(left, right, w, h, cfg)
+        finally:
+            s.close()
+        assert_same_disp(got, O.sgm(left, right, D, num_paths=8))
+
+
 @pytest.mark.parametrize("D,w,h", [(1, 9, 5), (7, 40, 9), (130, 150, 12), (200, 230, 20),
                                    (256, 260, 9)])
-def test_path_groups_disparity_ranges(ctx, D, w, h):
+def test_formats_disparity_ranges(ctx, D, w, h):
     """The summing winner-take-all at slice widths below one chunk, straddling
-    two chunks and at the maximum D, for every group count."""
+    two chunks and at the maximum D, in both volume formats."""
     left, right, _ = vxsgm.synth_pair(w, h, D, 3, 1, 900 + D)
     cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=8)
     want = O.sgm(left, right, D, num_paths=8)
     s = vxsgm.SGMStereo(ctx)
     try:
-        for g in (1, 2, 4, 8):
-            s.set_groups(D, g)
+        for fmt in ("auto", "u8", "u16"):
+            s.set_volume_format(D, fmt)
             assert_same_disp(s.compute_host(left, right, w, h, cfg), want)
     finally:
         s.close()
 
 
-def test_path_groups_enqueue_workspace(ctx):
-    """enqueue uses as many groups as the workspace holds: a one-volume
-    workspace runs ungrouped, a four-volume one grouped; both match."""
+def test_enqueue_workspace(ctx):
+    """enqueue runs in a workspace of vxh_sgm_workspace_bytes (either format)
+    and rejects a smaller one."""
     w, h, D = 128, 64, 48
     left, right, _ = vxsgm.synth_pair(w, h, D, 4, 1, 811)
     cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=8)
     want = O.sgm(left, right, D, num_paths=8)
-    ws1 = vxsgm.workspace_bytes(w, h, cfg)
-    ws4 = vxsgm.workspace_bytes(w, h, cfg, groups=4)
-    assert ws4 > ws1
+    ws = vxsgm.workspace_bytes(w, h, cfg)
     s = vxsgm.SGMStereo(ctx)
     try:
-        s.set_groups(D, 4)
         d_l = hip.DeviceBuffer.from_array(left)
         d_r = hip.DeviceBuffer.from_array(right)
-        for ws in (ws1, ws4):
+        d_ws = hip.DeviceBuffer(ws)
+        st = hip.Stream()
+        for fmt in ("u8", "u16"):
+            s.set_volume_format(D, fmt)
             out = hip.DeviceBuffer(w * h * 4)
             out.fill(0xFF)
-            d_ws = hip.DeviceBuffer(ws)
-            st = hip.Stream()
             s.enqueue(d_l.ptr, d_r.ptr, w, h, cfg, out.ptr, d_ws.ptr, ws, st.handle)
             st.synchronize()
-            st.close()
             assert_same_disp(out.download(np.float32), want)
+        with pytest.raises(vxsgm.InvalidConfig):
+            s.enqueue(d_l.ptr, d_r.ptr, w, h, cfg, out.ptr, d_ws.ptr, ws // 4, st.handle)
+        st.close()
     finally:
         s.close()
 
 
-def test_path_groups_setting_validated(ctx):
+def test_schedule_setting_validated(ctx):
     s = vxsgm.SGMStereo(ctx)
     try:
+        default = s.schedule(64)
         with pytest.raises(vxsgm.InvalidConfig):
-            s.set_groups(64, 3)
+            s.set_schedule(64, (64, 2))
         with pytest.raises(vxsgm.InvalidConfig):
-            s.set_groups(64, 16)
+            s.set_schedule(128, ((64, 2), (8, 8)))
         with pytest.raises(vxsgm.InvalidConfig):
-            s.set_groups(0, 2)
-        s.set_groups(64, 0)
+            s.set_schedule(0, (32, 2))
+        with pytest.raises(vxsgm.InvalidConfig):
+            s.set_volume_format(64, "u32")
+        s.set_schedule(64, (16, 4))
+        assert s.schedule(64) == ((16, 4), (16, 4))
+        s.set_schedule(64, None)
+        assert s.schedule(64) == default
     finally:
         s.close()
 
 
 def test_720p_d128_8path_bit_exact(ctx, sgm):
-    """Workload 720p_d128_8path at full size against the threaded oracle, with
-    the paths run one after another and as concurrent passes."""
+    """Workload 720p_d128_8path at full size against the threaded oracle, in
+    both volume formats and with the narrowest and widest layouts."""
     left, right, _, cfg = vxsgm.workload_pair("720p_d128_8path")
     want = O.sgm(left, right, 128, num_paths=8, nthreads=16)
     got = sgm.compute_host(left, right, 1280, 720, cfg)
     assert_same_disp(got, want)
     s = vxsgm.SGMStereo(ctx)
     try:
-        for g in (2, 4, 8):
-            s.set_groups(128, g)
-            assert_same_disp(s.compute_host(left, right, 1280, 720, cfg), want)
+        s.set_volume_format(128, "u16")
+        assert_same_disp(s.compute_host(left, right, 1280, 720, cfg), want)
+        s.set_volume_format(128, "auto")
+        s.set_schedule(128, ((32, 4), (64, 2)))
+        assert_same_disp(s.compute_host(left, right, 1280, 720, cfg), want)
     finally:
         s.close()
 
 
 def test_1080p_d256_8path_matches_golden_hash(ctx, sgm):
     """Workload 1080p_d256_8path: the GPU map's SHA-256 equals the oracle's,
-    recorded in tests/golden/workload_hashes.json."""
+    recorded in tests/golden/workload_hashes.json, in both volume formats."""
     with open(os.path.join(GOLDEN, "workload_hashes.json")) as f:
         golden = json.load(f)
     left, right, _, cfg = vxsgm.workload_pair("1080p_d256_8path")
@@ -318,7 +369,7 @@ def test_1080p_d256_8path_matches_golden_hash(ctx, sgm):
     assert hashlib.sha256(bits(got).tobytes()).hexdigest() == golden["1080p_d256_8path"]
     s = vxsgm.SGMStereo(ctx)
     try:
-        s.set_groups(256, 8)
+        s.set_volume_format(256, "u16")
         got = s.compute_host(left, right, 1920, 1080, cfg)
     finally:
         s.close()

@@ -12,7 +12,7 @@ from vxsgm import hip
 pytestmark = pytest.mark.gpu
 
 
-def run_batch(pairs, scheds=None, replays=1):
+def run_batch(pairs, scheds=None, replays=1, formats=None):
     """pairs: [(left, right, cfg)] host arrays.  Returns list of host maps."""
     bufs, descs = [], []
     for i, (l, r, cfg) in enumerate(pairs):
@@ -21,7 +21,8 @@ def run_batch(pairs, scheds=None, replays=1):
         do = hip.DeviceBuffer(h * w * 4)
         bufs.append((dl, dr, do))
         sched = scheds[i] if scheds else None
-        descs.append((dl.ptr, dr.ptr, do.ptr, w, h, cfg, sched))
+        fmt = formats[i] if formats else "auto"
+        descs.append((dl.ptr, dr.ptr, do.ptr, w, h, cfg, sched, fmt))
     b = vxsgm.SGMBatch(descs)
     outs = []
     for _ in range(replays):
@@ -38,29 +39,58 @@ def bits(a):
 
 
 def test_batch_small_mix_matches_oracle():
-    pairs = []
+    """Mixed sizes, D, path counts, LR checks, volume formats (u8 with a P2
+    that overflows bytes, u16) and penalties, over repeated replays."""
+    pairs, formats = [], []
     for i, (w, h, d, paths) in enumerate([(64, 40, 16, 2), (100, 31, 48, 4), (80, 80, 64, 8),
-                                          (129, 17, 100, 8), (300, 20, 256, 8), (33, 3, 8, 8)]):
+                                          (129, 17, 100, 8), (300, 20, 256, 8), (33, 3, 8, 8),
+                                          (90, 30, 32, 8), (70, 25, 24, 4)]):
         l, r = random_pair(w, h, seed=100 + i)
-        cfg = vxsgm.SGMStereoConfig(num_disparities=d, num_paths=paths,
+        p2 = 900.0 if i >= 6 else 120.0
+        cfg = vxsgm.SGMStereoConfig(num_disparities=d, num_paths=paths, p2=p2,
                                     lr_check=(i % 2 == 1))
         pairs.append((l, r, cfg))
-    outs = run_batch(pairs, replays=3)
-    for rep in outs:
-        for (l, r, cfg), got in zip(pairs, rep):
-            want = O.sgm(l, r, cfg.num_disparities, num_paths=cfg.num_paths,
-                         lr_check=cfg.lr_check, lr_tol=cfg.lr_tolerance)
-            assert (bits(got) == bits(want.ravel())).all()
+        formats.append(["auto", "u16", "auto", "u16", "auto", "u8", "u8", "u16"][i])
+    outs = run_batch(pairs, replays=3, formats=formats)
+    bad = []
+    for k, rep in enumerate(outs):
+        for i, ((l, r, cfg), got) in enumerate(zip(pairs, rep)):
+            want = O.sgm(l, r, cfg.num_disparities, p2=cfg.p2, num_paths=cfg.num_paths,
+                         lr_check=cfg.lr_check, lr_tol=cfg.lr_tolerance).ravel()
+            diff = bits(got) != bits(want)
+            if diff.any():
+                idx = np.flatnonzero(diff)[:4]
+                bad.append(f"replay {k} pair {i} ({l.shape[1]}x{l.shape[0]} D="
+                           f"{cfg.num_disparities} P={cfg.num_paths} lr={cfg.lr_check} "
+                           f"{formats[i]}): {int(diff.sum())} px, at {idx.tolist()} got "
+                           f"{got[idx].tolist()} want {want[idx].tolist()}")
+    assert not bad, "\n".join(bad)
 
 
-@pytest.mark.parametrize("sched", [(64, 2), (32, 4), (16, 8)])
-def test_batch_schedules_equal_plain(sgm, sched):
+@pytest.mark.parametrize("opts", [None, (1, 1, 256), (2, 4, 4096)])
+def test_batch_launch_schedules_equal_plain(ctx, opts):
+    """Every launch schedule the tuner tries gives the plain path's maps, and
+    the plain path gives the same maps under each of its lane layouts."""
     pairs = [vxsgm.mixture_pair(256, 96, 128, s)[:2] for s in range(3)]
     cfg = vxsgm.SGMStereoConfig(num_disparities=128, num_paths=8)
-    outs = run_batch([(l, r, cfg) for l, r in pairs], scheds=[sched] * 3)[0]
-    for (l, r), got in zip(pairs, outs):
-        plain = sgm.compute_host(l, r, 256, 96, cfg)
-        assert (bits(got) == bits(plain)).all()
+    bufs, descs = [], []
+    for l, r in pairs:
+        dl, dr = hip.DeviceBuffer.from_array(l), hip.DeviceBuffer.from_array(r)
+        do = hip.DeviceBuffer(256 * 96 * 4)
+        bufs.append((dl, dr, do))
+        descs.append((dl.ptr, dr.ptr, do.ptr, 256, 96, cfg, (16, 8)))
+    b = vxsgm.SGMBatch(descs, options=opts)
+    b.run()
+    outs = [do.download(np.float32) for _, _, do in bufs]
+    b.close()
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        for sched in (((64, 2), (16, 8)), ((32, 4), (32, 4))):
+            s.set_schedule(128, sched)
+            for (l, r), got in zip(pairs, outs):
+                assert (bits(got) == bits(s.compute_host(l, r, 256, 96, cfg))).all()
+    finally:
+        s.close()
 
 
 def test_batch_mixture_equals_plain(sgm):
@@ -86,5 +116,8 @@ def test_batch_rejects_bad_pair():
     with pytest.raises(vxsgm.InvalidConfig):
         vxsgm.SGMBatch([(l.ptr, l.ptr, o.ptr, 64, 8, bad, None)])
     ok = vxsgm.SGMStereoConfig(num_disparities=64)
+    for sched in ((16, 2), (32, 2), ((8, 8), (16, 4))):
+        with pytest.raises(vxsgm.InvalidConfig):
+            vxsgm.SGMBatch([(l.ptr, l.ptr, o.ptr, 64, 8, ok, sched)])
     with pytest.raises(vxsgm.InvalidConfig):
-        vxsgm.SGMBatch([(l.ptr, l.ptr, o.ptr, 64, 8, ok, (16, 2))])
+        vxsgm.SGMBatch([(l.ptr, l.ptr, o.ptr, 64, 8, ok, None)], options=(3, 2, 1024))

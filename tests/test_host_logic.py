@@ -16,20 +16,22 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 @pytest.mark.parametrize("D", [1, 16, 33, 64, 65, 128, 129, 200, 256])
 def test_tuner_candidates_cover_d(D):
     c = tuner.candidates(D)
-    assert c, f"no layout for D={D}"
+    assert len(c) == (2 if D > 128 else 3), f"no layouts for D={D}"
     for L, K in c:
-        assert L * K >= D and (L, K) in tuner.LAYOUTS
+        assert L * K >= D and L * K in (64, 128, 256) and 2 <= K <= 8
+    s = tuner.schedules(D)
+    assert len(s) == len(c) ** 2 and all(a in c and b in c for a, b in s)
 
 
 def test_tuner_cache_roundtrip(tmp_path):
     p = tmp_path / "t.json"
     t = tuner.Tuner(str(p))
-    t.table["batch:64x48:D16:P8"] = {"schedule": [16, 4], "ms": {"16x4": 1.0}}
+    t.table["batch:64x48:D16:P8"] = {"schedule": "32x2/8x8", "ms": {"32x2/8x8": 1.0}}
     t.save()
     t2 = tuner.Tuner(str(p))
-    assert t2.lookup("batch", 64, 48, 16, 8) == (16, 4)
+    assert t2.lookup("batch", 64, 48, 16, 8) == ((32, 2), (8, 8))
     assert t2.lookup("plain", 64, 48, 16, 8) is None
-    assert json.loads(p.read_text())["batch:64x48:D16:P8"]["schedule"] == [16, 4]
+    assert json.loads(p.read_text())["batch:64x48:D16:P8"]["schedule"] == "32x2/8x8"
 
 
 def test_mixture_is_seeded_and_in_range():

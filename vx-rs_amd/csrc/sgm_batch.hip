@@ -5,17 +5,19 @@
 // sgm.rs:135-261) and its Pipeline batches encodes of different kernels into
 // one command buffer (vx-vision/src/pipeline.rs:12-80).  Here a batch is a
 // work list of wave-sized items, each a slice of one pair's stage:
-//   phase 0            census rows of the left or right image
-//   phase 1 .. P       scanline groups of path direction i-1 (P = num_paths)
-//   phase P + 1        winner-take-all pixels
-//   phase P + 2        left-right check pixels (only with lr_check)
-// Items are ordered phase-major across pairs.  A wave takes the next item
-// from a global counter, waits until every item of the same pair's previous
-// phase has signalled completion, runs the slice with the same device code as
-// the per-pair kernels, and signals.  Every item depends only on items earlier
-// in the list, and an item is taken only by a running wave, so the waits
-// always end.  The summed volume of a pair is updated by one direction at a
-// time, exactly as in the per-pair path, so the output is bit-identical.
+//   phase 0   census rows of the left or right image
+//   phase 1   scanline groups of any path direction (the directions of a
+//             pair are independent: each writes its own volume)
+//   phase 2   sum of the direction volumes and winner-take-all, pixel ranges
+//   phase 3   left-right check pixels (only with lr_check)
+// A wave takes the next item from a global counter, waits until every item of
+// the same pair's previous phase has signalled completion, runs the slice with
+// the same device code as the per-pair kernels, and signals.  Every item
+// depends only on items earlier in the list, and an item is taken only by a
+// running wave, so the waits always end.  Items are ordered so that the
+// memory-bound sums of finished pairs run beside the aggregation of later
+// pairs.  Per pair the arithmetic is exactly the per-pair path's, so the
+// output is bit-identical.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,7 +25,7 @@
 #include <string>
 #include <vector>
 
-#include "sgm_device.h"
+#include "sgm_paths.h"
 #include "vxh/sgm.h"
 #include "vxh/sgm_batch.h"
 
@@ -32,20 +34,23 @@ namespace {
 
 using namespace dev;
 
-constexpr int kMaxPhase = 11;   // census + 8 paths + wta + lr
+constexpr int kMaxPhase = 4;   // census, paths, wta, lr
 
 enum ItemKind : int { kCensusL = 0, kCensusR = 1, kAgg = 2, kWta = 3, kLr = 4 };
 
 struct PairDesc {
     const uint8_t *left, *right;
     float *out;
-    uint2 *lc, *rc;
-    uint16_t *vol;
-    int16_t *best;
-    int w, h, D, Dp;
-    uint32_t p1, p2;
+    uint2 *lc, *rc, *flags;  // flags: kOutU8 only
+    uint8_t *vol, *hi;       // direction k's volume at vol + k * vol_bytes (hi likewise)
+    size_t vol_bytes;
+    uint16_t *sum;           // lr_check only: summed volume
+    int16_t *best;           // lr_check only: integer winners
+    int w, h, D, Dp, P, mode;
+    uint32_t p1x2, p2x2;
     int rx, ry;
-    int layout;   // index into kBatchLayouts
+    int cls;                 // layout class
+    int lr_tol;
 };
 
 struct Item {
@@ -53,14 +58,9 @@ struct Item {
     int kind;
     int dir;      // path index for kAgg
     int first;    // first row / scanline / pixel
-    int count;    // rows / pixels (scanlines: one wave's group)
+    int count;    // rows / pixels
     int phase;
 };
-
-// Layouts the persistent kernel can run, by index.
-constexpr int kBatchLayouts[][2] = {{64, 1}, {64, 2}, {64, 4}, {32, 2}, {32, 4},
-                                    {32, 8}, {16, 4}, {16, 8}, {16, 16}};
-constexpr int kNumBatchLayouts = sizeof(kBatchLayouts) / sizeof(kBatchLayouts[0]);
 
 __device__ __forceinline__ int dir_x(int i) {
     return i == 0 || i == 4 || i == 6 ? 1 : (i == 2 || i == 5 || i == 7 ? -1 : 0);
@@ -106,6 +106,21 @@ __device__ __forceinline__ bool wave_wait(const uint32_t *ctr, uint32_t target,
     return false;
 }
 
+// The batch runs the layout with 8 disparities per lane in every class (the
+// most disparities per lane within the register budget; a batch is bound by
+// instruction throughput, not by its longest scanline).  The persistent kernel
+// carries one aggregation body per class, so that register use stays at two
+// waves per SIMD.
+constexpr int kBatchLayout[kLayoutClasses] = {2, 2, 1};
+
+template <int C>
+__device__ __forceinline__ void agg_item(const PassParams &p, int first) {
+    constexpr int L = kClassLayouts[C][kBatchLayout[C]][0];
+    constexpr int K = kClassLayouts[C][kBatchLayout[C]][1];
+    static_assert(K == 8, "batch layouts hold 8 disparities per lane");
+    agg_pair_wave<L, K, -1>(p, first);
+}
+
 __global__ void __launch_bounds__(256)
 batch_kernel(const PairDesc *__restrict__ pairs, const Item *__restrict__ items,
              int nitems, uint32_t *next, uint32_t *done,
@@ -123,42 +138,47 @@ batch_kernel(const PairDesc *__restrict__ pairs, const Item *__restrict__ items,
         if (it.kind == kCensusL || it.kind == kCensusR) {
             const uint8_t *img = it.kind == kCensusL ? pd.left : pd.right;
             uint2 *out = it.kind == kCensusL ? pd.lc : pd.rc;
+            uint2 *fl = it.kind == kCensusL ? pd.flags : nullptr;
             for (int y = it.first; y < it.first + it.count; ++y)
-                for (int x = lane; x < pd.w; x += 64)
-                    out[(size_t)y * pd.w + x] = census_pixel(img, pd.w, pd.h, pd.rx, pd.ry, x, y);
+                for (int x = lane; x < pd.w; x += 64) {
+                    const size_t i = (size_t)y * pd.w + x;
+                    out[i] = census_pixel(img, pd.w, pd.h, pd.rx, pd.ry, x, y);
+                    if (fl) fl[i] = make_uint2(0, 0);
+                }
         } else if (it.kind == kAgg) {
-            AggParams p;
+            PassParams p;
             p.lc = pd.lc;
             p.rc = pd.rc;
-            p.vol = pd.vol;
+            p.vol = pd.vol + (size_t)it.dir * pd.vol_bytes;
+            p.hi = pd.hi ? pd.hi + (size_t)it.dir * pd.vol_bytes : nullptr;
+            p.flags = (uint8_t *)pd.flags;
             p.w = pd.w;
             p.h = pd.h;
             p.D = pd.D;
             p.Dp = pd.Dp;
-            p.p1 = pd.p1;
-            p.p2 = pd.p2;
+            p.p1x2 = pd.p1x2;
+            p.p2x2 = pd.p2x2;
             p.dir = it.dir;
+            p.dx = dir_x(it.dir);
+            p.dy = dir_y(it.dir);
             p.nscan = dir_nscan(it.dir, pd.w, pd.h);
-            p.accumulate = it.dir > 0;
-            p.wta_out = it.count < 0 ? pd.out : nullptr;
-            const int dx = dir_x(it.dir), dy = dir_y(it.dir);
-            switch (pd.layout) {
-            case 0: agg_wave<64, 1>(p, dx, dy, it.first); break;
-            case 1: agg_wave<64, 2>(p, dx, dy, it.first); break;
-            case 2: agg_wave<64, 4>(p, dx, dy, it.first); break;
-            case 3: agg_wave<32, 2>(p, dx, dy, it.first); break;
-            case 4: agg_wave<32, 4>(p, dx, dy, it.first); break;
-            case 5: agg_wave<32, 8>(p, dx, dy, it.first); break;
-            case 6: agg_wave<16, 4>(p, dx, dy, it.first); break;
-            case 7: agg_wave<16, 8>(p, dx, dy, it.first); break;
-            default: agg_wave<16, 16>(p, dx, dy, it.first); break;
-            }
+            p.mode = pd.mode;
+            if (pd.cls == 0) agg_item<0>(p, it.first);
+            else if (pd.cls == 1) agg_item<1>(p, it.first);
+            else agg_item<2>(p, it.first);
         } else if (it.kind == kWta) {
-            for (int i = it.first + lane; i < it.first + it.count; i += 64)
-                wta_pixel(pd.vol, pd.D, pd.Dp, (size_t)i, pd.out, pd.best);
+            // Four pixels per wave iteration, 16 lanes each.
+            const int npx = pd.w * pd.h;
+            const int end = it.first + it.count;
+            for (int base = it.first; base < end; base += 4) {
+                const int pix = base + (lane >> 4);
+                wta_sum_pixel(pd.vol, pd.vol_bytes, pd.hi, pd.flags, pd.P, pd.Dp > 128 ? 2 : 1,
+                              pd.mode, pix < end ? pix : npx, npx, pd.D, pd.Dp, pd.out, pd.best,
+                              pd.sum);
+            }
         } else {
             for (int i = it.first + lane; i < it.first + it.count; i += 64)
-                lr_pixel(pd.vol, pd.w, pd.D, pd.Dp, it.dir, (size_t)i, pd.best, pd.out);
+                lr_pixel(pd.sum, pd.w, pd.D, pd.Dp, pd.lr_tol, (size_t)i, pd.best, pd.out);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_fetch_add(done + it.pair * kMaxPhase + it.phase, lane == 0 ? 1u : 0u,
@@ -166,9 +186,10 @@ batch_kernel(const PairDesc *__restrict__ pairs, const Item *__restrict__ items,
     }
 }
 
-int layout_index(int L, int K) {
-    for (int i = 0; i < kNumBatchLayouts; ++i)
-        if (kBatchLayouts[i][0] == L && kBatchLayouts[i][1] == K) return i;
+// Index of (L, K) among the layouts of class c, -1 if none.
+int layout_index(int c, int L, int K) {
+    for (int i = 0; i < kClassCount[c]; ++i)
+        if (kClassLayouts[c][i][0] == L && kClassLayouts[c][i][1] == K) return i;
     return -1;
 }
 
@@ -209,7 +230,21 @@ const char *vxh_sgm_batch_last_error(void) { return g_batch_error.c_str(); }
 
 vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs, uint32_t n,
                                 vxh_sgm_batch **out) {
+    return vxh_sgm_batch_create_ex(device, pairs, n, nullptr, out);
+}
+
+vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, uint32_t n,
+                                   const vxh_sgm_batch_options *opt, vxh_sgm_batch **out) {
     using namespace vxh;
+    vxh_sgm_batch_options o{2, 2, 1024};
+    if (opt) {
+        if (opt->blocks_per_cu > 2 || opt->wta_pixels % 4 != 0 || opt->wta_pixels > (1u << 20) ||
+            opt->wta_lag > 64)
+            return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: batch options");
+        if (opt->blocks_per_cu) o.blocks_per_cu = opt->blocks_per_cu;
+        if (opt->wta_lag) o.wta_lag = opt->wta_lag;
+        if (opt->wta_pixels) o.wta_pixels = opt->wta_pixels;
+    }
     if (!out || (!pairs && n)) return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: null argument");
     *out = nullptr;
     if (n == 0 || n > 65535) return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: batch size");
@@ -226,30 +261,46 @@ vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs, uint
                          std::string("pair ") + std::to_string(i) + ": " + vxh_last_error());
         if (!q.d_left || !q.d_right || !q.d_out)
             return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: null pair buffer");
+        if (q.volume_format > 2)
+            return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: volume format");
         const int D = (int)q.cfg.num_disparities;
-        Layout lay = (q.lanes == 0 && q.per_lane == 0) ? default_layout(D)
-                                                       : Layout{(int)q.lanes, (int)q.per_lane};
-        const int li = layout_index(lay.L, lay.K);
-        if (li < 0 || lay.L * lay.K < D)
-            return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
-        const size_t npx = (size_t)q.width * q.height;
-        const int Dp = volume_stride(D);
+        const int cls = layout_class(D);
+        const int li = kBatchLayout[cls];
+        const bool sched_ok =
+            ((q.lanes == 0 && q.per_lane == 0) || layout_index(cls, (int)q.lanes, (int)q.per_lane) == li) &&
+            ((q.lanes_h == 0 && q.per_lane_h == 0) ||
+             layout_index(cls, (int)q.lanes_h, (int)q.per_lane_h) == li);
+        if (!sched_ok)
+            return bfail(VXH_ERR_INVALID_CONFIG,
+                         "invalid config: unsupported lane layout (the batch runs " +
+                             std::to_string(kClassLayouts[cls][li][0]) + "x" +
+                             std::to_string(kClassLayouts[cls][li][1]) + " for D=" +
+                             std::to_string(D) + ")");
+        const int mode = q.volume_format == 1 ? kOutU8
+                         : q.volume_format == 2 ? kOutU16
+                                                : auto_volume_mode(q.cfg.p2);
+        const WsLayout L = workspace_layout(q.width, q.height, (uint32_t)D, (int)q.cfg.num_paths,
+                                            mode, q.cfg.lr_check != 0);
         off[i] = slab;
-        slab += census_span(npx) + align256(npx * 8) + align256(npx * (size_t)Dp * 2) +
-                align256(npx * 2);
+        slab += ws_align(L.total);
         PairDesc &d = pd[i];
         d.left = q.d_left;
         d.right = q.d_right;
         d.out = q.d_out;
+        d.vol_bytes = L.vol_bytes;
         d.w = (int)q.width;
         d.h = (int)q.height;
         d.D = D;
-        d.Dp = Dp;
-        d.p1 = (uint32_t)q.cfg.p1 & 0xFFFFu;
-        d.p2 = (uint32_t)q.cfg.p2 & 0xFFFFu;
+        d.Dp = volume_stride(D);
+        d.P = (int)q.cfg.num_paths;
+        d.mode = mode;
+        const uint32_t p1 = (uint32_t)q.cfg.p1 & 0xFFFFu, p2 = (uint32_t)q.cfg.p2 & 0xFFFFu;
+        d.p1x2 = p1 | (p1 << 16);
+        d.p2x2 = p2 | (p2 << 16);
         d.rx = (int)q.cfg.census_radius_x;
         d.ry = (int)q.cfg.census_radius_y;
-        d.layout = li;
+        d.cls = cls;
+        d.lr_tol = (int)std::min<uint32_t>(q.cfg.lr_tolerance, 1024u);
     }
 
     vxh_sgm_batch *b = new vxh_sgm_batch();
@@ -262,15 +313,24 @@ vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs, uint
     }
     for (uint32_t i = 0; i < n; ++i) {
         char *base = (char *)b->slab + off[i];
-        const size_t npx = (size_t)pd[i].w * pd[i].h;
-        const size_t cs = census_span(npx) + align256(npx * 8);
-        pd[i].lc = (uint2 *)base;
-        pd[i].rc = (uint2 *)(base + census_span(npx));
-        pd[i].vol = (uint16_t *)(base + cs);
-        pd[i].best = (int16_t *)(base + cs + align256(npx * (size_t)pd[i].Dp * 2));
+        const WsLayout L = workspace_layout(pairs[i].width, pairs[i].height,
+                                            (uint32_t)pd[i].D, pd[i].P, pd[i].mode,
+                                            pairs[i].cfg.lr_check != 0);
+        const bool u8 = pd[i].mode == kOutU8, lr = pairs[i].cfg.lr_check != 0;
+        pd[i].lc = (uint2 *)(base + L.lc);
+        pd[i].rc = (uint2 *)(base + L.rc);
+        pd[i].flags = u8 ? (uint2 *)(base + L.flags) : nullptr;
+        pd[i].vol = (uint8_t *)(base + L.vol);
+        pd[i].hi = u8 ? (uint8_t *)(base + L.hi) : nullptr;
+        pd[i].sum = lr ? (uint16_t *)(base + L.sum) : nullptr;
+        pd[i].best = lr ? (int16_t *)(base + L.best) : nullptr;
     }
 
-    // Work list, phase-major; within a phase, larger pairs first.
+    // Work list.  Census of every pair first; then, pair by pair (largest
+    // first), the aggregation items (horizontal directions, the longest
+    // scanlines, first), each pair's sum + winner items placed two pairs
+    // later so that they find its directions finished and run beside the
+    // next pairs' aggregation; the LR checks last.
     std::vector<int> order(n);
     for (uint32_t i = 0; i < n; ++i) order[i] = (int)i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
@@ -283,46 +343,46 @@ vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs, uint
         items.push_back(Item{p, kind, dir, first, count, phase});
         need[(size_t)p * kMaxPhase + phase]++;
     };
-    int maxphase = 0;
-    for (uint32_t i = 0; i < n; ++i)
-        maxphase = std::max(maxphase, (int)pairs[i].cfg.num_paths + 2 +
-                                          (pairs[i].cfg.lr_check ? 1 : 0));
-    for (int phase = 0; phase < maxphase; ++phase) {
-        for (int p : order) {
-            const vxh_sgm_pair_desc &q = pairs[p];
-            const int P = (int)q.cfg.num_paths;
-            const PairDesc &d = pd[p];
-            if (phase == 0) {
-                const int rows = std::max(1, 4096 / d.w);
-                for (int y = 0; y < d.h; y += rows) {
-                    push(p, kCensusL, 0, y, std::min(rows, d.h - y), 0);
-                    push(p, kCensusR, 0, y, std::min(rows, d.h - y), 0);
-                }
-            } else if (phase <= P) {
-                const int dir = phase - 1;
-                const int G = 64 / kBatchLayouts[d.layout][0];
-                const int ns = num_scanlines(dir, d.w, d.h);
-                // Without the LR check the last path also picks the winner
-                // (count -1), so the pair needs no separate WTA phase.
-                const int cnt = (dir == P - 1 && !q.cfg.lr_check) ? -1 : G;
-                for (int s = 0; s < ns; s += G) push(p, kAgg, dir, s, cnt, phase);
-            } else if (phase == P + 1 && q.cfg.lr_check) {
-                const int npx = d.w * d.h, chunk = 64 * 64;
-                for (int i = 0; i < npx; i += chunk)
-                    push(p, kWta, 0, i, std::min(chunk, npx - i), phase);
-            } else if (phase == P + 2 && q.cfg.lr_check) {
-                const int npx = d.w * d.h, chunk = 64 * 64;
-                for (int i = 0; i < npx; i += chunk)
-                    push(p, kLr, (int)std::min<uint32_t>(q.cfg.lr_tolerance, 1024u), i,
-                         std::min(chunk, npx - i), phase);
-            }
+    const int px_item = (int)o.wta_pixels;
+    auto push_wta = [&](int p) {
+        const int npx = pd[p].w * pd[p].h;
+        for (int i = 0; i < npx; i += px_item)
+            push(p, kWta, 0, i, std::min(px_item, npx - i), 2);
+    };
+    for (int p : order) {
+        const PairDesc &d = pd[p];
+        const int rows = std::max(1, 4096 / d.w);
+        for (int y = 0; y < d.h; y += rows) {
+            push(p, kCensusL, 0, y, std::min(rows, d.h - y), 0);
+            push(p, kCensusR, 0, y, std::min(rows, d.h - y), 0);
         }
+    }
+    const size_t kWtaLag = o.wta_lag;
+    for (size_t k = 0; k < order.size(); ++k) {
+        const int p = order[k];
+        const PairDesc &d = pd[p];
+        for (int pass = 0; pass < 2; ++pass)
+            for (int dir = 0; dir < d.P; ++dir) {
+                const bool horiz = kDirY[dir] == 0;
+                if (horiz != (pass == 0)) continue;
+                const int per = 128 / kClassLayouts[d.cls][kBatchLayout[d.cls]][0];
+                const int ns = num_scanlines(dir, d.w, d.h);
+                for (int s0 = 0; s0 < ns; s0 += per) push(p, kAgg, dir, s0, per, 1);
+            }
+        if (k >= kWtaLag) push_wta(order[k - kWtaLag]);
+    }
+    for (size_t k = order.size() >= kWtaLag ? order.size() - kWtaLag : 0; k < order.size(); ++k)
+        push_wta(order[k]);
+    for (int p : order) {
+        if (!pairs[p].cfg.lr_check) continue;
+        const int npx = pd[p].w * pd[p].h, chunk = 64 * 64;
+        for (int i = 0; i < npx; i += chunk) push(p, kLr, 0, i, std::min(chunk, npx - i), 3);
     }
     for (uint32_t i = 0; i < n; ++i) {
         b->stats[i].items = 0;
         for (int ph = 0; ph < kMaxPhase; ++ph) b->stats[i].items += need[(size_t)i * kMaxPhase + ph];
-        b->stats[i].lanes = (uint32_t)kBatchLayouts[pd[i].layout][0];
-        b->stats[i].per_lane = (uint32_t)kBatchLayouts[pd[i].layout][1];
+        b->stats[i].lanes = (uint32_t)kClassLayouts[pd[i].cls][kBatchLayout[pd[i].cls]][0];
+        b->stats[i].per_lane = (uint32_t)kClassLayouts[pd[i].cls][kBatchLayout[pd[i].cls]][1];
     }
     b->nitems = (int)items.size();
     b->ncounters = 2 + (int)n * kMaxPhase;
@@ -353,8 +413,9 @@ vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs, uint
         vxh_sgm_batch_destroy(b);
         return bfail(VXH_ERR_GPU, std::string("GPU error: batch setup: ") + hipGetErrorString(e));
     }
-    // Persistent grid: 8 waves per CU (two 256-thread workgroups).
-    b->grid = std::max(1, cus) * 2;
+    // Persistent grid: blocks_per_cu 256-thread workgroups per CU (the kernel's
+    // registers allow two).
+    b->grid = std::max(1, cus) * (int)o.blocks_per_cu;
 
     // Capture [reset counters, persistent kernel] once; replays reuse it.
     e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);

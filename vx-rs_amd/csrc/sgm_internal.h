@@ -27,12 +27,6 @@ inline int num_scanlines(int dir, int w, int h) {
 // packed segment of it never straddles a pixel.
 inline int volume_stride(int D) { return (D + 15) & ~15; }
 
-// Lane layout of the aggregation kernel: L lanes own one scanline, each lane
-// K consecutive disparities (L * K >= D).
-struct Layout {
-    int L, K;
-};
-
 // The aggregation reads the right census at x - d for every d of a lane
 // through one base kCensusGuard bytes before rc; the reads with x - d < 0
 // (whose cost is replaced by 64) then land in the kCensusGuard readable bytes
@@ -46,18 +40,79 @@ inline size_t census_span(size_t npx) {
     return (b + 255) & ~(size_t)255;
 }
 
-struct AggParams {
+// Storage of the per-direction path volumes (entries pix*Dp + d).
+constexpr int kOutU8 = 0;    // low byte, plus high bytes of flagged slices
+constexpr int kOutU16 = 1;   // full u16 path cost
+
+// One direction pass of the packed-pair aggregation.
+struct PassParams {
     const uint2 *lc;       // left census, w*h (lo, hi)
     const uint2 *rc;       // right census, kCensusGuard readable bytes before it
-    uint16_t *vol;         // summed volume, (y*w + x)*Dp + d
+    uint8_t *vol;          // this direction's volume
+    uint8_t *hi;           // kOutU8: high bytes of flagged slices (1 byte per entry)
+    uint8_t *flags;        // kOutU8: 8 bytes per pixel, byte dir = 1 when flagged
     int w, h, D, Dp;
-    uint32_t p1, p2;       // penalties already truncated to u16
-    int dir;
-    int nscan;
-    int accumulate;        // 0: store (first path), 1: add mod 2^16
-    float *wta_out;        // non-null on the last path: winner-take-all is
-                           // computed from the final sums instead of storing them
+    uint32_t p1x2, p2x2;   // penalties truncated to u16, in both halves
+    int dir, dx, dy, nscan;
+    int mode;              // kOutU8 / kOutU16 (read only where the kernel takes it at run time)
 };
+
+// Lane layouts of the aggregation: layout class c covers D <= 64 << c with
+// the (L, K) choices kClassLayouts[c][0 .. kClassCount[c]), L lanes owning a
+// scanline pair and K disparities each (L * K = 64 << c).  (16, 16) is left
+// out: its census ring alone needs more than 128 registers.
+constexpr int kLayoutClasses = 3;
+constexpr int kLayoutsPerClass = 3;
+constexpr int kClassCount[kLayoutClasses] = {3, 3, 2};
+constexpr int kClassLayouts[kLayoutClasses][kLayoutsPerClass][2] = {
+    {{32, 2}, {16, 4}, {8, 8}},
+    {{64, 2}, {32, 4}, {16, 8}},
+    {{64, 4}, {32, 8}, {32, 8}},
+};
+inline int layout_class(int D) { return D <= 64 ? 0 : (D <= 128 ? 1 : 2); }
+
+// Lane layout of one direction class of a pair: horizontal passes (the
+// longest scanlines) and the other directions are laid out separately.
+struct Schedule {
+    int horiz;   // index into kClassLayouts[layout_class(D)]
+    int other;
+};
+
+constexpr size_t kWsAlign = 256;
+inline size_t ws_align(size_t v) { return (v + kWsAlign - 1) & ~(kWsAlign - 1); }
+
+// Scratch of one pair, byte offsets from its base: census of both images, the
+// exception flags (u8 volumes only, 8 bytes per pixel), one volume per path
+// direction (u8 or u16 entries, vol_bytes apart), the high-byte volumes (u8
+// only, also vol_bytes apart), and for the LR check the summed u16 volume and
+// the winner index.
+struct WsLayout {
+    size_t lc, rc, flags, vol, vol_bytes, hi, sum, best, total;
+};
+inline WsLayout workspace_layout(uint32_t w, uint32_t h, uint32_t D, int paths, int mode,
+                                 bool lr) {
+    WsLayout l;
+    const size_t n = (size_t)w * h;
+    const size_t Dp = (size_t)volume_stride((int)D);
+    const bool u8 = mode == kOutU8;
+    l.lc = 0;
+    l.rc = l.lc + census_span(n);
+    l.flags = l.rc + ws_align(n * 8);
+    l.vol = l.flags + (u8 ? ws_align(n * 8) : 0);
+    l.vol_bytes = ws_align(n * Dp * (u8 ? 1 : 2));
+    l.hi = l.vol + (size_t)paths * l.vol_bytes;
+    l.sum = l.hi + (u8 ? (size_t)paths * l.vol_bytes : 0);
+    l.best = l.sum + (lr ? ws_align(n * Dp * 2) : 0);
+    l.total = l.best + (lr ? ws_align(n * 2) : 0);
+    return l;
+}
+
+// Volume format a configuration selects by default: u8 entries whenever
+// every path cost that has not wrapped fits a byte (costs never exceed
+// 64 + P2, SGMStereo.metal:154-160), u16 otherwise.
+inline int auto_volume_mode(float p2) {
+    return ((uint32_t)p2 & 0xFFFFu) + 64u <= 255u ? kOutU8 : kOutU16;
+}
 
 // Several independent direction passes in one launch (each into its own
 // volume).  Blocks [first_block, first_block + nblocks) belong to a pass;
@@ -66,37 +121,37 @@ struct AggParams {
 // of the pass's scanlines and shares their census rows in its L2).
 constexpr int kMaxPasses = 8;
 struct PassDesc {
-    uint16_t *vol;
+    uint8_t *vol, *hi;
     int dir, dx, dy, nscan;
-    int accumulate;
+    int layout;
     int first_block, nblocks;
 };
-struct MultiAgg {
-    AggParams base;        // shared fields; vol/dir/nscan/accumulate come per pass
+struct MultiPass {
+    PassParams base;       // shared fields; vol/hi/dir/dx/dy/nscan come per pass
     int npass;
     PassDesc pass[kMaxPasses];
 };
 
+// Census of one image; with `zero_flags` also clears the 8 flag bytes of
+// every pixel (the kOutU8 exception flags of the pass that follows).
 hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
-                         uint2 *out, hipStream_t s);
-// Default lane layout for D disparities.
-Layout default_layout(int D);
-bool layout_supported(Layout l, int D);
-hipError_t launch_aggregate(const AggParams &p, Layout l, hipStream_t s);
-// Passes `dirs[0..n)` at once, pass j into `vols[j]` (accumulating when
-// `accumulate[j]`); no winner-take-all.  base supplies the shared fields.
-hipError_t launch_aggregate_multi(const AggParams &base, const int *dirs,
-                                  uint16_t *const *vols, const int *accumulate, int n,
-                                  Layout l, hipStream_t s);
-// Sums `nvol` volumes (vol0 + k * vol_stride, k < nvol) mod 2^16 and picks
-// each pixel's winner with parabola refinement; with `sum_out` the summed
-// volume is also stored there (may equal vol0), with `best` the integer winner.
-hipError_t launch_wta_sum(const uint16_t *vol0, size_t vol_stride, int nvol, int npx,
-                          int D, int Dp, float *out, int16_t *best, uint16_t *sum_out,
-                          hipStream_t s);
+                         uint2 *out, uint2 *zero_flags, hipStream_t s);
+// Default schedule for D disparities.
+Schedule default_schedule(int D);
+bool schedule_valid(Schedule sc, int D);
+// Runs the passes dirs[0..n) at once, pass j into vols[j] (and his[j] for
+// kOutU8).  base supplies the shared fields.
+hipError_t launch_paths(const PassParams &base, const int *dirs, uint8_t *const *vols,
+                        uint8_t *const *his, int n, Schedule sc, int mode, hipStream_t s);
+// Sums `nvol` per-direction volumes (vol0 + k * vol_stride bytes, k < nvol;
+// kOutU8 adds 256 * the high bytes of flagged slices, his at the same
+// offsets from hi0) mod 2^16 and picks each pixel's winner with parabola
+// refinement; with `sum_out` the summed u16 volume is also stored, with
+// `best` the integer winners.
+hipError_t launch_wta_sum(const uint8_t *vol0, size_t vol_stride, const uint8_t *hi0,
+                          const uint8_t *flags, int nvol, int mode, int npx, int D, int Dp,
+                          float *out, int16_t *best, uint16_t *sum_out, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
-hipError_t launch_wta(const uint16_t *vol, int w, int h, int D, int Dp,
-                      float *out, int16_t *best, hipStream_t s);
 hipError_t launch_lr_check(const uint16_t *vol, int w, int h, int D, int Dp,
                            int tol, const int16_t *best, float *out,
                            hipStream_t s);

@@ -1,0 +1,474 @@
+// Path aggregation over scanline pairs, packed two per register (gfx950).
+//
+// Reproduces vx-vision/shaders/SGMStereo.metal:72-173 (sgm_cost_aggregate) for
+// one direction, bit for bit, and extends it to the diagonals (a path starts
+// at every pixel whose predecessor lies outside the image).
+//
+// Lane layout.  A group of L lanes of one wave owns TWO scanlines of the same
+// direction (scanlines 2g and 2g+1 of the wave's range): every register holds
+// a u16 of the first scanline in its low half and of the second in its high
+// half, so the whole recurrence runs on packed u16 instructions
+// (v_pk_add_u16 / v_pk_sub_u16 / v_pk_min_u16), which wrap mod 2^16 per half
+// exactly as the shader's ushort arithmetic does.  Lane `sub` of the group owns
+// disparities d0 = sub*K .. d0+K-1 of both scanlines; cross-lane steps (the
+// d+1 neighbour, the carry from d-1, the minimum over d) are DPP / permlane
+// moves inside the wave.
+//
+// The reference updates prev[] in place over ascending d, so the d-1 term
+// reads the value already produced in the same step (SGMStereo.metal:155,167).
+// The d, d+1 and P2 terms do not depend on d-1 and are evaluated for all d at
+// once (A[d]); the d-1 chain runs sequentially inside each lane and is carried
+// across lanes by fixed-point rounds: every lane first assumes no carry-in,
+// then re-evaluates whenever the carry-in from the lane below changes
+// min(A[first], o1).  Lane 0's carry-in is fixed (none), so after r rounds
+// lanes 0..r are final; the loop stops as soon as no lane changes, which is
+// exactly the sequential result including every wrap.
+//
+// Output.  Each direction writes its own volume (entry pix*Dp + d):
+//   kOutU16  the u16 path cost;
+//   kOutU8   its low byte, plus, for the rare slices in which some entry
+//            exceeds 255 (only after a mod-2^16 wrap when 64 + P2 <= 255),
+//            the high bytes of the whole slice into a second volume and a
+//            flag byte (pixel, direction) that tells the summing pass to add
+//            them.  Sums are mod 2^16, so low + 256 * high reproduces the
+//            u16 path cost exactly.
+#pragma once
+
+#include "sgm_device.h"
+
+namespace vxh {
+namespace dev {
+
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t,
+                              (u16x2)(__builtin_bit_cast(u16x2, a) + __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t,
+                              (u16x2)(__builtin_bit_cast(u16x2, a) - __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_min(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_max(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
+}
+// Byte permute: result byte k = byte sel[k] of the 8-byte value (hi:lo), with
+// selector 0x0c giving zero.
+__device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) {
+    return __builtin_amdgcn_perm(hi, lo, sel);
+}
+
+template <int L>
+__device__ __forceinline__ uint32_t group_min_pk(uint32_t v) {
+    return group_reduce<L>(v, [](uint32_t a, uint32_t b) { return pk_min(a, b); });
+}
+
+// Stores the first BYTES bytes of w (BYTES in {2, 4, 8, 16, 32}; p aligned
+// to min(BYTES, 16)).
+template <int BYTES>
+__device__ __forceinline__ void store_seg(uint8_t *p, const uint32_t *w) {
+    if constexpr (BYTES == 2) *(uint16_t *)p = (uint16_t)w[0];
+    else if constexpr (BYTES == 4) *(uint32_t *)p = w[0];
+    else if constexpr (BYTES == 8) *(uint2 *)p = make_uint2(w[0], w[1]);
+    else {
+#pragma unroll
+        for (int j = 0; j < BYTES / 16; ++j)
+            ((uint4 *)p)[j] = make_uint4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
+    }
+}
+
+// One byte of every half of v[0..K) (byte 0 of each half for sel 0x06020400,
+// byte 1 for 0x07030501), regrouped per scanline in disparity order:
+// wa = bytes of the low halves, wb = of the high halves, 4 per word (for
+// K = 2, the two bytes in the low 16 bits).
+template <int K>
+__device__ __forceinline__ void bytes_of_pairs(const uint32_t *v, uint32_t sel, uint32_t *wa,
+                                               uint32_t *wb) {
+    uint32_t t[K / 2];   // [a_2j a_2j+1 b_2j b_2j+1]
+#pragma unroll
+    for (int j = 0; j < K / 2; ++j) t[j] = perm(v[2 * j + 1], v[2 * j], sel);
+    if constexpr (K == 2) {
+        wa[0] = t[0] & 0xFFFFu;
+        wb[0] = t[0] >> 16;
+    } else {
+#pragma unroll
+        for (int k = 0; k < K / 4; ++k) {
+            wa[k] = perm(t[2 * k + 1], t[2 * k], 0x05040100u);
+            wb[k] = perm(t[2 * k + 1], t[2 * k], 0x07060302u);
+        }
+    }
+}
+
+// Census loads of one step for both scanlines of a lane's group: left census
+// of the pixel and right census at x - d for the lane's K disparities.
+template <int K>
+struct PairBuf {
+    uint2 lv[2];
+    uint2 rv[2][K];
+    bool edge;   // some active scanline of the wave has x - d < 0 for a lane's d
+};
+
+// One wave aggregates scanlines s_first .. s_first + 2 * (64 / L) - 1 of
+// direction p.dir (those below p.nscan) into the direction's volume, in
+// storage MODE (kOutU8 or kOutU16; -1 = p.mode at run time).  Wave-level:
+// every lane of the wave must call it with the same arguments.
+template <int L, int K, int MODE>
+__device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first) {
+    static_assert(K >= 2 && K <= 16 && (K & (K - 1)) == 0, "K must be 2, 4, 8 or 16");
+    const int lane = threadIdx.x & 63;
+    const int g = lane / L, sub = lane % L;
+    const int w = p.w, h = p.h, D = p.D, Dp = p.Dp;
+    const int dx = p.dx, dy = p.dy;
+    const int d0 = sub * K;
+    const bool lane_real = d0 < D;
+    const bool lane_store = d0 < Dp;
+    const uint32_t p1x2 = p.p1x2, p2x2 = p.p2x2;
+    const uint2 *__restrict__ lc = p.lc;
+    const uint2 *__restrict__ rcg = (const uint2 *)((const char *)p.rc - kCensusGuard);
+
+    int sx[2] = {0, 0}, len[2] = {0, 0}, pix0[2] = {0, 0};
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+        const int s = s_first + 2 * g + hh;
+        int sy = 0;
+        if (s < p.nscan) scan_start(s, dx, dy, w, h, sx[hh], sy, len[hh]);
+        pix0[hh] = sy * w + sx[hh];
+    }
+    int maxlen = max(len[0], len[1]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, o));
+    if (maxlen == 0) return;
+    const int pstep = dy * w + dx;
+    const int mode = MODE >= 0 ? MODE : p.mode;
+    const size_t ebytes = mode == kOutU16 ? 2 : 1;
+    // Volume pointers of this lane's segment for both scanlines, advanced by
+    // one pixel step per step (64-bit: a volume may exceed 4 GB).
+    uint8_t *vp[2];
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh)
+        vp[hh] = p.vol + ((size_t)pix0[hh] * Dp + d0) * ebytes;
+    const ptrdiff_t vstep = (ptrdiff_t)pstep * Dp * (ptrdiff_t)ebytes;
+
+    // o2 is 0xFFFF where d + 1 >= D (SGMStereo.metal:156); entries d >= D
+    // are padding and never enter the minimum.
+    uint32_t om[K], pad[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        om[i] = (d0 + i + 1 < D) ? 0u : 0xFFFFFFFFu;
+        pad[i] = (d0 + i < D) ? 0u : 0xFFFFFFFFu;
+    }
+    uint32_t prev[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) prev[i] = 0;
+    uint32_t pm = 0;
+
+    auto issue = [&](PairBuf<K> &b, int t) {
+        bool e = false;
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            const bool a = t < len[hh];
+            const uint32_t pix = a ? (uint32_t)(pix0[hh] + t * pstep) : 0u;
+            b.lv[hh] = load_census(lc, pix << 3);
+            // One base, constant offsets: reads with x - d < 0 land in the
+            // previous row or the guard bytes before rc and are replaced by
+            // 64 in step().  Inactive scanlines read around pixel 0.
+            const uint32_t rb = ((pix - (uint32_t)(d0 + K - 1)) << 3) + (uint32_t)kCensusGuard;
+            static_for<K>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                b.rv[hh][i] = load_census<8 * (K - 1 - i)>(rcg, rb);
+            });
+            const int x = sx[hh] + t * dx;
+            e = e || (lanes_lt(a ? x : 0x7FFFFFFF, d0 + K - 1) != 0);
+        }
+        b.edge = e;
+    };
+
+    auto step = [&](PairBuf<K> &b, int t) {
+        const bool act0 = t < len[0], act1 = t < len[1];
+        uint32_t C[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const uint32_t c0 = __popc(b.lv[0].x ^ b.rv[0][i].x) + __popc(b.lv[0].y ^ b.rv[0][i].y);
+            const uint32_t c1 = __popc(b.lv[1].x ^ b.rv[1][i].x) + __popc(b.lv[1].y ^ b.rv[1][i].y);
+            C[i] = c0 | (c1 << 16);
+        }
+        if (b.edge) {
+            // Cost 64 where x - d < 0 (SGMStereo.metal:133-136); every cost is
+            // at most 64, so a packed max with 64 sets exactly those halves.
+            const int x0 = sx[0] + t * dx, x1 = sx[1] + t * dx;
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                const uint32_t e = (x0 < d0 + i ? 64u : 0u) | (x1 < d0 + i ? (64u << 16) : 0u);
+                C[i] = pk_max(C[i], e);
+            }
+        }
+
+        uint32_t agg[K];
+        if (t == 0) {
+#pragma unroll
+            for (int i = 0; i < K; ++i) agg[i] = C[i];
+        } else {
+            const uint32_t o3 = pk_add(pm, p2x2);
+            const uint32_t nb = from_upper_lane<L>(prev[0]);
+            uint32_t A[K], B[K];
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                const uint32_t nx = (i + 1 < K) ? prev[(i + 1) % K] : nb;
+                const uint32_t o2 = pk_add(nx, p1x2) | om[i];
+                A[i] = pk_min(pk_min(prev[i], o2), o3);
+                B[i] = pk_sub(C[i], pm);
+            }
+            // Speculative pass: no carry-in from the lane below.
+            agg[0] = pk_add(B[0], A[0]);
+#pragma unroll
+            for (int i = 1; i < K; ++i) agg[i] = pk_add(B[i], pk_min(A[i], pk_add(agg[i - 1], p1x2)));
+            // Fixed-point rounds over the carry-in.  Halves whose scanline has
+            // ended, the group's first lane and padding lanes take no carry.
+            const uint32_t take = (sub > 0 && lane_real)
+                                      ? ((act0 ? 0x0000FFFFu : 0u) | (act1 ? 0xFFFF0000u : 0u))
+                                      : 0u;
+            uint32_t e_used = A[0];
+            for (int r = 0; r < L - 1; ++r) {
+                const uint32_t up = from_lower_lane<L>(agg[K - 1]);
+                const uint32_t cand = pk_min(A[0], pk_add(up, p1x2));
+                const uint32_t e = (cand & take) | (e_used & ~take);
+                if (lanes_ne(e, e_used) == 0) break;
+                if (e != e_used) {
+                    e_used = e;
+                    agg[0] = pk_add(B[0], e);
+#pragma unroll
+                    for (int i = 1; i < K; ++i)
+                        agg[i] = pk_add(B[i], pk_min(A[i], pk_add(agg[i - 1], p1x2)));
+                }
+            }
+        }
+
+        uint32_t m = 0xFFFFFFFFu;
+#pragma unroll
+        for (int i = 0; i < K; ++i) m = pk_min(m, agg[i] | pad[i]);
+        pm = group_min_pk<L>(m);
+#pragma unroll
+        for (int i = 0; i < K; ++i) prev[i] = agg[i];
+
+        const bool st0 = act0 && lane_store, st1 = act1 && lane_store;
+        if (mode == kOutU16) {
+            // Halves to pixel order: [a_i | a_i+1] and [b_i | b_i+1].
+            uint32_t wa[K / 2], wb[K / 2];
+#pragma unroll
+            for (int i = 0; i < K / 2; ++i) {
+                wa[i] = perm(agg[2 * i + 1], agg[2 * i], 0x05040100u);
+                wb[i] = perm(agg[2 * i + 1], agg[2 * i], 0x07060302u);
+            }
+            store_seg<K * 2>(st0 ? vp[0] : store_sink<uint8_t>(), wa);
+            store_seg<K * 2>(st1 ? vp[1] : store_sink<uint8_t>(), wb);
+        } else {
+            uint32_t wa[(K + 3) / 4], wb[(K + 3) / 4];
+            bytes_of_pairs<K>(agg, 0x06020400u, wa, wb);
+            store_seg<K>(st0 ? vp[0] : store_sink<uint8_t>(), wa);
+            store_seg<K>(st1 ? vp[1] : store_sink<uint8_t>(), wb);
+            // Slices with an entry above 255: rare (after a wrap).
+            uint32_t o = 0;
+#pragma unroll
+            for (int i = 0; i < K; ++i) o |= agg[i];
+            const uint32_t exc = lane_real ? (((o & 0x0000FF00u) && act0 ? 1u : 0u) |
+                                              ((o & 0xFF000000u) && act1 ? 2u : 0u))
+                                           : 0u;
+            if (lanes_ne(exc, 0u) != 0) {
+                const uint32_t f = group_or<L>(exc);
+                uint32_t ha[(K + 3) / 4], hb[(K + 3) / 4];
+                bytes_of_pairs<K>(agg, 0x07030501u, ha, hb);
+                const ptrdiff_t hoff = p.hi - p.vol;
+                if ((f & 1u) && lane_store) store_seg<K>(vp[0] + hoff, ha);
+                if ((f & 2u) && lane_store) store_seg<K>(vp[1] + hoff, hb);
+                if (sub == 0) {
+#pragma unroll
+                    for (int hh = 0; hh < 2; ++hh)
+                        if (f & (1u << hh))
+                            p.flags[(size_t)(pix0[hh] + t * pstep) * 8 + p.dir] = 1;
+                }
+            }
+        }
+        vp[0] += vstep;
+        vp[1] += vstep;
+    };
+
+    // Loads run PF steps ahead through a ring of PF + 1 buffers; the loop is
+    // unrolled by the ring size so every buffer stays in fixed registers, and
+    // runs a whole number of rounds (the steps past maxlen are inactive on
+    // every lane: their loads read in-range words, their stores go to the
+    // sink).  With no branch around any load or store, every step issues the
+    // same memory operations and waits only for its own PF-old loads.
+    constexpr int PF = K <= 2 ? 6 : (K <= 4 ? 4 : (K <= 8 ? 2 : 1));
+    constexpr int NB = PF + 1;
+    const int rounds = __builtin_amdgcn_readfirstlane((maxlen + NB - 1) / NB);
+    PairBuf<K> buf[NB];
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+        issue(buf[j], j);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int r = 0; r < rounds; ++r) {
+        const int t = r * NB;
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            issue(buf[(u + PF) % NB], t + u + PF);
+            step(buf[u], t + u);
+        }
+    }
+}
+
+// Winner-take-all with parabola refinement of one pixel from its summed
+// slice, 16 lanes per pixel: in chunk c lane j holds entries
+// 128c + 8j .. 128c + 8j + 7 in v[c][0..3] (pairs in disparity order).  The
+// first strict minimum from 0xFFFF (SGMStereo.metal:185-199) is the smallest
+// key (value << 16 | d) over d < D; a slice of all 0xFFFF keeps d = 0.
+__device__ __forceinline__ void wta_slice(const u16x2 (&v)[2][4], int nc, int j, int D,
+                                          bool valid, int pix, float *__restrict__ out,
+                                          int16_t *__restrict__ best) {
+    uint32_t key = 0xFFFFFFFFu;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        if (c >= nc) break;
+        const int d0 = 128 * c + 8 * j;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const uint32_t val = v[c][e / 2][e & 1];
+            if (d0 + e < D) key = min(key, (val << 16) | (uint32_t)(d0 + e));
+        }
+    }
+    key = group_min<16>(key);
+    const int bd = (int)(key & 0xFFFFu);
+    const uint32_t bc = key >> 16;
+    uint32_t nb2 = 0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        if (c >= nc) break;
+        const int d0 = 128 * c + 8 * j;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const uint32_t val = v[c][e / 2][e & 1];
+            if (d0 + e == bd - 1) nb2 |= val << 16;
+            if (d0 + e == bd + 1) nb2 |= val;
+        }
+    }
+    nb2 = group_or<16>(nb2);
+    if (valid && j == 0) {
+        // Parabola through the neighbours (SGMStereo.metal:201-211), IEEE fp32.
+        const int bdw = bc == 0xFFFFu ? 0 : bd;
+        float disp = (float)bdw;
+        if (bdw > 0 && bdw < D - 1) {
+            const float c0 = (float)(nb2 >> 16);
+            const float c1 = (float)bc;
+            const float c2 = (float)(nb2 & 0xFFFFu);
+            const float den = 2.0f * (c0 + c2 - 2.0f * c1);
+            if (fabsf(den) > 1e-6f) disp = disp + (c0 - c2) / den;
+        }
+        if (out) out[pix] = disp;
+        if (best) best[pix] = (int16_t)bdw;
+    }
+}
+
+// Sums the nv per-direction volumes of pixel `pix` (vol0 + k * vstride
+// bytes, entries pix*Dp + d) mod 2^16 and takes its winner; 16 lanes per
+// pixel (j = lane & 15), nc = 1 for Dp <= 128, else 2.
+//   kOutU16: u16 entries, one 16-byte load per lane, volume and chunk.
+//   kOutU8:  u8 entries, 8 bytes per lane, volume and chunk; the byte sums
+//            of at most 8 volumes stay below 2^16, so even and odd bytes are
+//            summed as two u16 pairs without carries; slices flagged in the
+//            pixel's 8 flag bytes (byte k by direction k) add 256 * their
+//            high bytes (hi0 + k * vstride, same entry offsets).
+// With sum_out the summed u16 slice is also stored (pixel-major, stride Dp).
+// All lanes of the wave must call it (the winner uses cross-lane moves).
+__device__ __forceinline__ void wta_sum_pixel(const uint8_t *__restrict__ vol0, size_t vstride,
+                                              const uint8_t *__restrict__ hi0,
+                                              const uint2 *__restrict__ flags, int nv, int nc,
+                                              int mode, int pix, int npx, int D, int Dp,
+                                              float *__restrict__ out,
+                                              int16_t *__restrict__ best,
+                                              uint16_t *__restrict__ sum_out) {
+    const int j = (int)(threadIdx.x & 15);
+    const bool valid = pix < npx;
+    const size_t base = (size_t)(valid ? pix : 0) * (size_t)Dp;
+    u16x2 v[2][4];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[c][q] = (u16x2){0, 0};
+    if (mode == kOutU16) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            if (c >= nc) break;
+            const int d0 = 128 * c + 8 * j;
+            if (d0 < Dp) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (k >= nv) break;
+                    const uint4 x = *(const uint4 *)(vol0 + k * vstride + (base + d0) * 2);
+                    v[c][0] += __builtin_bit_cast(u16x2, x.x);
+                    v[c][1] += __builtin_bit_cast(u16x2, x.y);
+                    v[c][2] += __builtin_bit_cast(u16x2, x.z);
+                    v[c][3] += __builtin_bit_cast(u16x2, x.w);
+                }
+            }
+        }
+    } else {
+        const uint2 fl = flags[valid ? pix : 0];
+        const bool flagged = (fl.x | fl.y) != 0;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            if (c >= nc) break;
+            const int d0 = 128 * c + 8 * j;
+            uint32_t e0 = 0, o0 = 0, e1 = 0, o1 = 0;
+            if (d0 < Dp) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (k >= nv) break;
+                    const uint2 x = *(const uint2 *)(vol0 + k * vstride + base + d0);
+                    e0 += x.x & 0x00FF00FFu;
+                    o0 += perm(0u, x.x, 0x0c030c01u);
+                    e1 += x.y & 0x00FF00FFu;
+                    o1 += perm(0u, x.y, 0x0c030c01u);
+                }
+                if (flagged) {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        if (k >= nv) break;
+                        const uint32_t fb = k < 4 ? (fl.x >> (8 * k)) : (fl.y >> (8 * (k - 4)));
+                        if (fb & 0xFFu) {
+                            const uint2 x = *(const uint2 *)(hi0 + k * vstride + base + d0);
+                            e0 = pk_add(e0, (x.x & 0x00FF00FFu) << 8);
+                            o0 = pk_add(o0, perm(0u, x.x, 0x030c010cu));
+                            e1 = pk_add(e1, (x.y & 0x00FF00FFu) << 8);
+                            o1 = pk_add(o1, perm(0u, x.y, 0x030c010cu));
+                        }
+                    }
+                }
+            }
+            // Entries d0..d0+7 in disparity order: [e0.lo o0.lo] [e0.hi o0.hi] ...
+            v[c][0] = __builtin_bit_cast(u16x2, perm(o0, e0, 0x05040100u));
+            v[c][1] = __builtin_bit_cast(u16x2, perm(o0, e0, 0x07060302u));
+            v[c][2] = __builtin_bit_cast(u16x2, perm(o1, e1, 0x05040100u));
+            v[c][3] = __builtin_bit_cast(u16x2, perm(o1, e1, 0x07060302u));
+        }
+    }
+    if (sum_out != nullptr && valid) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            if (c >= nc) break;
+            const int d0 = 128 * c + 8 * j;
+            if (d0 < Dp)
+                *(uint4 *)(sum_out + base + d0) =
+                    make_uint4(__builtin_bit_cast(uint32_t, v[c][0]),
+                               __builtin_bit_cast(uint32_t, v[c][1]),
+                               __builtin_bit_cast(uint32_t, v[c][2]),
+                               __builtin_bit_cast(uint32_t, v[c][3]));
+        }
+    }
+    wta_slice(v, nc, j, D, valid, pix, out, best);
+}
+
+}  // namespace dev
+}  // namespace vxh

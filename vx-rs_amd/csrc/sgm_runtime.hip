@@ -52,27 +52,20 @@ vxh_status hip_fail(hipError_t e, const char *what, size_t bytes = 0) {
         if (e_ != hipSuccess) return hip_fail(e_, what); \
     } while (0)
 
-constexpr size_t kAlign = 256;
-inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 enum TexFormat { kGray8 = 1, kR32F = 2 };
 
-struct Layout {
-    size_t lc, rc, vol, vol_bytes, best, total;
-};
+using vxh::WsLayout;
+using vxh::workspace_layout;
 
-// Scratch of one pair: census of both images, `groups` summed volumes (one
-// per path group) and the winner index used by the LR check.
-Layout workspace_layout(uint32_t w, uint32_t h, uint32_t D, int groups = 1) {
-    Layout l;
-    size_t n = (size_t)w * h;
-    l.lc = 0;
-    l.rc = l.lc + vxh::census_span(n);
-    l.vol = l.rc + align_up(n * 8);
-    l.vol_bytes = align_up(n * (size_t)vxh::volume_stride((int)D) * 2);
-    l.best = l.vol + l.vol_bytes * (size_t)groups;
-    l.total = l.best + align_up(n * 2);
-    return l;
+int auto_mode(const vxh_sgm_config *c) { return vxh::auto_volume_mode(c->p2); }
+
+// Workspace large enough for either format.
+size_t workspace_total(uint32_t w, uint32_t h, const vxh_sgm_config *c) {
+    return std::max(workspace_layout(w, h, c->num_disparities, (int)c->num_paths,
+                                     vxh::kOutU8, c->lr_check != 0).total,
+                    workspace_layout(w, h, c->num_disparities, (int)c->num_paths,
+                                     vxh::kOutU16, c->lr_check != 0).total);
 }
 
 struct Slot {
@@ -81,7 +74,15 @@ struct Slot {
     size_t ws_bytes = 0;
 };
 
-bool groups_valid(uint32_t g) { return g == 1 || g == 2 || g == 4 || g == 8; }
+// Index of (lanes, per_lane) among the layouts of D's class, -1 if none.
+int layout_index(int D, uint32_t lanes, uint32_t per_lane) {
+    const int c = vxh::layout_class(D);
+    for (int i = 0; i < vxh::kClassCount[c]; ++i)
+        if ((uint32_t)vxh::kClassLayouts[c][i][0] == lanes &&
+            (uint32_t)vxh::kClassLayouts[c][i][1] == per_lane)
+            return i;
+    return -1;
+}
 
 }  // namespace
 
@@ -102,24 +103,23 @@ struct vxh_sgm {
     std::mutex mu;
     std::vector<Slot *> free_slots;
     std::vector<Slot *> all_slots;
-    vxh::Layout layouts[257] = {};   // per D; L == 0 selects the default
-    uint8_t groups[257] = {};        // per D; 0 selects the default
+    vxh::Schedule sched[257] = {};   // per D, used when sched_set[D]
+    bool sched_set[257] = {};
+    uint8_t format[257] = {};        // per D: 0 auto, 1 u8, 2 u16
 };
 
-static vxh::Layout layout_for(vxh_sgm *s, int D) {
+static vxh::Schedule schedule_for(vxh_sgm *s, int D) {
     std::lock_guard<std::mutex> g(s->mu);
-    vxh::Layout l = s->layouts[D];
-    return l.L == 0 ? vxh::default_layout(D) : l;
+    return s->sched_set[D] ? s->sched[D] : vxh::default_schedule(D);
 }
 
-static int groups_for(vxh_sgm *s, int D, int P) {
-    int g;
+static int mode_for(vxh_sgm *s, const vxh_sgm_config *c) {
+    int f;
     {
-        std::lock_guard<std::mutex> lk(s->mu);
-        g = s->groups[D] == 0 ? 1 : s->groups[D];
+        std::lock_guard<std::mutex> g(s->mu);
+        f = s->format[c->num_disparities];
     }
-    while (g > P) g /= 2;
-    return g;
+    return f == 1 ? vxh::kOutU8 : (f == 2 ? vxh::kOutU16 : auto_mode(c));
 }
 
 extern "C" {
@@ -304,133 +304,91 @@ void vxh_sgm_destroy(vxh_sgm *s) {
 
 size_t vxh_sgm_workspace_bytes(uint32_t w, uint32_t h, const vxh_sgm_config *c) {
     if (!c) return 0;
-    return workspace_layout(w, h, c->num_disparities).total;
+    return workspace_total(w, h, c);
 }
 
-size_t vxh_sgm_workspace_bytes_groups(uint32_t w, uint32_t h, const vxh_sgm_config *c,
-                                      uint32_t groups) {
-    if (!c || !groups_valid(groups)) return 0;
-    return workspace_layout(w, h, c->num_disparities, (int)groups).total;
-}
-
-// Runs every path into `vol`.  With `wta_out`, the last path also picks the
-// winner from its final sums and writes the disparity map instead of storing
-// its sums.
-static vxh_status aggregate_impl(const uint2 *lc, const uint2 *rc, uint32_t w,
-                                 uint32_t h, const vxh_sgm_config *c, uint16_t *vol,
-                                 vxh::Layout lay, hipStream_t st,
-                                 float *wta_out = nullptr) {
+// Aggregates every path of `c` at once, direction i into the i-th volume of
+// the workspace (`mode` storage; the u8 flags must already be clear), then
+// sums the volumes and takes the winner: into `out` (may be null), `best`
+// (null unless the LR check follows) and, with `sum_out`, the summed u16
+// volume.  With `mark`, records it on `st` between the two launches.
+static vxh_status paths_and_wta(const uint2 *lc, const uint2 *rc, uint32_t w, uint32_t h,
+                                const vxh_sgm_config *c, char *ws, const WsLayout &L,
+                                int mode, vxh::Schedule sc, hipStream_t st, float *out,
+                                int16_t *best, uint16_t *sum_out,
+                                hipEvent_t mark = nullptr) {
     const int D = (int)c->num_disparities;
-    if (!vxh::layout_supported(lay, D))
-        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
-    vxh::AggParams p{};
-    p.lc = lc;
-    p.rc = rc;
-    p.vol = vol;
-    p.w = (int)w;
-    p.h = (int)h;
-    p.D = D;
-    p.Dp = vxh::volume_stride(D);
-    p.p1 = (uint32_t)c->p1 & 0xFFFFu;
-    p.p2 = (uint32_t)c->p2 & 0xFFFFu;
-    for (uint32_t i = 0; i < c->num_paths; ++i) {
-        p.dir = (int)i;
-        p.nscan = vxh::num_scanlines((int)i, (int)w, (int)h);
-        p.accumulate = i > 0;
-        p.wta_out = (i + 1 == c->num_paths) ? wta_out : nullptr;
-        VXH_HIP(vxh::launch_aggregate(p, lay, st), "aggregate");
-    }
-    return VXH_OK;
-}
-
-// Runs the P paths as P / groups launches of `groups` concurrent passes:
-// launch k runs directions k*groups .. k*groups + groups - 1, direction i
-// accumulating into volume i % groups (the slices of `vol_bytes` after
-// `vol`), and the summing winner-take-all adds the volumes.  Sums are
-// mod 2^16 and therefore independent of the grouping.  With `sum_out` the
-// summed volume is stored into the first slice and `best` receives the
-// integer winners (for the LR check).
-static vxh_status run_path_groups(const uint2 *lc, const uint2 *rc, uint32_t w, uint32_t h,
-                                  const vxh_sgm_config *c, uint16_t *vol, size_t vol_bytes,
-                                  int groups, vxh::Layout lay, hipStream_t st, float *out,
-                                  int16_t *best, bool store_sum) {
-    const int D = (int)c->num_disparities;
-    if (!vxh::layout_supported(lay, D))
-        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
     const int P = (int)c->num_paths;
-    vxh::AggParams base{};
+    vxh::PassParams base{};
     base.lc = lc;
     base.rc = rc;
+    base.flags = mode == vxh::kOutU8 ? (uint8_t *)(ws + L.flags) : nullptr;
     base.w = (int)w;
     base.h = (int)h;
     base.D = D;
     base.Dp = vxh::volume_stride(D);
-    base.p1 = (uint32_t)c->p1 & 0xFFFFu;
-    base.p2 = (uint32_t)c->p2 & 0xFFFFu;
-    for (int k = 0; k < P / groups; ++k) {
-        int dirs[vxh::kMaxPasses], acc[vxh::kMaxPasses];
-        uint16_t *vols[vxh::kMaxPasses];
-        for (int g = 0; g < groups; ++g) {
-            dirs[g] = k * groups + g;
-            vols[g] = (uint16_t *)((char *)vol + (size_t)g * vol_bytes);
-            acc[g] = k > 0;
-        }
-        VXH_HIP(vxh::launch_aggregate_multi(base, dirs, vols, acc, groups, lay, st),
-                "aggregate");
+    const uint32_t p1 = (uint32_t)c->p1 & 0xFFFFu, p2 = (uint32_t)c->p2 & 0xFFFFu;
+    base.p1x2 = p1 | (p1 << 16);
+    base.p2x2 = p2 | (p2 << 16);
+    int dirs[vxh::kMaxPasses];
+    uint8_t *vols[vxh::kMaxPasses], *his[vxh::kMaxPasses];
+    for (int i = 0; i < P; ++i) {
+        dirs[i] = i;
+        vols[i] = (uint8_t *)(ws + L.vol + (size_t)i * L.vol_bytes);
+        his[i] = mode == vxh::kOutU8 ? (uint8_t *)(ws + L.hi + (size_t)i * L.vol_bytes)
+                                     : nullptr;
     }
-    VXH_HIP(vxh::launch_wta_sum(vol, vol_bytes / 2, groups, (int)(w * h), D, base.Dp, out,
-                                best, store_sum ? vol : nullptr, st),
+    VXH_HIP(vxh::launch_paths(base, dirs, vols, his, P, sc, mode, st), "aggregate");
+    if (mark) VXH_HIP(hipEventRecord(mark, st), "hipEventRecord");
+    VXH_HIP(vxh::launch_wta_sum((const uint8_t *)(ws + L.vol), L.vol_bytes,
+                                mode == vxh::kOutU8 ? (const uint8_t *)(ws + L.hi) : nullptr,
+                                (const uint8_t *)base.flags, P, mode, (int)(w * h), D, base.Dp,
+                                out, best, sum_out, st),
             "wta");
     return VXH_OK;
 }
 
+// The whole pipeline on `st`.  With `marks`, records marks[0] after the
+// census pass, marks[1] after the path aggregation and marks[2] at the end.
 static vxh_status enqueue_impl(vxh_sgm *s, const uint8_t *d_left, const uint8_t *d_right,
                                uint32_t w, uint32_t h, const vxh_sgm_config *c,
                                float *d_out, void *d_ws, size_t ws_bytes,
-                               hipStream_t st) {
+                               hipStream_t st, hipEvent_t const *marks = nullptr) {
     vxh_status v = vxh_sgm_config_validate(c, w, h);
     if (v != VXH_OK) return v;
     const int D = (int)c->num_disparities;
-    Layout L = workspace_layout(w, h, D);
     if (!d_left || !d_right || !d_out || !d_ws)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null device buffer");
+    const int mode = mode_for(s, c);
+    const bool lr = c->lr_check != 0;
+    const WsLayout L = workspace_layout(w, h, D, (int)c->num_paths, mode, lr);
     if (ws_bytes < L.total) {
         char buf[128];
         snprintf(buf, sizeof buf, "invalid config: workspace %zu < %zu bytes",
                  ws_bytes, L.total);
         return fail(VXH_ERR_INVALID_CONFIG, buf);
     }
-    // The largest path-group count up to the handle's setting whose volumes
-    // fit the workspace given.
-    int groups = groups_for(s, D, (int)c->num_paths);
-    while (groups > 1 && workspace_layout(w, h, D, groups).total > ws_bytes) groups /= 2;
-    L = workspace_layout(w, h, D, groups);
     char *base = (char *)d_ws;
     uint2 *lc = (uint2 *)(base + L.lc);
     uint2 *rc = (uint2 *)(base + L.rc);
-    uint16_t *vol = (uint16_t *)(base + L.vol);
-    int16_t *best = (int16_t *)(base + L.best);
-    VXH_HIP(vxh::launch_census(d_left, w, h, c->census_radius_x, c->census_radius_y, lc, st),
+    uint2 *flags = mode == vxh::kOutU8 ? (uint2 *)(base + L.flags) : nullptr;
+    VXH_HIP(vxh::launch_census(d_left, w, h, c->census_radius_x, c->census_radius_y, lc,
+                               flags, st),
             "census");
-    VXH_HIP(vxh::launch_census(d_right, w, h, c->census_radius_x, c->census_radius_y, rc, st),
+    VXH_HIP(vxh::launch_census(d_right, w, h, c->census_radius_x, c->census_radius_y, rc,
+                               nullptr, st),
             "census");
-    const vxh::Layout lay = layout_for(s, D);
-    const int Dp = vxh::volume_stride(D);
-    if (groups > 1) {
-        vxh_status v2 = run_path_groups(lc, rc, w, h, c, vol, L.vol_bytes, groups, lay, st,
-                                        d_out, c->lr_check ? best : nullptr, c->lr_check);
-        if (v2 != VXH_OK || !c->lr_check) return v2;
-    } else if (!c->lr_check) {
-        return aggregate_impl(lc, rc, w, h, c, vol, lay, st, d_out);
-    } else {
-        // The LR check reads sums of other pixels, so the volume is completed
-        // first and the winner taken by separate passes.
-        vxh_status v2 = aggregate_impl(lc, rc, w, h, c, vol, lay, st);
-        if (v2 != VXH_OK) return v2;
-        VXH_HIP(vxh::launch_wta(vol, w, h, D, Dp, d_out, best, st), "wta");
-    }
-    VXH_HIP(vxh::launch_lr_check(vol, w, h, D, Dp, (int)c->lr_tolerance, best, d_out, st),
-            "lr_check");
+    if (marks) VXH_HIP(hipEventRecord(marks[0], st), "hipEventRecord");
+    int16_t *best = lr ? (int16_t *)(base + L.best) : nullptr;
+    uint16_t *sum = lr ? (uint16_t *)(base + L.sum) : nullptr;
+    v = paths_and_wta(lc, rc, w, h, c, base, L, mode, schedule_for(s, D), st, d_out, best,
+                      sum, marks ? marks[1] : nullptr);
+    if (v != VXH_OK) return v;
+    if (lr)
+        VXH_HIP(vxh::launch_lr_check(sum, w, h, D, vxh::volume_stride(D),
+                                     (int)c->lr_tolerance, best, d_out, st),
+                "lr_check");
+    if (marks) VXH_HIP(hipEventRecord(marks[2], st), "hipEventRecord");
     return VXH_OK;
 }
 
@@ -441,6 +399,18 @@ vxh_status vxh_sgm_enqueue(vxh_sgm *s, const uint8_t *d_left, const uint8_t *d_r
     VXH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
     return enqueue_impl(s, d_left, d_right, w, h, c, d_out, d_ws, ws_bytes,
                         (hipStream_t)stream);
+}
+
+vxh_status vxh_sgm_enqueue_marked(vxh_sgm *s, const uint8_t *d_left, const uint8_t *d_right,
+                                  uint32_t w, uint32_t h, const vxh_sgm_config *c,
+                                  float *d_out, void *d_ws, size_t ws_bytes, void *stream,
+                                  void *const *events3) {
+    if (!s) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null handle");
+    if (!events3 || !events3[0] || !events3[1] || !events3[2])
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: three events needed");
+    VXH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
+    return enqueue_impl(s, d_left, d_right, w, h, c, d_out, d_ws, ws_bytes,
+                        (hipStream_t)stream, (hipEvent_t const *)events3);
 }
 
 static vxh_status acquire_slot(vxh_sgm *s, size_t bytes, Slot **out) {
@@ -500,8 +470,7 @@ vxh_status vxh_sgm_compute_disparity(vxh_sgm *s, vxh_context *ctx,
     if (v != VXH_OK) return v;
     VXH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
     Slot *sl = nullptr;
-    const int groups = groups_for(s, (int)c->num_disparities, (int)c->num_paths);
-    v = acquire_slot(s, workspace_layout(w, h, c->num_disparities, groups).total, &sl);
+    v = acquire_slot(s, workspace_total(w, h, c), &sl);
     if (v != VXH_OK) return v;
     v = enqueue_impl(s, (const uint8_t *)left->dptr, (const uint8_t *)right->dptr, w, h,
                      c, (float *)out->dptr, sl->ws, sl->ws_bytes, sl->stream);
@@ -559,29 +528,48 @@ vxh_status vxh_sgm_census(const uint8_t *d_img, uint32_t w, uint32_t h, uint32_t
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: census arguments");
     if ((2ull * rx + 1) * (2ull * ry + 1) - 1 > 64)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: census window over 64 bits");
-    VXH_HIP(vxh::launch_census(d_img, w, h, rx, ry, (uint2 *)d_census, (hipStream_t)stream),
+    VXH_HIP(vxh::launch_census(d_img, w, h, rx, ry, (uint2 *)d_census, nullptr,
+                               (hipStream_t)stream),
             "census");
     return VXH_OK;
 }
 
-vxh_status vxh_sgm_set_layout(vxh_sgm *s, uint32_t nd, uint32_t lanes, uint32_t per_lane) {
+vxh_status vxh_sgm_set_schedule(vxh_sgm *s, uint32_t nd, uint32_t lanes_h,
+                                uint32_t per_lane_h, uint32_t lanes, uint32_t per_lane) {
     if (!s || nd < 1 || nd > 256)
-        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: layout target");
-    vxh::Layout l{(int)lanes, (int)per_lane};
-    if (!(lanes == 0 && per_lane == 0) && !vxh::layout_supported(l, (int)nd))
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: schedule target");
+    if (lanes_h == 0 && per_lane_h == 0 && lanes == 0 && per_lane == 0) {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->sched_set[nd] = false;
+        return VXH_OK;
+    }
+    const int ih = layout_index((int)nd, lanes_h, per_lane_h);
+    const int io = layout_index((int)nd, lanes, per_lane);
+    if (ih < 0 || io < 0)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
     std::lock_guard<std::mutex> g(s->mu);
-    s->layouts[nd] = l;
+    s->sched[nd] = vxh::Schedule{ih, io};
+    s->sched_set[nd] = true;
     return VXH_OK;
 }
 
-vxh_status vxh_sgm_set_groups(vxh_sgm *s, uint32_t nd, uint32_t groups) {
-    if (!s || nd < 1 || nd > 256)
-        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: groups target");
-    if (groups != 0 && !groups_valid(groups))
-        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: path groups not in {1,2,4,8}");
+vxh_status vxh_sgm_get_schedule(vxh_sgm *s, uint32_t nd, uint32_t *out4) {
+    if (!s || nd < 1 || nd > 256 || !out4)
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: schedule target");
+    const vxh::Schedule sc = schedule_for(s, (int)nd);
+    const int c = vxh::layout_class((int)nd);
+    out4[0] = (uint32_t)vxh::kClassLayouts[c][sc.horiz][0];
+    out4[1] = (uint32_t)vxh::kClassLayouts[c][sc.horiz][1];
+    out4[2] = (uint32_t)vxh::kClassLayouts[c][sc.other][0];
+    out4[3] = (uint32_t)vxh::kClassLayouts[c][sc.other][1];
+    return VXH_OK;
+}
+
+vxh_status vxh_sgm_set_volume_format(vxh_sgm *s, uint32_t nd, uint32_t format) {
+    if (!s || nd < 1 || nd > 256 || format > 2)
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: volume format");
     std::lock_guard<std::mutex> g(s->mu);
-    s->groups[nd] = (uint8_t)groups;
+    s->format[nd] = (uint8_t)format;
     return VXH_OK;
 }
 
@@ -602,24 +590,30 @@ vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,
     if (v != VXH_OK) return v;
     if (!d_lc || !d_rc || !d_volume)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null device buffer");
-    vxh::Layout lay = (lanes == 0 && per_lane == 0)
-                          ? vxh::default_layout((int)c->num_disparities)
-                          : vxh::Layout{(int)lanes, (int)per_lane};
-    if (!vxh::layout_supported(lay, (int)c->num_disparities))
-        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
-    // The kernels read up to kCensusGuard bytes in front of the right census,
-    // which a caller's buffer need not have: copy it behind such a guard
-    // (stream-ordered allocation, so the call stays asynchronous).
+    const int D = (int)c->num_disparities;
+    vxh::Schedule sc = vxh::default_schedule(D);
+    if (lanes != 0 || per_lane != 0) {
+        const int i = layout_index(D, lanes, per_lane);
+        if (i < 0) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
+        sc = vxh::Schedule{i, i};
+    }
+    // Stream-ordered scratch (the call stays asynchronous): the right census
+    // behind the kCensusGuard readable bytes the kernels may touch in front
+    // of it (a caller's buffer need not have them), the flags and the
+    // per-direction volumes.
     hipStream_t st = (hipStream_t)stream;
-    const size_t cb = (size_t)w * h * 8;
+    const size_t n = (size_t)w * h;
+    const int mode = auto_mode(c);
+    const WsLayout L = workspace_layout(w, h, D, (int)c->num_paths, mode, false);
+    const size_t bytes = L.total;
     char *g = nullptr;
-    hipError_t e = hipMallocAsync((void **)&g, vxh::kCensusGuard + cb, st);
-    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync", vxh::kCensusGuard + cb);
-    e = hipMemcpyAsync(g + vxh::kCensusGuard, d_rc, cb, hipMemcpyDeviceToDevice, st);
+    hipError_t e = hipMallocAsync((void **)&g, bytes, st);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync", bytes);
+    e = hipMemcpyAsync(g + L.rc, d_rc, n * 8, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess && mode == vxh::kOutU8) e = hipMemsetAsync(g + L.flags, 0, n * 8, st);
     vxh_status v2 = e == hipSuccess
-                        ? aggregate_impl((const uint2 *)d_lc,
-                                         (const uint2 *)(g + vxh::kCensusGuard), w, h, c,
-                                         d_volume, lay, st)
+                        ? paths_and_wta((const uint2 *)d_lc, (const uint2 *)(g + L.rc), w, h,
+                                        c, g, L, mode, sc, st, nullptr, nullptr, d_volume)
                         : hip_fail(e, "hipMemcpyAsync");
     hipError_t e2 = hipFreeAsync(g, st);
     if (v2 != VXH_OK) return v2;
@@ -641,8 +635,9 @@ vxh_status vxh_sgm_wta(const uint16_t *d_volume, uint32_t w, uint32_t h,
         return fail(VXH_ERR_INVALID_CONFIG,
                     "invalid config: the stage-level WTA entry point has no LR check");
     const int D = (int)c->num_disparities;
-    VXH_HIP(vxh::launch_wta(d_volume, w, h, D, vxh::volume_stride(D), d_out, nullptr,
-                            (hipStream_t)stream),
+    VXH_HIP(vxh::launch_wta_sum((const uint8_t *)d_volume, 0, nullptr, nullptr, 1, vxh::kOutU16,
+                                (int)(w * h), D, vxh::volume_stride(D), d_out, nullptr,
+                                nullptr, (hipStream_t)stream),
             "wta");
     return VXH_OK;
 }

@@ -2,7 +2,8 @@
 //
 // One scheduler thread owns the device side: it polls in-flight requests for
 // completion, then admits queued requests in arrival order while the sum of
-// admitted cost-volume bytes stays within the HBM budget and fewer than
+// admitted cost-volume bytes (each request's pipeline workspace: the
+// per-direction volumes) stays within the HBM budget and fewer than
 // max_inflight are running.  An admitted request takes a device-resident slot
 // (inputs, output, census + volume scratch) from a pool keyed by its size,
 // like the reference's TexturePool (vx-vision/src/pool.rs:61-97), and runs
@@ -33,7 +34,6 @@ double now_s() {
     return std::chrono::duration<double>(Clock::now().time_since_epoch()).count();
 }
 
-uint32_t stride16(uint32_t d) { return (d + 15u) & ~15u; }
 
 struct Slot {
     uint32_t w = 0, h = 0;
@@ -282,7 +282,7 @@ extern "C" {
 
 uint64_t vxh_server_request_bytes(uint32_t w, uint32_t h, const vxh_sgm_config *cfg) {
     if (!cfg) return 0;
-    return (uint64_t)w * h * stride16(cfg->num_disparities) * 2;
+    return (uint64_t)vxh_sgm_workspace_bytes(w, h, cfg);
 }
 
 vxh_status vxh_server_create(int device, const vxh_server_config *cfg, vxh_server **out) {

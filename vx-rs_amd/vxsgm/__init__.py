@@ -86,13 +86,15 @@ _sig("vxh_sgm_compute_host", ctypes.c_int, _vp, _vp, _vp, _u32, _u32, _cfgp, _vp
 _sig("vxh_sgm_workspace_bytes", _sz, _u32, _u32, _cfgp)
 _sig("vxh_sgm_enqueue", ctypes.c_int, _vp, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp,
      _sz, _vp)
+_sig("vxh_sgm_enqueue_marked", ctypes.c_int, _vp, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp,
+     _sz, _vp, ctypes.POINTER(_vp))
 _sig("vxh_sgm_census", ctypes.c_int, _vp, _u32, _u32, _u32, _u32, _vp, _vp)
 _sig("vxh_sgm_aggregate", ctypes.c_int, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp)
 _sig("vxh_sgm_wta", ctypes.c_int, _vp, _u32, _u32, _cfgp, _vp, _vp)
 _sig("vxh_sgm_volume_stride", _u32, _u32)
-_sig("vxh_sgm_set_layout", ctypes.c_int, _vp, _u32, _u32, _u32)
-_sig("vxh_sgm_set_groups", ctypes.c_int, _vp, _u32, _u32)
-_sig("vxh_sgm_workspace_bytes_groups", _sz, _u32, _u32, _cfgp, _u32)
+_sig("vxh_sgm_set_schedule", ctypes.c_int, _vp, _u32, _u32, _u32, _u32, _u32)
+_sig("vxh_sgm_get_schedule", ctypes.c_int, _vp, _u32, ctypes.POINTER(_u32))
+_sig("vxh_sgm_set_volume_format", ctypes.c_int, _vp, _u32, _u32)
 _sig("vxh_util_copy", ctypes.c_int, _vp, _vp, _sz, _vp)
 _sig("vxh_sgm_aggregate_layout", ctypes.c_int, _vp, _vp, _u32, _u32, _cfgp, _u32, _u32,
      _vp, _vp)
@@ -298,14 +300,28 @@ class SGMStereo:
                                        height, ctypes.byref(c), out.ctypes.data))
         return out
 
-    def set_layout(self, num_disparities, schedule):
-        """Lane layout (lanes, per_lane) for pairs with this D; None = default."""
-        lanes, per = schedule if schedule else (0, 0)
-        check(lib.vxh_sgm_set_layout(self._h, num_disparities, lanes, per))
+    def set_schedule(self, num_disparities, schedule):
+        """Lane layouts for pairs with this D: ((lanes_h, per_lane_h), (lanes,
+        per_lane)) for the horizontal and the other directions, or one
+        (lanes, per_lane) for all of them; None = default."""
+        if not schedule:
+            h = o = (0, 0)
+        elif isinstance(schedule[0], (tuple, list)):
+            h, o = schedule
+        else:
+            h = o = schedule
+        check(lib.vxh_sgm_set_schedule(self._h, num_disparities, h[0], h[1], o[0], o[1]))
 
-    def set_groups(self, num_disparities, groups):
-        """Concurrent direction passes (1, 2, 4, 8) for pairs with this D; 0/None = default."""
-        check(lib.vxh_sgm_set_groups(self._h, num_disparities, groups or 0))
+    def schedule(self, num_disparities):
+        """((lanes_h, per_lane_h), (lanes, per_lane)) in effect for this D."""
+        out = (_u32 * 4)()
+        check(lib.vxh_sgm_get_schedule(self._h, num_disparities, out))
+        return (out[0], out[1]), (out[2], out[3])
+
+    def set_volume_format(self, num_disparities, fmt):
+        """Per-direction volume storage for this D: "auto", "u8" or "u16"."""
+        code = {"auto": 0, "u8": 1, "u16": 2}.get(fmt, 0xFFFFFFFF)   # the library rejects others
+        check(lib.vxh_sgm_set_volume_format(self._h, num_disparities, code))
 
     def enqueue(self, d_left, d_right, width, height, config, d_out, d_workspace,
                 workspace_bytes, stream=None):
@@ -313,6 +329,15 @@ class SGMStereo:
         check(lib.vxh_sgm_enqueue(self._h, d_left, d_right, width, height,
                                   ctypes.byref(c), d_out, d_workspace,
                                   workspace_bytes, stream))
+
+    def enqueue_marked(self, d_left, d_right, width, height, config, d_out, d_workspace,
+                       workspace_bytes, stream, events):
+        """enqueue recording events[0..2] after census, aggregation and winner."""
+        c = config.to_c()
+        ev = (_vp * 3)(*[e.handle for e in events])
+        check(lib.vxh_sgm_enqueue_marked(self._h, d_left, d_right, width, height,
+                                         ctypes.byref(c), d_out, d_workspace,
+                                         workspace_bytes, stream, ev))
 
     def close(self):
         if self._h:
@@ -323,7 +348,12 @@ class SGMStereo:
 class vxh_sgm_pair_desc(ctypes.Structure):
     _fields_ = [("d_left", _vp), ("d_right", _vp), ("d_out", _vp), ("width", _u32),
                 ("height", _u32), ("cfg", vxh_sgm_config), ("lanes", _u32),
-                ("per_lane", _u32)]
+                ("per_lane", _u32), ("lanes_h", _u32), ("per_lane_h", _u32),
+                ("volume_format", _u32)]
+
+
+class vxh_sgm_batch_options(ctypes.Structure):
+    _fields_ = [("blocks_per_cu", _u32), ("wta_lag", _u32), ("wta_pixels", _u32)]
 
 
 class vxh_sgm_batch_stats(ctypes.Structure):
@@ -332,6 +362,9 @@ class vxh_sgm_batch_stats(ctypes.Structure):
 
 _sig("vxh_sgm_batch_create", ctypes.c_int, ctypes.c_int, ctypes.POINTER(vxh_sgm_pair_desc),
      _u32, ctypes.POINTER(_vp))
+_sig("vxh_sgm_batch_create_ex", ctypes.c_int, ctypes.c_int,
+     ctypes.POINTER(vxh_sgm_pair_desc), _u32, ctypes.POINTER(vxh_sgm_batch_options),
+     ctypes.POINTER(_vp))
 _sig("vxh_sgm_batch_launch", ctypes.c_int, _vp, _vp)
 _sig("vxh_sgm_batch_run", ctypes.c_int, _vp)
 _sig("vxh_sgm_batch_stats_get", ctypes.c_int, _vp, _u32, ctypes.POINTER(vxh_sgm_batch_stats))
@@ -350,15 +383,27 @@ class SGMBatch:
     """Many pairs in one persistent launch replayed from a hipGraph.
 
     `pairs` is a list of (d_left, d_right, d_out, width, height, config,
-    schedule) with device addresses; schedule is (lanes, per_lane) or None."""
+    schedule[, volume_format]) with device addresses; schedule is
+    (lanes, per_lane) for every direction, ((lanes_h, per_lane_h),
+    (lanes, per_lane)), or None; volume_format "auto", "u8" or "u16".
+    `options` is (blocks_per_cu, wta_lag, wta_pixels) or None (defaults)."""
 
-    def __init__(self, pairs, device=0):
+    def __init__(self, pairs, device=0, options=None):
         arr = (vxh_sgm_pair_desc * len(pairs))()
-        for i, (l, r, o, w, h, cfg, sched) in enumerate(pairs):
-            lanes, per = sched if sched else (0, 0)
-            arr[i] = vxh_sgm_pair_desc(l, r, o, w, h, cfg.to_c(), lanes, per)
+        for i, desc in enumerate(pairs):
+            l, r, o, w, h, cfg, sched = desc[:7]
+            fmt = {"auto": 0, "u8": 1, "u16": 2}[desc[7] if len(desc) > 7 else "auto"]
+            if not sched:
+                hs = os = (0, 0)
+            elif isinstance(sched[0], (tuple, list)):
+                hs, os = sched
+            else:
+                hs = os = sched
+            arr[i] = vxh_sgm_pair_desc(l, r, o, w, h, cfg.to_c(), os[0], os[1], hs[0],
+                                       hs[1], fmt)
         h = _vp()
-        _check_batch(lib.vxh_sgm_batch_create(device, arr, len(pairs), ctypes.byref(h)))
+        opt = ctypes.byref(vxh_sgm_batch_options(*options)) if options else None
+        _check_batch(lib.vxh_sgm_batch_create_ex(device, arr, len(pairs), opt, ctypes.byref(h)))
         self._h = h
         self.n = len(pairs)
 
@@ -507,11 +552,9 @@ def mixture_pair(w, h, d, pair_seed):
     return left, right, cfg
 
 
-def workspace_bytes(width, height, config, groups=1):
+def workspace_bytes(width, height, config):
     c = config.to_c()
-    if groups == 1:
-        return lib.vxh_sgm_workspace_bytes(width, height, ctypes.byref(c))
-    return lib.vxh_sgm_workspace_bytes_groups(width, height, ctypes.byref(c), groups)
+    return lib.vxh_sgm_workspace_bytes(width, height, ctypes.byref(c))
 
 
 def synth_pair(width, height, num_disparities, n_rects, n_planes, seed):

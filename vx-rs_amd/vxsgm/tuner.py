@@ -1,16 +1,21 @@
-"""Autotuner for the aggregation schedule (lane layout) of each pair class.
+"""Autotuner for the aggregation schedule of each pair class.
 
-A schedule is (lanes, per_lane): `lanes` lanes of a wave own one scanline and
-each owns `per_lane` consecutive disparities.  Results never depend on it;
-speed does, and differently for the two forms: a lone pair is bound by the
-length of its longest scanline (few, wide groups win), a batch by total
-instruction throughput (more disparities per lane win).  The tuner therefore
-times every candidate in the form it will serve:
+A lane layout is (lanes, per_lane): `lanes` lanes of a wave own a pair of
+scanlines and each lane owns `per_lane` consecutive disparities of both;
+lanes * per_lane is 64, 128 or 256 for D <= 64, <= 128, <= 256, giving three
+layouts per class (two for D > 128).  Results never depend on the schedule;
+speed does, and differently for the two forms:
 
-  tune_batch(w, h, D, paths)  a batch of copies of one class, persistent
-                              kernel replayed from its hipGraph;
-  tune_plain(w, h, D, paths)  the per-pair pipeline of one pair, also over
-                              the number of concurrent path groups.
+  tune_plain(sgm, w, h, D, paths)  a lone pair is bound by the length of its
+      longest scanline: the schedule gives one layout to the horizontal
+      directions (the longest scanlines; wide groups cut the critical path)
+      and one to the others, and every pair of layouts is timed through the
+      per-pair pipeline.
+  tune_batch(pairs)  a batch is bound by instruction throughput, so its
+      persistent kernel always runs 8 disparities per lane; what is tuned is
+      the launch schedule: resident workgroups per CU, how far behind the
+      aggregation each pair's sum + winner items are placed, and the pixels
+      per such item.  Timed on the batch itself, replayed from its hipGraph.
 
 Timings use HIP events on the launch stream; the median of `reps` replays
 after one warm-up wins.  Winners are cached in a JSON file keyed by form and
@@ -23,18 +28,42 @@ import statistics
 import vxsgm
 from vxsgm import hip
 
-LAYOUTS = [(64, 1), (64, 2), (64, 4), (32, 2), (32, 4), (32, 8), (16, 4), (16, 8),
-           (16, 16)]
+CLASS_LAYOUTS = [[(32, 2), (16, 4), (8, 8)], [(64, 2), (32, 4), (16, 8)],
+                 [(64, 4), (32, 8)]]
+# Persistent-launch schedules tried for a batch: (blocks_per_cu, wta_lag,
+# wta_pixels).
+BATCH_OPTIONS = [(2, 2, 1024), (1, 2, 1024), (2, 1, 1024), (2, 4, 1024), (2, 2, 4096),
+                 (2, 2, 256)]
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 
+def layout_class(D):
+    return 0 if D <= 64 else (1 if D <= 128 else 2)
+
+
 def candidates(D):
-    """Layouts that cover D without more than doubling it (padding lanes)."""
-    return [(L, K) for L, K in LAYOUTS if L * K >= D and L * K <= 2 * max(D, 32)]
+    """The lane layouts of D's class."""
+    return list(CLASS_LAYOUTS[layout_class(D)])
+
+
+def schedules(D):
+    """Every (horizontal, other) layout pair of D's class."""
+    c = candidates(D)
+    return [(a, b) for a in c for b in c]
 
 
 def _key(form, w, h, D, paths):
     return f"{form}:{w}x{h}:D{D}:P{paths}"
+
+
+def _name(sched):
+    (a, b), (c, d) = sched
+    return f"{a}x{b}/{c}x{d}"
+
+
+def _parse(name):
+    hs, os_ = name.split("/")
+    return tuple(int(v) for v in hs.split("x")), tuple(int(v) for v in os_.split("x"))
 
 
 class Tuner:
@@ -52,74 +81,59 @@ class Tuner:
 
     def lookup(self, form, w, h, D, paths):
         e = self.table.get(_key(form, w, h, D, paths))
-        return tuple(e["schedule"]) if e else None
+        return _parse(e["schedule"]) if e else None
 
-    def tune_batch(self, w, h, D, paths, copies=4, reps=5, force=False):
-        key = _key("batch", w, h, D, paths)
+    def _time(self, launch, stream, reps):
+        launch()
+        ms = []
+        for _ in range(reps):
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record(stream)
+            launch()
+            e1.record(stream)
+            e1.synchronize()
+            ms.append(e0.elapsed_ms(e1))
+        return statistics.median(ms)
+
+    def tune_batch(self, descs, key="batch", reps=3, force=False):
+        """Best launch schedule (blocks_per_cu, wta_lag, wta_pixels) for the
+        batch of pair descriptors `descs` (as for SGMBatch)."""
         if not force and key in self.table:
-            return tuple(self.table[key]["schedule"])
-        left, right, cfg = vxsgm.mixture_pair(w, h, D, 0x5451)
-        cfg.num_paths = paths
-        dl, dr = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
-        outs = [hip.DeviceBuffer(w * h * 4) for _ in range(copies)]
+            return tuple(self.table[key]["options"])
         stream = hip.Stream()
         times = {}
-        for sched in candidates(D):
-            b = vxsgm.SGMBatch([(dl.ptr, dr.ptr, o.ptr, w, h, cfg, sched) for o in outs])
-            b.launch(stream.handle)
-            ms = []
-            for _ in range(reps):
-                e0, e1 = hip.Event(), hip.Event()
-                e0.record(stream)
-                b.launch(stream.handle)
-                e1.record(stream)
-                e1.synchronize()
-                ms.append(e0.elapsed_ms(e1))
+        for opt in BATCH_OPTIONS:
+            b = vxsgm.SGMBatch(descs, options=opt)
+            times["%d/%d/%d" % opt] = self._time(lambda: b.launch(stream.handle), stream, reps)
             b.close()
-            times[f"{sched[0]}x{sched[1]}"] = statistics.median(ms)
         best = min(times, key=times.get)
-        sched = tuple(int(v) for v in best.split("x"))
-        self.table[key] = {"schedule": list(sched), "ms": times}
-        return sched
+        opt = tuple(int(v) for v in best.split("/"))
+        self.table[key] = {"options": list(opt), "ms": times}
+        return opt
 
     def tune_plain(self, sgm, w, h, D, paths, reps=5, force=False):
-        """Sweeps lane layout x concurrent path groups (1, 2, 4, 8) for one pair
-        and leaves the winner set on `sgm`; returns (schedule, groups)."""
+        """Sweeps every schedule of the class for one pair through the
+        per-pair pipeline and leaves the winner set on `sgm`; returns it."""
         key = _key("plain", w, h, D, paths)
         if not force and key in self.table:
-            e = self.table[key]
-            sched, groups = tuple(e["schedule"]), e.get("groups", 1)
-            sgm.set_layout(D, sched)
-            sgm.set_groups(D, groups)
-            return sched, groups
+            sched = _parse(self.table[key]["schedule"])
+            sgm.set_schedule(D, sched)
+            return sched
         left, right, cfg = vxsgm.mixture_pair(w, h, D, 0x5452)
         cfg.num_paths = paths
         dl, dr = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
         out = hip.DeviceBuffer(w * h * 4)
-        group_opts = [g for g in (1, 2, 4, 8) if g <= paths]
-        wsb = vxsgm.workspace_bytes(w, h, cfg, groups=max(group_opts))
+        wsb = vxsgm.workspace_bytes(w, h, cfg)
         ws = hip.DeviceBuffer(wsb)
         stream = hip.Stream()
         times = {}
-        for sched in candidates(D):
-            sgm.set_layout(D, sched)
-            for g in group_opts:
-                sgm.set_groups(D, g)
-                sgm.enqueue(dl.ptr, dr.ptr, w, h, cfg, out.ptr, ws.ptr, wsb, stream.handle)
-                ms = []
-                for _ in range(reps):
-                    e0, e1 = hip.Event(), hip.Event()
-                    e0.record(stream)
-                    sgm.enqueue(dl.ptr, dr.ptr, w, h, cfg, out.ptr, ws.ptr, wsb,
-                                stream.handle)
-                    e1.record(stream)
-                    e1.synchronize()
-                    ms.append(e0.elapsed_ms(e1))
-                times[f"{sched[0]}x{sched[1]}/g{g}"] = statistics.median(ms)
+        for sched in schedules(D):
+            sgm.set_schedule(D, sched)
+            times[_name(sched)] = self._time(
+                lambda: sgm.enqueue(dl.ptr, dr.ptr, w, h, cfg, out.ptr, ws.ptr, wsb,
+                                    stream.handle), stream, reps)
         best = min(times, key=times.get)
-        lk, g = best.split("/g")
-        sched, groups = tuple(int(v) for v in lk.split("x")), int(g)
-        sgm.set_layout(D, sched)
-        sgm.set_groups(D, groups)
-        self.table[key] = {"schedule": list(sched), "groups": groups, "ms": times}
-        return sched, groups
+        sched = _parse(best)
+        sgm.set_schedule(D, sched)
+        self.table[key] = {"schedule": best, "ms": times}
+        return sched

@@ -63,6 +63,9 @@ def parse():
                          "the built-in default")
     ap.add_argument("--volume-format", default="auto", choices=["auto", "u8", "u16"],
                     help="per-direction volume storage of the plain path")
+    ap.add_argument("--batch-options", default="",
+                    help="launch schedule of the batched form, blocks_per_cu,wta_lag,"
+                         "wta_pixels (default: tuned, or the built-in default with --no-tune)")
     return ap.parse_args()
 
 
@@ -82,7 +85,10 @@ def batched_leg(vx, hip, np, args, sgm, tuner):
         ob, op = hip.DeviceBuffer(w * h * 4), hip.DeviceBuffer(w * h * 4)
         bufs.append((dl, dr, ob, op))
         descs.append((dl.ptr, dr.ptr, ob.ptr, w, h, cfg, None))
-    opts = None if tuner is None else tuner.tune_batch(descs)
+    if args.batch_options:
+        opts = tuple(int(v) for v in args.batch_options.split(","))
+    else:
+        opts = None if tuner is None else tuner.tune_batch(descs)
     log(f"batched: {len(host)} pairs, launch schedule {opts}")
     stream = hip.Stream()
     batch = vx.SGMBatch(descs, options=opts)
@@ -228,6 +234,7 @@ def main():
         tuner.tune_plain(sgm, w, h, D, paths)
         log(f"plain schedules: {tuner.table}")
     plain_sched = sgm.schedule(D)
+    plain_split = sgm.split(D)
 
     for _ in range(args.warmup):
         sgm.enqueue(d_left.ptr, d_right.ptr, w, h, cfg, d_out.ptr, ws.ptr, ws_bytes,
@@ -271,7 +278,7 @@ def main():
                    "num_disparities": D, "num_paths": paths, "p1": cfg.p1,
                    "p2": cfg.p2, "census": "9x7", "form": "plain per-pair",
                    "schedule": {"horizontal": list(plain_sched[0]),
-                                "other": list(plain_sched[1])},
+                                "other": list(plain_sched[1]), "split": plain_split},
                    "volume_format": args.volume_format},
         "roofline": {"bound": "hbm", "kernel": "paths_kernel (aggregation, all directions)",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -115,11 +115,26 @@ def test_synth_pair_deterministic_and_sane():
     assert a[2].min() >= 0 and a[2].max() <= 15
 
 
-def test_server_request_bytes_is_volume_size():
-    cfg = vxsgm.SGMStereoConfig(num_disparities=100)
-    assert vxsgm.request_bytes(640, 480, cfg) == 640 * 480 * 112 * 2
-    cfg = vxsgm.SGMStereoConfig(num_disparities=128)
-    assert vxsgm.request_bytes(1920, 1080, cfg) == 1920 * 1080 * 128 * 2
+def align256(v):
+    return (v + 255) // 256 * 256
+
+
+@pytest.mark.parametrize("w,h,D,paths,p2", [(640, 480, 100, 2, 120.0), (1920, 1080, 128, 8, 120.0),
+                                            (320, 240, 64, 4, 400.0), (7, 5, 3, 8, 120.0)])
+def test_server_request_bytes_is_workspace(w, h, D, paths, p2):
+    """A served request is admitted against its whole pipeline workspace: census
+    of both images (the left one at least 2 KiB, the guard the aggregation may
+    read in front of the right one), the exception flags (8 bytes per pixel),
+    the split launch's synchronisation words (ticket, error flag, one flag per
+    tail wave), and per direction a u8 volume plus its high-byte volume, or a
+    u16 volume; the larger of the two formats is reserved."""
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=paths, p2=p2)
+    n, Dp = w * h, (D + 15) // 16 * 16
+    sync = align256(4 * (2 + 8 * ((w + h) // 2 + 1)))
+    u8 = align256(max(n * 8, 2048)) + 2 * align256(n * 8) + sync + 2 * paths * align256(n * Dp)
+    u16 = align256(max(n * 8, 2048)) + align256(n * 8) + sync + paths * align256(n * Dp * 2)
+    assert vxsgm.workspace_bytes(w, h, cfg) == max(u8, u16)
+    assert vxsgm.request_bytes(w, h, cfg) == max(u8, u16)
 
 
 @pytest.mark.parametrize("kw", [dict(hbm_budget_bytes=0), dict(max_inflight=0),

@@ -341,6 +341,46 @@ def test_schedule_setting_validated(ctx):
         s.close()
 
 
+@pytest.mark.parametrize("w,h,D", [(200, 130, 64), (110, 300, 100), (256, 96, 128), (300, 40, 256),
+                                   (96, 96, 16)])
+@pytest.mark.parametrize("fmt", ["u8", "u16"])
+def test_split_scanlines_match_oracle(ctx, w, h, D, fmt):
+    """Scanlines of 96 pixels or more run as a head and a tail half at once;
+    the head's overwrite of the tail's first steps must leave the oracle's map,
+    with and without the LR check, and identical to the unsplit run."""
+    left, right, _ = vxsgm.synth_pair(w, h, D, 4, 1, 5000 + w + h + D)
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        s.set_volume_format(D, fmt)
+        for lr in (False, True):
+            cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=8, lr_check=lr)
+            want = O.sgm(left, right, D, num_paths=8, lr_check=lr)
+            s.set_split(D, True)
+            assert s.split(D)
+            assert_same_disp(s.compute_host(left, right, w, h, cfg), want)
+            s.set_split(D, False)
+            assert_same_disp(s.compute_host(left, right, w, h, cfg), want)
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("p2", [300, 2000])
+def test_split_scanlines_with_wraps(ctx, p2):
+    """u8 volumes forced with a large P2: the tail's and head's exception flags
+    and high bytes in the re-run steps must resolve to the oracle's sums."""
+    D, w, h = 64, 220, 110
+    left, right, _ = vxsgm.synth_pair(w, h, D, 5, 1, 6000 + p2)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, p2=p2, num_paths=8)
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        s.set_volume_format(D, "u8")
+        s.set_split(D, True)
+        got = s.compute_host(left, right, w, h, cfg)
+    finally:
+        s.close()
+    assert_same_disp(got, O.sgm(left, right, D, p2=p2, num_paths=8))
+
+
 def test_720p_d128_8path_bit_exact(ctx, sgm):
     """Workload 720p_d128_8path at full size against the threaded oracle, in
     both volume formats and with the narrowest and widest layouts."""
@@ -354,6 +394,8 @@ def test_720p_d128_8path_bit_exact(ctx, sgm):
         assert_same_disp(s.compute_host(left, right, 1280, 720, cfg), want)
         s.set_volume_format(128, "auto")
         s.set_schedule(128, ((32, 4), (64, 2)))
+        assert_same_disp(s.compute_host(left, right, 1280, 720, cfg), want)
+        s.set_split(128, False)
         assert_same_disp(s.compute_host(left, right, 1280, 720, cfg), want)
     finally:
         s.close()

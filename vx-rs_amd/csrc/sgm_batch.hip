@@ -34,7 +34,8 @@ namespace {
 
 using namespace dev;
 
-constexpr int kMaxPhase = 4;   // census, paths, wta, lr
+constexpr int kMaxPhase = 4;      // census, paths, wta, lr
+constexpr int kLaunchKinds = 4;   // (u8, u16) x (D % 8 == 0, else)
 
 enum ItemKind : int { kCensusL = 0, kCensusR = 1, kAgg = 2, kWta = 3, kLr = 4 };
 
@@ -113,15 +114,20 @@ __device__ __forceinline__ bool wave_wait(const uint32_t *ctr, uint32_t target,
 // waves per SIMD.
 constexpr int kBatchLayout[kLayoutClasses] = {2, 2, 1};
 
-template <int C>
+template <int C, int MODE, bool ALIGNED>
 __device__ __forceinline__ void agg_item(const PassParams &p, int first) {
     constexpr int L = kClassLayouts[C][kBatchLayout[C]][0];
     constexpr int K = kClassLayouts[C][kBatchLayout[C]][1];
     static_assert(K == 8, "batch layouts hold 8 disparities per lane");
-    agg_pair_wave<L, K, -1>(p, first);
+    agg_pair_wave<L, K, MODE, ALIGNED>(p, first);
 }
 
-__global__ void __launch_bounds__(256)
+// One persistent launch runs the pairs of one volume format (MODE) whose D
+// is (ALIGNED) or is not a multiple of 8, so that it carries a single
+// aggregation body per layout class and keeps two waves per SIMD resident
+// (the persistent grid is sized for them).
+template <int MODE, bool ALIGNED>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 batch_kernel(const PairDesc *__restrict__ pairs, const Item *__restrict__ items,
              int nitems, uint32_t *next, uint32_t *done,
              const uint32_t *__restrict__ need, uint32_t *error) {
@@ -162,20 +168,22 @@ batch_kernel(const PairDesc *__restrict__ pairs, const Item *__restrict__ items,
             p.dx = dir_x(it.dir);
             p.dy = dir_y(it.dir);
             p.nscan = dir_nscan(it.dir, pd.w, pd.h);
-            p.mode = pd.mode;
-            if (pd.cls == 0) agg_item<0>(p, it.first);
-            else if (pd.cls == 1) agg_item<1>(p, it.first);
-            else agg_item<2>(p, it.first);
+            p.mode = MODE;
+            if (pd.cls == 0) agg_item<0, MODE, ALIGNED>(p, it.first);
+            else if (pd.cls == 1) agg_item<1, MODE, ALIGNED>(p, it.first);
+            else agg_item<2, MODE, ALIGNED>(p, it.first);
         } else if (it.kind == kWta) {
-            // Four pixels per wave iteration, 16 lanes each.
+            // Four pixels per wave step, 16 lanes each.
             const int npx = pd.w * pd.h;
             const int end = it.first + it.count;
-            for (int base = it.first; base < end; base += 4) {
-                const int pix = base + (lane >> 4);
-                wta_sum_pixel(pd.vol, pd.vol_bytes, pd.hi, pd.flags, pd.P, pd.Dp > 128 ? 2 : 1,
-                              pd.mode, pix < end ? pix : npx, npx, pd.D, pd.Dp, pd.out, pd.best,
-                              pd.sum);
-            }
+            const int nc = pd.Dp > 128 ? 2 : 1;
+            auto px = [&](int b) {
+                const int q = b + (lane >> 4);
+                return q < end ? q : npx;
+            };
+            for (int base = it.first; base < end; base += 4)
+                wta_sum_pixel(pd.vol, pd.vol_bytes, pd.hi, pd.flags, pd.P, nc, MODE, px(base),
+                              npx, pd.D, pd.Dp, pd.out, pd.best, pd.sum);
         } else {
             for (int i = it.first + lane; i < it.first + it.count; i += 64)
                 lr_pixel(pd.sum, pd.w, pd.D, pd.Dp, pd.lr_tol, (size_t)i, pd.best, pd.out);
@@ -203,12 +211,13 @@ struct vxh_sgm_batch {
     int npairs = 0;
     int nitems = 0;
     int grid = 0;
+    int nlaunch = 0;             // persistent launches (one per format and alignment)
     void *slab = nullptr;       // per-pair workspaces
     void *meta = nullptr;       // pairs, items, counters
     vxh::PairDesc *d_pairs = nullptr;
     vxh::Item *d_items = nullptr;
     uint32_t *d_need = nullptr;
-    uint32_t *d_counters = nullptr;   // [0] = next, [1] = error, [2..] = done
+    uint32_t *d_counters = nullptr;   // [0..4) = next per launch, [4] = error, [5..] = done
     int ncounters = 0;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -384,8 +393,22 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
         b->stats[i].lanes = (uint32_t)kClassLayouts[pd[i].cls][kBatchLayout[pd[i].cls]][0];
         b->stats[i].per_lane = (uint32_t)kClassLayouts[pd[i].cls][kBatchLayout[pd[i].cls]][1];
     }
+    // One persistent launch per (volume format, D % 8 == 0) class of pairs:
+    // the list is partitioned stably, so each launch keeps the order above.
+    auto launch_of = [&](int p) {
+        return (pd[p].mode == kOutU8 ? 0 : 2) + (pd[p].D % 8 == 0 ? 0 : 1);
+    };
+    std::vector<Item> sorted;
+    int lstart[kLaunchKinds + 1] = {};
+    for (int k = 0; k < kLaunchKinds; ++k) {
+        lstart[k] = (int)sorted.size();
+        for (const Item &it : items)
+            if (launch_of(it.pair) == k) sorted.push_back(it);
+    }
+    lstart[kLaunchKinds] = (int)sorted.size();
+    items.swap(sorted);
     b->nitems = (int)items.size();
-    b->ncounters = 2 + (int)n * kMaxPhase;
+    b->ncounters = kLaunchKinds + 1 + (int)n * kMaxPhase;
     const size_t meta = align256(n * sizeof(PairDesc)) + align256(items.size() * sizeof(Item)) +
                         align256(need.size() * 4) + align256((size_t)b->ncounters * 4);
     if (hipMalloc(&b->meta, meta) != hipSuccess) {
@@ -422,9 +445,23 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
     if (e == hipSuccess) {
         hipLaunchKernelGGL(batch_reset_kernel, dim3((b->ncounters + 255) / 256), dim3(256), 0,
                            b->stream, b->d_counters, b->ncounters);
-        hipLaunchKernelGGL(batch_kernel, dim3(b->grid), dim3(256), 0, b->stream, b->d_pairs,
-                           b->d_items, b->nitems, b->d_counters, b->d_counters + 2, b->d_need,
-                           b->d_counters + 1);
+        uint32_t *const err = b->d_counters + kLaunchKinds;
+        uint32_t *const done = b->d_counters + kLaunchKinds + 1;
+        for (int k = 0; k < kLaunchKinds; ++k) {
+            const int cnt = lstart[k + 1] - lstart[k];
+            if (cnt == 0) continue;
+            const Item *its = b->d_items + lstart[k];
+            uint32_t *next = b->d_counters + k;
+#define VXH_BATCH_LAUNCH(MM, AL)                                                             \
+    hipLaunchKernelGGL((batch_kernel<MM, AL>), dim3(b->grid), dim3(256), 0, b->stream,       \
+                       b->d_pairs, its, cnt, next, done, b->d_need, err)
+            if (k == 0) VXH_BATCH_LAUNCH(kOutU8, true);
+            else if (k == 1) VXH_BATCH_LAUNCH(kOutU8, false);
+            else if (k == 2) VXH_BATCH_LAUNCH(kOutU16, true);
+            else VXH_BATCH_LAUNCH(kOutU16, false);
+#undef VXH_BATCH_LAUNCH
+            ++b->nlaunch;
+        }
         e = hipStreamEndCapture(b->stream, &b->graph);
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
@@ -457,7 +494,7 @@ vxh_status vxh_sgm_batch_run(vxh_sgm_batch *b) {
 vxh_status vxh_sgm_batch_check(vxh_sgm_batch *b) {
     if (!b) return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: null batch");
     uint32_t err = 0;
-    hipError_t e = hipMemcpy(&err, b->d_counters + 1, 4, hipMemcpyDeviceToHost);
+    hipError_t e = hipMemcpy(&err, b->d_counters + vxh::kLaunchKinds, 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess)
         return bfail(VXH_ERR_GPU, std::string("GPU error: batch check: ") + hipGetErrorString(e));
     if (err)

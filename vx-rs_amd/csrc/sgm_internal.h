@@ -76,7 +76,13 @@ inline int layout_class(int D) { return D <= 64 ? 0 : (D <= 128 ? 1 : 2); }
 struct Schedule {
     int horiz;   // index into kClassLayouts[layout_class(D)]
     int other;
+    int split;   // 1: long scanlines run as head and tail halves at once
 };
+
+// Words of the split launch's synchronisation area for a w x h pair: the
+// ticket counter, the error flag and one completion flag per tail wave (at
+// most 8 passes of ceil(scanlines / 2) waves).
+inline size_t sync_words(uint32_t w, uint32_t h) { return 2 + 8 * ((size_t)(w + h) / 2 + 1); }
 
 constexpr size_t kWsAlign = 256;
 inline size_t ws_align(size_t v) { return (v + kWsAlign - 1) & ~(kWsAlign - 1); }
@@ -87,7 +93,7 @@ inline size_t ws_align(size_t v) { return (v + kWsAlign - 1) & ~(kWsAlign - 1); 
 // only, also vol_bytes apart), and for the LR check the summed u16 volume and
 // the winner index.
 struct WsLayout {
-    size_t lc, rc, flags, vol, vol_bytes, hi, sum, best, total;
+    size_t lc, rc, flags, sync, vol, vol_bytes, hi, sum, best, total;
 };
 inline WsLayout workspace_layout(uint32_t w, uint32_t h, uint32_t D, int paths, int mode,
                                  bool lr) {
@@ -98,7 +104,8 @@ inline WsLayout workspace_layout(uint32_t w, uint32_t h, uint32_t D, int paths, 
     l.lc = 0;
     l.rc = l.lc + census_span(n);
     l.flags = l.rc + ws_align(n * 8);
-    l.vol = l.flags + (u8 ? ws_align(n * 8) : 0);
+    l.sync = l.flags + (u8 ? ws_align(n * 8) : 0);
+    l.vol = l.sync + ws_align(sync_words(w, h) * 4);
     l.vol_bytes = ws_align(n * Dp * (u8 ? 1 : 2));
     l.hi = l.vol + (size_t)paths * l.vol_bytes;
     l.sum = l.hi + (u8 ? (size_t)paths * l.vol_bytes : 0);
@@ -124,25 +131,38 @@ struct PassDesc {
     uint8_t *vol, *hi;
     int dir, dx, dy, nscan;
     int layout;
-    int first_block, nblocks;
+    int first_block, nblocks;   // unsplit launches: blocks of this pass
+    int first_ticket, nwaves;   // split launches: the pass's tickets start, waves per role
+    int split;                  // split launches: this pass runs tail and head roles
 };
 struct MultiPass {
     PassParams base;       // shared fields; vol/hi/dir/dx/dy/nscan come per pass
     int npass;
+    int split;             // some pass runs as tail and head roles (ticketed launch)
+    int total_waves;       // split: tickets over all passes
+    uint32_t *sync;        // split: [0] ticket, [1] error, [2 + t] done flag of tail ticket t
     PassDesc pass[kMaxPasses];
 };
 
+
 // Census of one image; with `zero_flags` also clears the 8 flag bytes of
-// every pixel (the kOutU8 exception flags of the pass that follows).
+// every pixel (the kOutU8 exception flags of the pass that follows), with
+// `zero_words` the first nwords words there (the split launch's
+// synchronisation area, which launch_paths expects cleared).
 hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
-                         uint2 *out, uint2 *zero_flags, hipStream_t s);
+                         uint2 *out, uint2 *zero_flags, uint32_t *zero_words, size_t nwords,
+                         hipStream_t s);
+// Zeroes n words with a kernel (safe inside captured graphs).
+hipError_t launch_zero(uint32_t *p, size_t n, hipStream_t s);
 // Default schedule for D disparities.
 Schedule default_schedule(int D);
 bool schedule_valid(Schedule sc, int D);
 // Runs the passes dirs[0..n) at once, pass j into vols[j] (and his[j] for
-// kOutU8).  base supplies the shared fields.
+// kOutU8).  base supplies the shared fields; `sync` (sync_words words,
+// cleared beforehand) enables the split launch.
 hipError_t launch_paths(const PassParams &base, const int *dirs, uint8_t *const *vols,
-                        uint8_t *const *his, int n, Schedule sc, int mode, hipStream_t s);
+                        uint8_t *const *his, int n, Schedule sc, int mode, uint32_t *sync,
+                        hipStream_t s);
 // Sums `nvol` per-direction volumes (vol0 + k * vol_stride bytes, k < nvol;
 // kOutU8 adds 256 * the high bytes of flagged slices, his at the same
 // offsets from hi0) mod 2^16 and picks each pixel's winner with parabola

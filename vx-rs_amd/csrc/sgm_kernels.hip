@@ -11,13 +11,15 @@ using namespace dev;
 
 __global__ void __launch_bounds__(256)
 census_kernel(const uint8_t *__restrict__ img, int w, int h, int rx, int ry,
-              uint2 *__restrict__ out, uint2 *__restrict__ zero_flags) {
+              uint2 *__restrict__ out, uint2 *__restrict__ zero_flags,
+              uint32_t *__restrict__ zero_words, size_t nwords) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= w || y >= h) return;
     const size_t i = (size_t)y * w + x;
     out[i] = census_pixel(img, w, h, rx, ry, x, y);
     if (zero_flags) zero_flags[i] = make_uint2(0, 0);
+    for (size_t k = i; k < nwords; k += (size_t)w * h) zero_words[k] = 0;
 }
 
 // Field `f` of pass i (i wave-uniform): unrolled selects, so the kernel
@@ -37,6 +39,76 @@ census_kernel(const uint8_t *__restrict__ img, int w, int h, int rx, int ry,
 // index is permuted so that the blocks the hardware places on one XCD
 // (b = x mod 8) take a contiguous range of scanlines, whose census rows then
 // stay in that XCD's L2.
+template <int C, int MODE, bool SPLIT>
+__device__ __forceinline__ void run_layout(const PassParams &p, int layout, int wave, int role,
+                                           const uint32_t *tail_done, uint32_t *error) {
+    constexpr int L0 = kClassLayouts[C][0][0], K0 = kClassLayouts[C][0][1];
+    constexpr int L1 = kClassLayouts[C][1][0], K1 = kClassLayouts[C][1][1];
+    constexpr int L2 = kClassLayouts[C][2][0], K2 = kClassLayouts[C][2][1];
+#define VXH_RUN(LL, KK)                                                                       \
+    if (p.D % KK == 0)                                                                        \
+        agg_pair_wave<LL, KK, MODE, true, SPLIT>(p, wave * (128 / LL), role, tail_done, error); \
+    else                                                                                      \
+        agg_pair_wave<LL, KK, MODE, false, SPLIT>(p, wave * (128 / LL), role, tail_done, error)
+    if (layout == 0) {
+        VXH_RUN(L0, K0);
+    } else if (layout == 1 || kClassCount[C] < 3) {
+        VXH_RUN(L1, K1);
+    } else {
+        VXH_RUN(L2, K2);
+    }
+#undef VXH_RUN
+}
+
+// Pass i's per-pass fields into p (i wave-uniform).
+__device__ __forceinline__ int pass_params(const MultiPass &m, int i, PassParams &p) {
+    p = m.base;
+    p.vol = VXH_PASS_FIELD(m, i, vol);
+    p.hi = VXH_PASS_FIELD(m, i, hi);
+    p.dir = VXH_PASS_FIELD(m, i, dir);
+    p.dx = VXH_PASS_FIELD(m, i, dx);
+    p.dy = VXH_PASS_FIELD(m, i, dy);
+    p.nscan = VXH_PASS_FIELD(m, i, nscan);
+    return VXH_PASS_FIELD(m, i, layout);
+}
+
+// Split launch: every wave takes a ticket in the order waves start.  Pass j
+// owns tickets [first_ticket, first_ticket + nwaves * (split ? 2 : 1)): a
+// split pass's first nwaves tickets are its tail roles and the next nwaves
+// its head roles, an unsplit pass's are whole scanlines.  A head therefore
+// starts only after its tail has started, so its wait for the tail's
+// completion flag always ends; passes are ordered longest first, so the
+// split ones (the longest) start first and their roles run side by side.
+template <int C, int MODE>
+__global__ void __launch_bounds__(256)
+paths_split_kernel(MultiPass m) {
+    const int lane = threadIdx.x & 63;
+    uint32_t *const sync = m.sync;
+    const uint32_t tk = __hip_atomic_fetch_add(sync, lane == 0 ? 1u : 0u, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+    const int ticket = __builtin_amdgcn_readfirstlane((int)tk);
+    if (ticket >= m.total_waves) return;
+    int i = 0;
+#pragma unroll
+    for (int j = 1; j < kMaxPasses; ++j)
+        if (j < m.npass && ticket >= m.pass[j].first_ticket) i = j;
+    PassParams p;
+    const int layout = pass_params(m, i, p);
+    const int local = ticket - VXH_PASS_FIELD(m, i, first_ticket);
+    const int nw = VXH_PASS_FIELD(m, i, nwaves);
+    const bool split = VXH_PASS_FIELD(m, i, split) != 0;
+    const int role = !split ? kRoleWhole : (local < nw ? kRoleTail : kRoleHead);
+    const int wave = local < nw ? local : local - nw;
+    // The tail of this scanline group holds ticket first_ticket + wave.
+    const int tail_ticket = ticket - local + wave;
+    run_layout<C, MODE, true>(p, layout, wave, role, sync + 2 + tail_ticket, sync + 1);
+    if (role == kRoleTail) {
+        // Every store of this tail is visible before its flag.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(sync + 2 + ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 template <int C, int MODE>
 __global__ void __launch_bounds__(256)
 paths_kernel(MultiPass m) {
@@ -48,21 +120,15 @@ paths_kernel(MultiPass m) {
     const int local = b - VXH_PASS_FIELD(m, i, first_block);
     const int per = VXH_PASS_FIELD(m, i, nblocks) >> 3;
     const int lb = (local & 7) * per + (local >> 3);
-    PassParams p = m.base;
-    p.vol = VXH_PASS_FIELD(m, i, vol);
-    p.hi = VXH_PASS_FIELD(m, i, hi);
-    p.dir = VXH_PASS_FIELD(m, i, dir);
-    p.dx = VXH_PASS_FIELD(m, i, dx);
-    p.dy = VXH_PASS_FIELD(m, i, dy);
-    p.nscan = VXH_PASS_FIELD(m, i, nscan);
-    const int layout = VXH_PASS_FIELD(m, i, layout);
+    PassParams p;
+    const int layout = pass_params(m, i, p);
     const int wave = lb * 4 + (int)(threadIdx.x >> 6);
-    constexpr int L0 = kClassLayouts[C][0][0], K0 = kClassLayouts[C][0][1];
-    constexpr int L1 = kClassLayouts[C][1][0], K1 = kClassLayouts[C][1][1];
-    constexpr int L2 = kClassLayouts[C][2][0], K2 = kClassLayouts[C][2][1];
-    if (layout == 0) agg_pair_wave<L0, K0, MODE>(p, wave * (128 / L0));
-    else if (layout == 1 || kClassCount[C] < 3) agg_pair_wave<L1, K1, MODE>(p, wave * (128 / L1));
-    else agg_pair_wave<L2, K2, MODE>(p, wave * (128 / L2));
+    run_layout<C, MODE, false>(p, layout, wave, kRoleWhole, nullptr, nullptr);
+}
+
+__global__ void __launch_bounds__(256) zero_kernel(uint32_t *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // Sum of NV per-direction volumes (u8 or u16 entries, MODE) and
@@ -103,25 +169,37 @@ int scan_length(int dir, int w, int h) {
 
 }  // namespace
 
+// The fastest schedule of a 1280x720 pair at D = 128, 8 paths, carried over
+// to the other classes: the middle layout for the horizontal directions, the
+// widest per-lane layout for the others, scanlines not split.
 Schedule default_schedule(int D) {
-    return layout_class(D) == 2 ? Schedule{0, 1} : Schedule{0, 2};
+    return layout_class(D) == 2 ? Schedule{0, 1, 0} : Schedule{1, 2, 0};
 }
 
 bool schedule_valid(Schedule sc, int D) {
     const int n = kClassCount[layout_class(D)];
-    return sc.horiz >= 0 && sc.horiz < n && sc.other >= 0 && sc.other < n;
+    return sc.horiz >= 0 && sc.horiz < n && sc.other >= 0 && sc.other < n &&
+           (sc.split == 0 || sc.split == 1);
+}
+
+hipError_t launch_zero(uint32_t *p, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
-                         uint2 *out, uint2 *zero_flags, hipStream_t s) {
+                         uint2 *out, uint2 *zero_flags, uint32_t *zero_words, size_t nwords,
+                         hipStream_t s) {
     dim3 block(64, 4);
     dim3 grid((w + 63) / 64, (h + 3) / 4);
-    hipLaunchKernelGGL(census_kernel, grid, block, 0, s, img, w, h, rx, ry, out, zero_flags);
+    hipLaunchKernelGGL(census_kernel, grid, block, 0, s, img, w, h, rx, ry, out, zero_flags,
+                       zero_words, zero_words ? nwords : 0);
     return hipGetLastError();
 }
 
 hipError_t launch_paths(const PassParams &base, const int *dirs, uint8_t *const *vols,
-                        uint8_t *const *his, int n, Schedule sc, int mode, hipStream_t s) {
+                        uint8_t *const *his, int n, Schedule sc, int mode, uint32_t *sync,
+                        hipStream_t s) {
     if (n < 1 || n > kMaxPasses || base.D < 1 || base.D > 256 || !schedule_valid(sc, base.D) ||
         (mode != kOutU8 && mode != kOutU16))
         return hipErrorInvalidValue;
@@ -153,11 +231,33 @@ hipError_t launch_paths(const PassParams &base, const int *dirs, uint8_t *const 
         d.first_block = blocks;
         d.nblocks = (((waves + 3) / 4) + 7) & ~7;
         blocks += d.nblocks;
+        d.nwaves = waves;
+    }
+    // Split the passes whose scanlines are long enough and set the critical
+    // path: longer than 2/3 of the longest (at 1280x720 the horizontal ones,
+    // whose halves then run no longer than the vertical scanlines).
+    int longest = 0;
+    for (int j = 0; j < n; ++j) longest = std::max(longest, scan_length(m.pass[j].dir, base.w, base.h));
+    int tickets = 0, nsplit = 0;
+    for (int j = 0; j < n; ++j) {
+        PassDesc &d = m.pass[j];
+        const int len = scan_length(d.dir, base.w, base.h);
+        d.split = sc.split && len >= kSplitMin && 3 * len > 2 * longest;
+        d.first_ticket = tickets;
+        tickets += d.nwaves * (d.split ? 2 : 1);
+        nsplit += d.split;
     }
     for (int j = n; j < kMaxPasses; ++j) m.pass[j] = m.pass[0];
+    m.split = sync != nullptr && nsplit > 0 && (size_t)tickets + 2 <= sync_words(base.w, base.h);
+    m.total_waves = tickets;
+    m.sync = sync;
+    if (m.split) blocks = (tickets + 3) / 4;
 #define VXH_PATHS_CASE(CC, MM)                                                              \
     if (C == CC && mode == MM) {                                                            \
-        hipLaunchKernelGGL((paths_kernel<CC, MM>), dim3(blocks), dim3(256), 0, s, m);       \
+        if (m.split)                                                                        \
+            hipLaunchKernelGGL((paths_split_kernel<CC, MM>), dim3(blocks), dim3(256), 0, s, m); \
+        else                                                                                \
+            hipLaunchKernelGGL((paths_kernel<CC, MM>), dim3(blocks), dim3(256), 0, s, m);   \
         return hipGetLastError();                                                           \
     }
     VXH_PATHS_CASE(0, kOutU8) VXH_PATHS_CASE(1, kOutU8) VXH_PATHS_CASE(2, kOutU8)

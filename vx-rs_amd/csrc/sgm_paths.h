@@ -34,6 +34,8 @@
 //            u16 path cost exactly.
 #pragma once
 
+#include <type_traits>
+
 #include "sgm_device.h"
 
 namespace vxh {
@@ -111,12 +113,55 @@ struct PairBuf {
     bool edge;   // some active scanline of the wave has x - d < 0 for a lane's d
 };
 
+// Split scanlines (head role): the tail's stored values of one step, both
+// scanlines (this lane's entries: u8 K bytes or u16 2K bytes) and their
+// exception flag bytes.
+template <int K>
+struct TailBuf {
+    uint32_t v[2][(2 * K + 3) / 4];
+    uint32_t flag[2];
+};
+
+// A scanline of length len is split at split_point(len): its tail role runs
+// [split, len) from the same fresh state as any path start, its head role
+// runs [0, split) and then continues into the tail's range until its state
+// equals the tail's stored one bit for bit (every later step then agrees, the
+// recurrence being deterministic), overwriting the tail's values until then.
+constexpr int kSplitMin = 96;
+__device__ __forceinline__ int split_point(int len) { return len >= kSplitMin ? len / 2 : len; }
+
+// Roles of a wave in a split launch.
+constexpr int kRoleWhole = 0, kRoleHead = 1, kRoleTail = 2;
+
+// Bounded wait for a tail's completion flag (seconds of polling): a head
+// that times out raises *error and treats the tail as complete.
+__device__ __forceinline__ void wait_flag(const uint32_t *flag, uint32_t *error) {
+    for (uint32_t n = 0; n < (1u << 22); ++n) {
+        const uint32_t v = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((uint32_t)__builtin_amdgcn_readfirstlane((int)v) != 0) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            return;
+        }
+        __builtin_amdgcn_s_sleep(4);
+    }
+    __hip_atomic_store(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 // One wave aggregates scanlines s_first .. s_first + 2 * (64 / L) - 1 of
 // direction p.dir (those below p.nscan) into the direction's volume, in
-// storage MODE (kOutU8 or kOutU16; -1 = p.mode at run time).  Wave-level:
-// every lane of the wave must call it with the same arguments.
-template <int L, int K, int MODE>
-__device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first) {
+// storage MODE (kOutU8 or kOutU16; -1 = p.mode at run time).  ALIGNED: the
+// caller guarantees D % K == 0, so every lane is either wholly inside D or
+// wholly padding and only the element at D - 1 needs its d+1 term masked.
+// SPLIT: `role` is kRoleHead or kRoleTail of split scanlines (see
+// split_point); a head waits on `tail_done` (set by its tail wave once all
+// the tail's stores are visible) before it reads the tail's values.
+// Wave-level: every lane of the wave must call it with the same arguments.
+template <int L, int K, int MODE, bool ALIGNED, bool SPLIT = false>
+__device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
+                                              int role = kRoleWhole,
+                                              const uint32_t *tail_done = nullptr,
+                                              uint32_t *error = nullptr) {
     static_assert(K >= 2 && K <= 16 && (K & (K - 1)) == 0, "K must be 2, 4, 8 or 16");
     const int lane = threadIdx.x & 63;
     const int g = lane / L, sub = lane % L;
@@ -130,11 +175,22 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first) 
     const uint2 *__restrict__ rcg = (const uint2 *)((const char *)p.rc - kCensusGuard);
 
     int sx[2] = {0, 0}, len[2] = {0, 0}, pix0[2] = {0, 0};
+    // Head role: steps [vfrom, len) of a scanline check the tail's values.
+    int vfrom[2] = {0x7FFFFFFF, 0x7FFFFFFF};
 #pragma unroll
     for (int hh = 0; hh < 2; ++hh) {
         const int s = s_first + 2 * g + hh;
         int sy = 0;
         if (s < p.nscan) scan_start(s, dx, dy, w, h, sx[hh], sy, len[hh]);
+        if (SPLIT && role == kRoleTail) {
+            const int m = split_point(len[hh]);
+            sx[hh] += m * dx;
+            sy += m * dy;
+            len[hh] -= m;
+        } else if (SPLIT && role == kRoleHead) {
+            const int m = split_point(len[hh]);
+            if (m < len[hh]) vfrom[hh] = m;
+        }
         pix0[hh] = sy * w + sx[hh];
     }
     int maxlen = max(len[0], len[1]);
@@ -151,6 +207,7 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first) 
     for (int hh = 0; hh < 2; ++hh)
         vp[hh] = p.vol + ((size_t)pix0[hh] * Dp + d0) * ebytes;
     const ptrdiff_t vstep = (ptrdiff_t)pstep * Dp * (ptrdiff_t)ebytes;
+    int tcur = 0;   // the step vp points at
 
     // o2 is 0xFFFF where d + 1 >= D (SGMStereo.metal:156); entries d >= D
     // are padding and never enter the minimum.
@@ -160,13 +217,37 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first) 
         om[i] = (d0 + i + 1 < D) ? 0u : 0xFFFFFFFFu;
         pad[i] = (d0 + i < D) ? 0u : 0xFFFFFFFFu;
     }
+    // Aligned: a padding lane's minimum is masked once per step.
+    const uint32_t lane_pad = lane_real ? 0u : 0xFFFFFFFFu;
     uint32_t prev[K];
 #pragma unroll
     for (int i = 0; i < K; ++i) prev[i] = 0;
     uint32_t pm = 0;
 
+    // Head role: the first step at which any scanline of the wave reads the
+    // tail's values (the wait for the tail comes before its loads), and the
+    // scanlines whose state has met the tail's (from then on the tail's
+    // values stand and this wave stores nothing more for them).
+    int vfirst = min(vfrom[0], vfrom[1]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) vfirst = min(vfirst, __shfl_xor(vfirst, o));
+    vfirst = __builtin_amdgcn_readfirstlane(vfirst);
+    bool met[2] = {false, false};
+    // Bytes of this lane's stored segment that hold real entries (d < D).
+    uint32_t realmask[(2 * K + 3) / 4];
+#pragma unroll
+    for (int q = 0; q < (2 * K + 3) / 4; ++q) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int e = (q * 4 + b) / (int)ebytes;   // entry index of byte b
+            if (d0 + e < D) m |= 0xFFu << (8 * b);
+        }
+        realmask[q] = m;
+    }
+
     auto issue = [&](PairBuf<K> &b, int t) {
-        bool e = false;
+        uint64_t e = 0;
 #pragma unroll
         for (int hh = 0; hh < 2; ++hh) {
             const bool a = t < len[hh];
@@ -181,12 +262,75 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first) 
                 b.rv[hh][i] = load_census<8 * (K - 1 - i)>(rcg, rb);
             });
             const int x = sx[hh] + t * dx;
-            e = e || (lanes_lt(a ? x : 0x7FFFFFFF, d0 + K - 1) != 0);
+            e |= lanes_lt(a ? x : 0x7FFFFFFF, d0 + K - 1);
         }
-        b.edge = e;
+        // A mask, not a branch: the step that uses it runs PF steps later.
+        b.edge = e != 0;
+    };
+    // Head role: the tail's stored values of step t (loaded only after the
+    // tail has finished; other lanes and steps read the sink).
+    auto issue_tail = [&](TailBuf<K> &tb, int t) {
+        if (t < vfirst) return;   // wave-uniform: only the checking steps load
+        if (t == vfirst) wait_flag(tail_done, error);
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            const bool v = t >= vfrom[hh] && t < len[hh] && lane_store;
+            const uint8_t *src = v ? vp[hh] + (ptrdiff_t)(t - tcur) * vstep
+                                   : (const uint8_t *)store_sink<uint8_t>();
+            if (mode == kOutU16) {
+                if constexpr (K == 2) tb.v[hh][0] = *(const uint32_t *)src;
+                else if constexpr (K == 4) { uint2 q = *(const uint2 *)src; tb.v[hh][0] = q.x; tb.v[hh][1] = q.y; }
+                else {
+#pragma unroll
+                    for (int j = 0; j < K / 8; ++j) {
+                        uint4 q = ((const uint4 *)src)[j];
+                        tb.v[hh][4 * j] = q.x; tb.v[hh][4 * j + 1] = q.y;
+                        tb.v[hh][4 * j + 2] = q.z; tb.v[hh][4 * j + 3] = q.w;
+                    }
+                }
+                tb.flag[hh] = 0;
+            } else {
+                if constexpr (K == 2) tb.v[hh][0] = *(const uint16_t *)src;
+                else if constexpr (K == 4) tb.v[hh][0] = *(const uint32_t *)src;
+                else if constexpr (K == 8) { uint2 q = *(const uint2 *)src; tb.v[hh][0] = q.x; tb.v[hh][1] = q.y; }
+                else { uint4 q = *(const uint4 *)src; tb.v[hh][0] = q.x; tb.v[hh][1] = q.y; tb.v[hh][2] = q.z; tb.v[hh][3] = q.w; }
+                const uint8_t *fp = v ? p.flags + (size_t)(pix0[hh] + t * pstep) * 8 + p.dir
+                                      : (const uint8_t *)store_sink<uint8_t>();
+                tb.flag[hh] = *fp;
+            }
+        }
     };
 
-    auto step = [&](PairBuf<K> &b, int t) {
+    // Slices of step tp with an entry above 255 are rare (only after a wrap).
+    // `high` holds that step's high bytes (OR over the lane's entries); the
+    // wave tests them one step later, when the test no longer waits on the
+    // recurrence, and only then works out which scanline of which group is
+    // affected: their high bytes (still in prev) go to the high-byte volume
+    // and the pixel's flag is set.
+    uint32_t high = 0;
+    uint64_t high_lanes = 0;   // lanes with high != 0, taken when high is set
+    auto flush_high = [&](int tp) {
+        if (high_lanes == 0) return;
+        // `high` was masked to the scanlines that stored at step tp.
+        const uint32_t exc = lane_real ? (((high & 0x0000FF00u) ? 1u : 0u) |
+                                          ((high & 0xFF000000u) ? 2u : 0u))
+                                       : 0u;
+        const uint32_t f = group_or<L>(exc);
+        uint32_t ha[(K + 3) / 4], hb[(K + 3) / 4];
+        bytes_of_pairs<K>(prev, 0x07030501u, ha, hb);
+        const ptrdiff_t hoff = p.hi - p.vol;
+        if ((f & 1u) && lane_store) store_seg<K>(vp[0] - vstep + hoff, ha);
+        if ((f & 2u) && lane_store) store_seg<K>(vp[1] - vstep + hoff, hb);
+        if (sub == 0) {
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh)
+                if (f & (1u << hh)) p.flags[(size_t)(pix0[hh] + tp * pstep) * 8 + p.dir] = 1;
+        }
+        high = 0;
+        high_lanes = 0;
+    };
+
+    auto step = [&](PairBuf<K> &b, TailBuf<K> &tb, int t) {
         const bool act0 = t < len[0], act1 = t < len[1];
         uint32_t C[K];
 #pragma unroll
@@ -206,18 +350,20 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first) 
             }
         }
 
+        // With prev = 0 and pm = 0 before the first step, the recurrence
+        // below gives agg = C there (best = min(0, ...) = 0), the first
+        // pixel's copy of the cost (SGMStereo.metal:152), so no step is
+        // special-cased.
         uint32_t agg[K];
-        if (t == 0) {
-#pragma unroll
-            for (int i = 0; i < K; ++i) agg[i] = C[i];
-        } else {
+        {
             const uint32_t o3 = pk_add(pm, p2x2);
             const uint32_t nb = from_upper_lane<L>(prev[0]);
             uint32_t A[K], B[K];
 #pragma unroll
             for (int i = 0; i < K; ++i) {
                 const uint32_t nx = (i + 1 < K) ? prev[(i + 1) % K] : nb;
-                const uint32_t o2 = pk_add(nx, p1x2) | om[i];
+                uint32_t o2 = pk_add(nx, p1x2);
+                if (!ALIGNED || i == K - 1) o2 |= om[i];
                 A[i] = pk_min(pk_min(prev[i], o2), o3);
                 B[i] = pk_sub(C[i], pm);
             }
@@ -248,12 +394,54 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first) 
 
         uint32_t m = 0xFFFFFFFFu;
 #pragma unroll
-        for (int i = 0; i < K; ++i) m = pk_min(m, agg[i] | pad[i]);
+        for (int i = 0; i < K; ++i) m = pk_min(m, ALIGNED ? agg[i] : (agg[i] | pad[i]));
+        if (ALIGNED) m |= lane_pad;
         pm = group_min_pk<L>(m);
+        // The previous step's high-byte slices, if any (see flush_high),
+        // before prev is overwritten.
+        if (mode == kOutU8) flush_high(t - 1);
 #pragma unroll
         for (int i = 0; i < K; ++i) prev[i] = agg[i];
 
-        const bool st0 = act0 && lane_store, st1 = act1 && lane_store;
+        bool st0 = act0 && lane_store, st1 = act1 && lane_store;
+        uint32_t clear_flag = 0;   // head role: u8 flags this step rewrites (bit per scanline)
+        if (SPLIT && role == kRoleHead && t >= vfirst) {
+            // Compare this step's state with the tail's stored one, scanline
+            // by scanline over the whole group (entries d < D): from the
+            // first step where they agree, the tail's values stand.
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                const bool ver = t >= vfrom[hh] && t < len[hh] && !met[hh];
+                uint32_t diff = 0;
+                if (mode == kOutU16) {
+#pragma unroll
+                    for (int i = 0; i < K / 2; ++i) {
+                        const uint32_t mine = perm(agg[2 * i + 1], agg[2 * i],
+                                                   hh == 0 ? 0x05040100u : 0x07060302u);
+                        diff |= (mine ^ tb.v[hh][i]) & realmask[i];
+                    }
+                } else {
+                    uint32_t lo[(K + 3) / 4], hi[(K + 3) / 4], lo2[(K + 3) / 4], hi2[(K + 3) / 4];
+                    bytes_of_pairs<K>(agg, 0x06020400u, lo, lo2);
+                    bytes_of_pairs<K>(agg, 0x07030501u, hi, hi2);
+#pragma unroll
+                    for (int i = 0; i < (K + 3) / 4; ++i) {
+                        const uint32_t l = hh == 0 ? lo[i] : lo2[i];
+                        const uint32_t hb = hh == 0 ? hi[i] : hi2[i];
+                        diff |= ((l ^ tb.v[hh][i]) | hb) & realmask[i];
+                    }
+                    diff |= tb.flag[hh];
+                }
+                const uint32_t g_diff = group_or<L>(lane_real && ver && diff != 0 ? 1u : 0u);
+                if (ver && g_diff == 0) met[hh] = true;
+            }
+            // Scanlines that have met store nothing more; the others
+            // overwrite the tail's values, and (u8) their flag byte, which the
+            // high-byte pass of the next step sets again if needed.
+            st0 = st0 && !met[0];
+            st1 = st1 && !met[1];
+            clear_flag = (st0 && t >= vfrom[0] ? 1u : 0u) | (st1 && t >= vfrom[1] ? 2u : 0u);
+        }
         if (mode == kOutU16) {
             // Halves to pixel order: [a_i | a_i+1] and [b_i | b_i+1].
             uint32_t wa[K / 2], wb[K / 2];
@@ -269,30 +457,22 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first) 
             bytes_of_pairs<K>(agg, 0x06020400u, wa, wb);
             store_seg<K>(st0 ? vp[0] : store_sink<uint8_t>(), wa);
             store_seg<K>(st1 ? vp[1] : store_sink<uint8_t>(), wb);
-            // Slices with an entry above 255: rare (after a wrap).
+            if (SPLIT && clear_flag != 0 && sub == 0) {
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh)
+                    if (clear_flag & (1u << hh))
+                        p.flags[(size_t)(pix0[hh] + t * pstep) * 8 + p.dir] = 0;
+            }
+            // High bytes of this step, tested (and stored) one step later.
             uint32_t o = 0;
 #pragma unroll
             for (int i = 0; i < K; ++i) o |= agg[i];
-            const uint32_t exc = lane_real ? (((o & 0x0000FF00u) && act0 ? 1u : 0u) |
-                                              ((o & 0xFF000000u) && act1 ? 2u : 0u))
-                                           : 0u;
-            if (lanes_ne(exc, 0u) != 0) {
-                const uint32_t f = group_or<L>(exc);
-                uint32_t ha[(K + 3) / 4], hb[(K + 3) / 4];
-                bytes_of_pairs<K>(agg, 0x07030501u, ha, hb);
-                const ptrdiff_t hoff = p.hi - p.vol;
-                if ((f & 1u) && lane_store) store_seg<K>(vp[0] + hoff, ha);
-                if ((f & 2u) && lane_store) store_seg<K>(vp[1] + hoff, hb);
-                if (sub == 0) {
-#pragma unroll
-                    for (int hh = 0; hh < 2; ++hh)
-                        if (f & (1u << hh))
-                            p.flags[(size_t)(pix0[hh] + t * pstep) * 8 + p.dir] = 1;
-                }
-            }
+            high = o & ((st0 ? 0x0000FF00u : 0u) | (st1 ? 0xFF000000u : 0u));
+            high_lanes = lanes_ne(high, 0u);
         }
         vp[0] += vstep;
         vp[1] += vstep;
+        ++tcur;
     };
 
     // Loads run PF steps ahead through a ring of PF + 1 buffers; the loop is
@@ -301,23 +481,43 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first) 
     // every lane: their loads read in-range words, their stores go to the
     // sink).  With no branch around any load or store, every step issues the
     // same memory operations and waits only for its own PF-old loads.
+    // The head role's checks of the tail's values load them in the same
+    // ring, PF steps ahead (after the wait on the tail).  A head ends as soon
+    // as every scanline of the wave has met its tail or ended (checked once
+    // per round).
     constexpr int PF = K <= 2 ? 6 : (K <= 4 ? 4 : (K <= 8 ? 2 : 1));
     constexpr int NB = PF + 1;
     const int rounds = __builtin_amdgcn_readfirstlane((maxlen + NB - 1) / NB);
     PairBuf<K> buf[NB];
+    TailBuf<K> tbuf[SPLIT ? NB : 1];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
         issue(buf[j], j);
+        if constexpr (SPLIT) issue_tail(tbuf[j], j);
         __builtin_amdgcn_sched_barrier(0);
     }
-    for (int r = 0; r < rounds; ++r) {
+    int r = 0;
+    for (; r < rounds; ++r) {
         const int t = r * NB;
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             issue(buf[(u + PF) % NB], t + u + PF);
-            step(buf[u], t + u);
+            if constexpr (SPLIT) issue_tail(tbuf[(u + PF) % NB], t + u + PF);
+            step(buf[u], tbuf[SPLIT ? u : 0], t + u);
+        }
+        if (SPLIT && role == kRoleHead && t + NB > vfirst) {
+            const int tn = t + NB;
+            const bool live = (tn < len[0] && !met[0]) || (tn < len[1] && !met[1]);
+            if (__builtin_amdgcn_uicmp((uint32_t)live, 0u, 33) == 0) {
+                ++r;
+                break;
+            }
         }
     }
+    if (mode == kOutU8) flush_high(r * NB - 1);
+    // Loads still in flight (the ring runs PF steps ahead) complete before
+    // the wave ends.
+    __builtin_amdgcn_s_waitcnt(0);
 }
 
 // Winner-take-all with parabola refinement of one pixel from its summed
@@ -371,24 +571,57 @@ __device__ __forceinline__ void wta_slice(const u16x2 (&v)[2][4], int nc, int j,
     }
 }
 
-// Sums the nv per-direction volumes of pixel `pix` (vol0 + k * vstride
-// bytes, entries pix*Dp + d) mod 2^16 and takes its winner; 16 lanes per
-// pixel (j = lane & 15), nc = 1 for Dp <= 128, else 2.
-//   kOutU16: u16 entries, one 16-byte load per lane, volume and chunk.
-//   kOutU8:  u8 entries, 8 bytes per lane, volume and chunk; the byte sums
-//            of at most 8 volumes stay below 2^16, so even and odd bytes are
-//            summed as two u16 pairs without carries; slices flagged in the
-//            pixel's 8 flag bytes (byte k by direction k) add 256 * their
-//            high bytes (hi0 + k * vstride, same entry offsets).
+// Raw loads of one pixel's direction slices for the summing winner-take-all:
+// q[c][k] holds volume k's entries 128c + 8j .. +7 of the pixel (E = uint4,
+// 16 bytes, for u16 entries; E = uint2, 8 bytes, for u8), fl the pixel's
+// exception flags (u8 only).  Split from the sums so that a caller can keep
+// the loads of the next pixels in flight while it sums the current ones.
+template <int MODE>
+struct WtaIn {
+    using E = typename std::conditional<MODE == kOutU16, uint4, uint2>::type;
+    uint2 fl;
+    E q[2][8];
+};
+
+// Loads of pixel `pix` (16 lanes per pixel, j = lane & 15; nc = 1 for
+// Dp <= 128, else 2; volume k at vol0 + k * vstride bytes, entries
+// pix*Dp + d).
+template <int MODE>
+__device__ __forceinline__ void wta_load(WtaIn<MODE> &in, const uint8_t *__restrict__ vol0,
+                                         size_t vstride, const uint2 *__restrict__ flags,
+                                         int nv, int nc, int pix, int npx, int Dp) {
+    using E = typename WtaIn<MODE>::E;
+    const int j = (int)(threadIdx.x & 15);
+    const bool valid = pix < npx;
+    const size_t base = (size_t)(valid ? pix : 0) * (size_t)Dp;
+    in.fl = MODE == kOutU8 ? flags[valid ? pix : 0] : make_uint2(0, 0);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        if (c >= nc) break;
+        const int d0 = 128 * c + 8 * j;
+        if (d0 >= Dp) continue;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k >= nv) break;
+            in.q[c][k] = *(const E *)(vol0 + k * vstride + (base + d0) * sizeof(E) / 8);
+        }
+    }
+}
+
+// Sums the loaded slices mod 2^16 and takes the pixel's winner.
+//   kOutU16: u16 entries added as u16 pairs.
+//   kOutU8:  the byte sums of at most 8 volumes stay below 2^16, so even and
+//            odd bytes are summed as two u16 pairs without carries; slices
+//            flagged in the pixel's 8 flag bytes (byte k by direction k) add
+//            256 * their high bytes (hi0 + k * vstride, same entry offsets).
 // With sum_out the summed u16 slice is also stored (pixel-major, stride Dp).
 // All lanes of the wave must call it (the winner uses cross-lane moves).
-__device__ __forceinline__ void wta_sum_pixel(const uint8_t *__restrict__ vol0, size_t vstride,
-                                              const uint8_t *__restrict__ hi0,
-                                              const uint2 *__restrict__ flags, int nv, int nc,
-                                              int mode, int pix, int npx, int D, int Dp,
-                                              float *__restrict__ out,
-                                              int16_t *__restrict__ best,
-                                              uint16_t *__restrict__ sum_out) {
+template <int MODE>
+__device__ __forceinline__ void wta_finish(const WtaIn<MODE> &in,
+                                           const uint8_t *__restrict__ hi0, size_t vstride,
+                                           int nv, int nc, int pix, int npx, int D, int Dp,
+                                           float *__restrict__ out, int16_t *__restrict__ best,
+                                           uint16_t *__restrict__ sum_out) {
     const int j = (int)(threadIdx.x & 15);
     const bool valid = pix < npx;
     const size_t base = (size_t)(valid ? pix : 0) * (size_t)Dp;
@@ -397,25 +630,23 @@ __device__ __forceinline__ void wta_sum_pixel(const uint8_t *__restrict__ vol0, 
     for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[c][q] = (u16x2){0, 0};
-    if (mode == kOutU16) {
+    if constexpr (MODE == kOutU16) {
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             if (c >= nc) break;
-            const int d0 = 128 * c + 8 * j;
-            if (d0 < Dp) {
+            if (128 * c + 8 * j >= Dp) continue;
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    if (k >= nv) break;
-                    const uint4 x = *(const uint4 *)(vol0 + k * vstride + (base + d0) * 2);
-                    v[c][0] += __builtin_bit_cast(u16x2, x.x);
-                    v[c][1] += __builtin_bit_cast(u16x2, x.y);
-                    v[c][2] += __builtin_bit_cast(u16x2, x.z);
-                    v[c][3] += __builtin_bit_cast(u16x2, x.w);
-                }
+            for (int k = 0; k < 8; ++k) {
+                if (k >= nv) break;
+                const uint4 x = in.q[c][k];
+                v[c][0] += __builtin_bit_cast(u16x2, x.x);
+                v[c][1] += __builtin_bit_cast(u16x2, x.y);
+                v[c][2] += __builtin_bit_cast(u16x2, x.z);
+                v[c][3] += __builtin_bit_cast(u16x2, x.w);
             }
         }
     } else {
-        const uint2 fl = flags[valid ? pix : 0];
+        const uint2 fl = in.fl;
         const bool flagged = (fl.x | fl.y) != 0;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
@@ -426,7 +657,7 @@ __device__ __forceinline__ void wta_sum_pixel(const uint8_t *__restrict__ vol0, 
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     if (k >= nv) break;
-                    const uint2 x = *(const uint2 *)(vol0 + k * vstride + base + d0);
+                    const uint2 x = in.q[c][k];
                     e0 += x.x & 0x00FF00FFu;
                     o0 += perm(0u, x.x, 0x0c030c01u);
                     e1 += x.y & 0x00FF00FFu;
@@ -468,6 +699,26 @@ __device__ __forceinline__ void wta_sum_pixel(const uint8_t *__restrict__ vol0, 
         }
     }
     wta_slice(v, nc, j, D, valid, pix, out, best);
+}
+
+// Sum of the nv direction slices of pixel `pix` and its winner (wta_load
+// then wta_finish), in the volume storage `mode`.
+__device__ __forceinline__ void wta_sum_pixel(const uint8_t *__restrict__ vol0, size_t vstride,
+                                              const uint8_t *__restrict__ hi0,
+                                              const uint2 *__restrict__ flags, int nv, int nc,
+                                              int mode, int pix, int npx, int D, int Dp,
+                                              float *__restrict__ out,
+                                              int16_t *__restrict__ best,
+                                              uint16_t *__restrict__ sum_out) {
+    if (mode == kOutU16) {
+        WtaIn<kOutU16> in;
+        wta_load(in, vol0, vstride, flags, nv, nc, pix, npx, Dp);
+        wta_finish(in, hi0, vstride, nv, nc, pix, npx, D, Dp, out, best, sum_out);
+    } else {
+        WtaIn<kOutU8> in;
+        wta_load(in, vol0, vstride, flags, nv, nc, pix, npx, Dp);
+        wta_finish(in, hi0, vstride, nv, nc, pix, npx, D, Dp, out, best, sum_out);
+    }
 }
 
 }  // namespace dev

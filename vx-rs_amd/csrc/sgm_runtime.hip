@@ -338,7 +338,8 @@ static vxh_status paths_and_wta(const uint2 *lc, const uint2 *rc, uint32_t w, ui
         his[i] = mode == vxh::kOutU8 ? (uint8_t *)(ws + L.hi + (size_t)i * L.vol_bytes)
                                      : nullptr;
     }
-    VXH_HIP(vxh::launch_paths(base, dirs, vols, his, P, sc, mode, st), "aggregate");
+    VXH_HIP(vxh::launch_paths(base, dirs, vols, his, P, sc, mode, (uint32_t *)(ws + L.sync), st),
+            "aggregate");
     if (mark) VXH_HIP(hipEventRecord(mark, st), "hipEventRecord");
     VXH_HIP(vxh::launch_wta_sum((const uint8_t *)(ws + L.vol), L.vol_bytes,
                                 mode == vxh::kOutU8 ? (const uint8_t *)(ws + L.hi) : nullptr,
@@ -373,10 +374,10 @@ static vxh_status enqueue_impl(vxh_sgm *s, const uint8_t *d_left, const uint8_t 
     uint2 *rc = (uint2 *)(base + L.rc);
     uint2 *flags = mode == vxh::kOutU8 ? (uint2 *)(base + L.flags) : nullptr;
     VXH_HIP(vxh::launch_census(d_left, w, h, c->census_radius_x, c->census_radius_y, lc,
-                               flags, st),
+                               flags, (uint32_t *)(base + L.sync), vxh::sync_words(w, h), st),
             "census");
     VXH_HIP(vxh::launch_census(d_right, w, h, c->census_radius_x, c->census_radius_y, rc,
-                               nullptr, st),
+                               nullptr, nullptr, 0, st),
             "census");
     if (marks) VXH_HIP(hipEventRecord(marks[0], st), "hipEventRecord");
     int16_t *best = lr ? (int16_t *)(base + L.best) : nullptr;
@@ -475,9 +476,17 @@ vxh_status vxh_sgm_compute_disparity(vxh_sgm *s, vxh_context *ctx,
     v = enqueue_impl(s, (const uint8_t *)left->dptr, (const uint8_t *)right->dptr, w, h,
                      c, (float *)out->dptr, sl->ws, sl->ws_bytes, sl->stream);
     hipError_t e = hipStreamSynchronize(sl->stream);
+    // The split launch's error word: a head whose wait for its tail timed out.
+    uint32_t err = 0;
+    if (v == VXH_OK && e == hipSuccess) {
+        const WsLayout L = workspace_layout(w, h, c->num_disparities, (int)c->num_paths,
+                                            mode_for(s, c), c->lr_check != 0);
+        e = hipMemcpy(&err, (char *)sl->ws + L.sync + 4, 4, hipMemcpyDeviceToHost);
+    }
     release_slot(s, sl);
     if (v != VXH_OK) return v;
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    if (err) return fail(VXH_ERR_GPU, "GPU error: a split scanline's wait timed out");
     return VXH_OK;
 }
 
@@ -528,7 +537,7 @@ vxh_status vxh_sgm_census(const uint8_t *d_img, uint32_t w, uint32_t h, uint32_t
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: census arguments");
     if ((2ull * rx + 1) * (2ull * ry + 1) - 1 > 64)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: census window over 64 bits");
-    VXH_HIP(vxh::launch_census(d_img, w, h, rx, ry, (uint2 *)d_census, nullptr,
+    VXH_HIP(vxh::launch_census(d_img, w, h, rx, ry, (uint2 *)d_census, nullptr, nullptr, 0,
                                (hipStream_t)stream),
             "census");
     return VXH_OK;
@@ -538,9 +547,12 @@ vxh_status vxh_sgm_set_schedule(vxh_sgm *s, uint32_t nd, uint32_t lanes_h,
                                 uint32_t per_lane_h, uint32_t lanes, uint32_t per_lane) {
     if (!s || nd < 1 || nd > 256)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: schedule target");
+    const vxh::Schedule cur = schedule_for(s, (int)nd);
     if (lanes_h == 0 && per_lane_h == 0 && lanes == 0 && per_lane == 0) {
+        const vxh::Schedule d = vxh::default_schedule((int)nd);
         std::lock_guard<std::mutex> g(s->mu);
-        s->sched_set[nd] = false;
+        s->sched[nd] = vxh::Schedule{d.horiz, d.other, cur.split};
+        s->sched_set[nd] = true;
         return VXH_OK;
     }
     const int ih = layout_index((int)nd, lanes_h, per_lane_h);
@@ -548,12 +560,22 @@ vxh_status vxh_sgm_set_schedule(vxh_sgm *s, uint32_t nd, uint32_t lanes_h,
     if (ih < 0 || io < 0)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
     std::lock_guard<std::mutex> g(s->mu);
-    s->sched[nd] = vxh::Schedule{ih, io};
+    s->sched[nd] = vxh::Schedule{ih, io, cur.split};
     s->sched_set[nd] = true;
     return VXH_OK;
 }
 
-vxh_status vxh_sgm_get_schedule(vxh_sgm *s, uint32_t nd, uint32_t *out4) {
+vxh_status vxh_sgm_set_split(vxh_sgm *s, uint32_t nd, uint32_t split) {
+    if (!s || nd < 1 || nd > 256 || split > 1)
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: split setting");
+    const vxh::Schedule cur = schedule_for(s, (int)nd);
+    std::lock_guard<std::mutex> g(s->mu);
+    s->sched[nd] = vxh::Schedule{cur.horiz, cur.other, (int)split};
+    s->sched_set[nd] = true;
+    return VXH_OK;
+}
+
+vxh_status vxh_sgm_get_schedule(vxh_sgm *s, uint32_t nd, uint32_t *out4) {  // out4[0..5)
     if (!s || nd < 1 || nd > 256 || !out4)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: schedule target");
     const vxh::Schedule sc = schedule_for(s, (int)nd);
@@ -562,6 +584,7 @@ vxh_status vxh_sgm_get_schedule(vxh_sgm *s, uint32_t nd, uint32_t *out4) {
     out4[1] = (uint32_t)vxh::kClassLayouts[c][sc.horiz][1];
     out4[2] = (uint32_t)vxh::kClassLayouts[c][sc.other][0];
     out4[3] = (uint32_t)vxh::kClassLayouts[c][sc.other][1];
+    out4[4] = (uint32_t)sc.split;
     return VXH_OK;
 }
 
@@ -611,6 +634,8 @@ vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,
     if (e != hipSuccess) return hip_fail(e, "hipMallocAsync", bytes);
     e = hipMemcpyAsync(g + L.rc, d_rc, n * 8, hipMemcpyDeviceToDevice, st);
     if (e == hipSuccess && mode == vxh::kOutU8) e = hipMemsetAsync(g + L.flags, 0, n * 8, st);
+    if (e == hipSuccess)
+        e = vxh::launch_zero((uint32_t *)(g + L.sync), vxh::sync_words(w, h), st);
     vxh_status v2 = e == hipSuccess
                         ? paths_and_wta((const uint2 *)d_lc, (const uint2 *)(g + L.rc), w, h,
                                         c, g, L, mode, sc, st, nullptr, nullptr, d_volume)

@@ -21,7 +21,8 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libvxsgm.so")
+# VXSGM_LIB selects another build of the same library (same C ABI).
+LIB_PATH = os.environ.get("VXSGM_LIB") or os.path.join(_HERE, "libvxsgm.so")
 INCLUDE_DIR = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include")
 
 if not os.path.exists(LIB_PATH):
@@ -94,6 +95,7 @@ _sig("vxh_sgm_wta", ctypes.c_int, _vp, _u32, _u32, _cfgp, _vp, _vp)
 _sig("vxh_sgm_volume_stride", _u32, _u32)
 _sig("vxh_sgm_set_schedule", ctypes.c_int, _vp, _u32, _u32, _u32, _u32, _u32)
 _sig("vxh_sgm_get_schedule", ctypes.c_int, _vp, _u32, ctypes.POINTER(_u32))
+_sig("vxh_sgm_set_split", ctypes.c_int, _vp, _u32, _u32)
 _sig("vxh_sgm_set_volume_format", ctypes.c_int, _vp, _u32, _u32)
 _sig("vxh_util_copy", ctypes.c_int, _vp, _vp, _sz, _vp)
 _sig("vxh_sgm_aggregate_layout", ctypes.c_int, _vp, _vp, _u32, _u32, _cfgp, _u32, _u32,
@@ -314,9 +316,18 @@ class SGMStereo:
 
     def schedule(self, num_disparities):
         """((lanes_h, per_lane_h), (lanes, per_lane)) in effect for this D."""
-        out = (_u32 * 4)()
+        out = (_u32 * 5)()
         check(lib.vxh_sgm_get_schedule(self._h, num_disparities, out))
         return (out[0], out[1]), (out[2], out[3])
+
+    def split(self, num_disparities):
+        """Whether long scanlines run as two halves at once for this D."""
+        out = (_u32 * 5)()
+        check(lib.vxh_sgm_get_schedule(self._h, num_disparities, out))
+        return bool(out[4])
+
+    def set_split(self, num_disparities, split):
+        check(lib.vxh_sgm_set_split(self._h, num_disparities, int(bool(split))))
 
     def set_volume_format(self, num_disparities, fmt):
         """Per-direction volume storage for this D: "auto", "u8" or "u16"."""

@@ -62,7 +62,7 @@ def _name(sched):
 
 
 def _parse(name):
-    hs, os_ = name.split("/")
+    hs, os_ = name.split("/")[:2]
     return tuple(int(v) for v in hs.split("x")), tuple(int(v) for v in os_.split("x"))
 
 
@@ -116,8 +116,10 @@ class Tuner:
         per-pair pipeline and leaves the winner set on `sgm`; returns it."""
         key = _key("plain", w, h, D, paths)
         if not force and key in self.table:
-            sched = _parse(self.table[key]["schedule"])
+            name = self.table[key]["schedule"]
+            sched = _parse(name)
             sgm.set_schedule(D, sched)
+            sgm.set_split(D, name.endswith("/split"))
             return sched
         left, right, cfg = vxsgm.mixture_pair(w, h, D, 0x5452)
         cfg.num_paths = paths
@@ -127,13 +129,16 @@ class Tuner:
         ws = hip.DeviceBuffer(wsb)
         stream = hip.Stream()
         times = {}
-        for sched in schedules(D):
-            sgm.set_schedule(D, sched)
-            times[_name(sched)] = self._time(
-                lambda: sgm.enqueue(dl.ptr, dr.ptr, w, h, cfg, out.ptr, ws.ptr, wsb,
-                                    stream.handle), stream, reps)
+        for split in (True, False):
+            sgm.set_split(D, split)
+            for sched in schedules(D):
+                sgm.set_schedule(D, sched)
+                times[_name(sched) + ("/split" if split else "")] = self._time(
+                    lambda: sgm.enqueue(dl.ptr, dr.ptr, w, h, cfg, out.ptr, ws.ptr, wsb,
+                                        stream.handle), stream, reps)
         best = min(times, key=times.get)
         sched = _parse(best)
+        sgm.set_split(D, best.endswith("/split"))
         sgm.set_schedule(D, sched)
         self.table[key] = {"schedule": best, "ms": times}
         return sched

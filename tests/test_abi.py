@@ -123,16 +123,17 @@ def align256(v):
                                             (320, 240, 64, 4, 400.0), (7, 5, 3, 8, 120.0)])
 def test_server_request_bytes_is_workspace(w, h, D, paths, p2):
     """A served request is admitted against its whole pipeline workspace: census
-    of both images (the left one at least 2 KiB, the guard the aggregation may
-    read in front of the right one), the exception flags (8 bytes per pixel),
-    the split launch's synchronisation words (ticket, error flag, one flag per
-    tail wave), and per direction a u8 volume plus its high-byte volume, or a
-    u16 volume; the larger of the two formats is reserved."""
+    of both images, the u8 matching-cost volume (Dp bytes per pixel), the
+    exception flags (8 bytes per pixel), the split launch's synchronisation
+    words (ticket, error flag, one flag per tail wave), and per direction a u8
+    volume plus its high-byte volume, or a u16 volume; the larger of the two
+    formats is reserved."""
     cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=paths, p2=p2)
     n, Dp = w * h, (D + 15) // 16 * 16
     sync = align256(4 * (2 + 8 * ((w + h) // 2 + 1)))
-    u8 = align256(max(n * 8, 2048)) + 2 * align256(n * 8) + sync + 2 * paths * align256(n * Dp)
-    u16 = align256(max(n * 8, 2048)) + align256(n * 8) + sync + paths * align256(n * Dp * 2)
+    common = 2 * align256(n * 8) + align256(n * Dp) + sync
+    u8 = common + align256(n * 8) + 2 * paths * align256(n * Dp)
+    u16 = common + paths * align256(n * Dp * 2)
     assert vxsgm.workspace_bytes(w, h, cfg) == max(u8, u16)
     assert vxsgm.request_bytes(w, h, cfg) == max(u8, u16)
 

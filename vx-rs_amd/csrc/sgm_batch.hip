@@ -6,10 +6,11 @@
 // one command buffer (vx-vision/src/pipeline.rs:12-80).  Here a batch is a
 // work list of wave-sized items, each a slice of one pair's stage:
 //   phase 0   census rows of the left or right image
-//   phase 1   scanline groups of any path direction (the directions of a
+//   phase 1   matching-cost rows (64-pixel tiles, as the per-pair kernel)
+//   phase 2   scanline groups of any path direction (the directions of a
 //             pair are independent: each writes its own volume)
-//   phase 2   sum of the direction volumes and winner-take-all, pixel ranges
-//   phase 3   left-right check pixels (only with lr_check)
+//   phase 3   sum of the direction volumes and winner-take-all, pixel ranges
+//   phase 4   left-right check pixels (only with lr_check)
 // A wave takes the next item from a global counter, waits until every item of
 // the same pair's previous phase has signalled completion, runs the slice with
 // the same device code as the per-pair kernels, and signals.  Every item
@@ -34,15 +35,16 @@ namespace {
 
 using namespace dev;
 
-constexpr int kMaxPhase = 4;      // census, paths, wta, lr
+constexpr int kMaxPhase = 5;      // census, cost, paths, wta, lr
 constexpr int kLaunchKinds = 4;   // (u8, u16) x (D % 8 == 0, else)
 
-enum ItemKind : int { kCensusL = 0, kCensusR = 1, kAgg = 2, kWta = 3, kLr = 4 };
+enum ItemKind : int { kCensusL = 0, kCensusR = 1, kCost = 2, kAgg = 3, kWta = 4, kLr = 5 };
 
 struct PairDesc {
     const uint8_t *left, *right;
     float *out;
     uint2 *lc, *rc, *flags;  // flags: kOutU8 only
+    uint8_t *cost;           // matching-cost volume
     uint8_t *vol, *hi;       // direction k's volume at vol + k * vol_bytes (hi likewise)
     size_t vol_bytes;
     uint16_t *sum;           // lr_check only: summed volume
@@ -131,6 +133,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 batch_kernel(const PairDesc *__restrict__ pairs, const Item *__restrict__ items,
              int nitems, uint32_t *next, uint32_t *done,
              const uint32_t *__restrict__ need, uint32_t *error) {
+    __shared__ uint2 span[4][kCostSpan];
     const int lane = threadIdx.x & 63;
     for (uint32_t idx = wave_take(next, lane); idx < (uint32_t)nitems;
          idx = wave_take(next, lane)) {
@@ -151,10 +154,14 @@ batch_kernel(const PairDesc *__restrict__ pairs, const Item *__restrict__ items,
                     out[i] = census_pixel(img, pd.w, pd.h, pd.rx, pd.ry, x, y);
                     if (fl) fl[i] = make_uint2(0, 0);
                 }
+        } else if (it.kind == kCost) {
+            for (int y = it.first; y < it.first + it.count; ++y)
+                for (int x0 = 0; x0 < pd.w; x0 += kCostTile)
+                    cost_tile(pd.lc, pd.rc, pd.w, pd.D, pd.Dp, y, x0, pd.cost,
+                              span[threadIdx.x >> 6]);
         } else if (it.kind == kAgg) {
             PassParams p;
-            p.lc = pd.lc;
-            p.rc = pd.rc;
+            p.cost = pd.cost;
             p.vol = pd.vol + (size_t)it.dir * pd.vol_bytes;
             p.hi = pd.hi ? pd.hi + (size_t)it.dir * pd.vol_bytes : nullptr;
             p.flags = (uint8_t *)pd.flags;
@@ -328,6 +335,7 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
         const bool u8 = pd[i].mode == kOutU8, lr = pairs[i].cfg.lr_check != 0;
         pd[i].lc = (uint2 *)(base + L.lc);
         pd[i].rc = (uint2 *)(base + L.rc);
+        pd[i].cost = (uint8_t *)(base + L.cost);
         pd[i].flags = u8 ? (uint2 *)(base + L.flags) : nullptr;
         pd[i].vol = (uint8_t *)(base + L.vol);
         pd[i].hi = u8 ? (uint8_t *)(base + L.hi) : nullptr;
@@ -336,10 +344,11 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
     }
 
     // Work list.  Census of every pair first; then, pair by pair (largest
-    // first), the aggregation items (horizontal directions, the longest
-    // scanlines, first), each pair's sum + winner items placed two pairs
-    // later so that they find its directions finished and run beside the
-    // next pairs' aggregation; the LR checks last.
+    // first), the next pair's cost rows and this pair's aggregation items
+    // (horizontal directions, the longest scanlines, first), each pair's
+    // sum + winner items placed wta_lag pairs later so that they find its
+    // directions finished and run beside the next pairs' aggregation; the LR
+    // checks last.
     std::vector<int> order(n);
     for (uint32_t i = 0; i < n; ++i) order[i] = (int)i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
@@ -356,7 +365,7 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
     auto push_wta = [&](int p) {
         const int npx = pd[p].w * pd[p].h;
         for (int i = 0; i < npx; i += px_item)
-            push(p, kWta, 0, i, std::min(px_item, npx - i), 2);
+            push(p, kWta, 0, i, std::min(px_item, npx - i), 3);
     };
     for (int p : order) {
         const PairDesc &d = pd[p];
@@ -366,17 +375,24 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
             push(p, kCensusR, 0, y, std::min(rows, d.h - y), 0);
         }
     }
+    auto push_cost = [&](int p) {
+        const int rows = std::max(1, 2048 / pd[p].w);
+        for (int y = 0; y < pd[p].h; y += rows)
+            push(p, kCost, 0, y, std::min(rows, pd[p].h - y), 1);
+    };
+    push_cost(order[0]);
     const size_t kWtaLag = o.wta_lag;
     for (size_t k = 0; k < order.size(); ++k) {
         const int p = order[k];
         const PairDesc &d = pd[p];
+        if (k + 1 < order.size()) push_cost(order[k + 1]);
         for (int pass = 0; pass < 2; ++pass)
             for (int dir = 0; dir < d.P; ++dir) {
                 const bool horiz = kDirY[dir] == 0;
                 if (horiz != (pass == 0)) continue;
                 const int per = 128 / kClassLayouts[d.cls][kBatchLayout[d.cls]][0];
                 const int ns = num_scanlines(dir, d.w, d.h);
-                for (int s0 = 0; s0 < ns; s0 += per) push(p, kAgg, dir, s0, per, 1);
+                for (int s0 = 0; s0 < ns; s0 += per) push(p, kAgg, dir, s0, per, 2);
             }
         if (k >= kWtaLag) push_wta(order[k - kWtaLag]);
     }
@@ -385,7 +401,7 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
     for (int p : order) {
         if (!pairs[p].cfg.lr_check) continue;
         const int npx = pd[p].w * pd[p].h, chunk = 64 * 64;
-        for (int i = 0; i < npx; i += chunk) push(p, kLr, 0, i, std::min(chunk, npx - i), 3);
+        for (int i = 0; i < npx; i += chunk) push(p, kLr, 0, i, std::min(chunk, npx - i), 4);
     }
     for (uint32_t i = 0; i < n; ++i) {
         b->stats[i].items = 0;

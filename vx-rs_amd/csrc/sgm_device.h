@@ -42,11 +42,8 @@ __device__ __forceinline__ uint32_t dpp_mov(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
 }
 
-// Wave mask of lanes with a < b (signed) / a != b: one v_cmp whose result the
-// branch reads directly.
-__device__ __forceinline__ uint64_t lanes_lt(int a, int b) {
-    return __builtin_amdgcn_sicmp(a, b, 40);
-}
+// Wave mask of lanes with a != b: one v_cmp whose result the branch reads
+// directly.
 __device__ __forceinline__ uint64_t lanes_ne(uint32_t a, uint32_t b) {
     return __builtin_amdgcn_uicmp(a, b, 33);
 }
@@ -150,14 +147,64 @@ __device__ __forceinline__ T *store_sink() {
     return (T *)&g_store_sink[(threadIdx.x & 63) * 2];
 }
 
-// Census word at byte offset `off` (a pixel index inside the image times 8;
-// images have fewer than 2^29 pixels, so the offset fits 32 bits and the load
-// takes the scalar-base + 32-bit-offset form).  Plain global loads, not buffer
-// loads: the compiler counts these exactly when placing its waits, so a step
-// waits only for its own prefetched loads.
-template <int IMM = 0>
-__device__ __forceinline__ uint2 load_census(const uint2 *__restrict__ c, uint32_t off) {
-    return *(const uint2 *)((const char *)c + off + IMM);
+// Matching-cost volume tiles.  A wave computes the costs of kCostTile
+// consecutive pixels of one row (one pixel per lane) for all Dp entries of
+// their slices, staging the right census row segment it reads in LDS.
+constexpr int kCostTile = 64;
+constexpr int kCostSpan = kCostTile + 255;   // right census entries of a tile (D <= 256)
+
+// Costs of pixels x0 .. x0 + 63 of row y into cost[(y*w + x)*Dp + d]: for
+// d < D the Hamming distance of the left census at x and the right census at
+// x - d, or 64 where x - d < 0 (SGMStereo.metal:125-137); 64 for the padding
+// entries d >= D.  `span` is this wave's LDS area of kCostSpan entries.
+// Wave-level: all 64 lanes call it with the same arguments.
+__device__ __forceinline__ void cost_tile(const uint2 *__restrict__ lc,
+                                          const uint2 *__restrict__ rc, int w, int D, int Dp,
+                                          int y, int x0, uint8_t *__restrict__ cost,
+                                          uint2 *span) {
+    const int lane = threadIdx.x & 63;
+    const int xs0 = max(0, x0 - (D - 1));
+    const int xs1 = min(w, x0 + kCostTile);
+    const uint2 *rrow = rc + (size_t)y * w + xs0;
+    for (int j = lane; j < xs1 - xs0; j += 64) span[j] = rrow[j];
+    // The span's entries come from other lanes of this wave.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int x = x0 + lane;
+    if (x < w) {
+        const uint2 lv = lc[(size_t)y * w + x];
+        uint4 *dst = (uint4 *)(cost + ((size_t)y * w + x) * Dp);
+        const int rel = x - xs0;   // span index of x - d is rel - d
+        // Every entry of the chunk is real and every x - d >= 0.
+        auto chunk = [&](int dc, auto checked) {
+            uint32_t wd[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int d = dc + 4 * q + b;
+                    uint32_t c;
+                    if constexpr (decltype(checked)::value) {
+                        const uint2 r = span[max(rel - d, 0)];
+                        c = (d < D && d <= x) ? __popc(lv.x ^ r.x) + __popc(lv.y ^ r.y) : 64u;
+                    } else {
+                        const uint2 r = span[rel - d];
+                        c = __popc(lv.x ^ r.x) + __popc(lv.y ^ r.y);
+                    }
+                    v |= c << (8 * b);
+                }
+                wd[q] = v;
+            }
+            dst[dc / 16] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+        };
+        const bool inside = x0 >= D - 1;   // wave-uniform: no lane has x - d < 0
+        for (int dc = 0; dc < Dp; dc += 16) {
+            if (inside && dc + 16 <= D) chunk(dc, std::false_type{});
+            else chunk(dc, std::true_type{});
+        }
+    }
 }
 
 // Left-right check of pixel i: D_R(x') is the lowest d minimising

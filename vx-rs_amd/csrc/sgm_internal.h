@@ -27,27 +27,13 @@ inline int num_scanlines(int dir, int w, int h) {
 // packed segment of it never straddles a pixel.
 inline int volume_stride(int D) { return (D + 15) & ~15; }
 
-// The aggregation reads the right census at x - d for every d of a lane
-// through one base kCensusGuard bytes before rc; the reads with x - d < 0
-// (whose cost is replaced by 64) then land in the kCensusGuard readable bytes
-// that every rc buffer must have in front of it (the left census, or padding).
-constexpr size_t kCensusGuard = 2048;
-
-// Bytes reserved for the left census in front of the right one, so that rc
-// has kCensusGuard readable bytes before it.
-inline size_t census_span(size_t npx) {
-    const size_t b = npx * 8 > kCensusGuard ? npx * 8 : kCensusGuard;
-    return (b + 255) & ~(size_t)255;
-}
-
 // Storage of the per-direction path volumes (entries pix*Dp + d).
 constexpr int kOutU8 = 0;    // low byte, plus high bytes of flagged slices
 constexpr int kOutU16 = 1;   // full u16 path cost
 
 // One direction pass of the packed-pair aggregation.
 struct PassParams {
-    const uint2 *lc;       // left census, w*h (lo, hi)
-    const uint2 *rc;       // right census, kCensusGuard readable bytes before it
+    const uint8_t *cost;   // matching costs, entry pix*Dp + d (u8, 64 where x - d < 0)
     uint8_t *vol;          // this direction's volume
     uint8_t *hi;           // kOutU8: high bytes of flagged slices (1 byte per entry)
     uint8_t *flags;        // kOutU8: 8 bytes per pixel, byte dir = 1 when flagged
@@ -59,8 +45,7 @@ struct PassParams {
 
 // Lane layouts of the aggregation: layout class c covers D <= 64 << c with
 // the (L, K) choices kClassLayouts[c][0 .. kClassCount[c]), L lanes owning a
-// scanline pair and K disparities each (L * K = 64 << c).  (16, 16) is left
-// out: its census ring alone needs more than 128 registers.
+// scanline pair and K disparities each (L * K = 64 << c).
 constexpr int kLayoutClasses = 3;
 constexpr int kLayoutsPerClass = 3;
 constexpr int kClassCount[kLayoutClasses] = {3, 3, 2};
@@ -88,12 +73,13 @@ constexpr size_t kWsAlign = 256;
 inline size_t ws_align(size_t v) { return (v + kWsAlign - 1) & ~(kWsAlign - 1); }
 
 // Scratch of one pair, byte offsets from its base: census of both images, the
-// exception flags (u8 volumes only, 8 bytes per pixel), one volume per path
-// direction (u8 or u16 entries, vol_bytes apart), the high-byte volumes (u8
-// only, also vol_bytes apart), and for the LR check the summed u16 volume and
-// the winner index.
+// matching-cost volume (u8, pix*Dp + d), the exception flags (u8 volumes
+// only, 8 bytes per pixel), the split launch's synchronisation words, one
+// volume per path direction (u8 or u16 entries, vol_bytes apart), the
+// high-byte volumes (u8 only, also vol_bytes apart), and for the LR check the
+// summed u16 volume and the winner index.
 struct WsLayout {
-    size_t lc, rc, flags, sync, vol, vol_bytes, hi, sum, best, total;
+    size_t lc, rc, cost, flags, sync, vol, vol_bytes, hi, sum, best, total;
 };
 inline WsLayout workspace_layout(uint32_t w, uint32_t h, uint32_t D, int paths, int mode,
                                  bool lr) {
@@ -102,8 +88,9 @@ inline WsLayout workspace_layout(uint32_t w, uint32_t h, uint32_t D, int paths, 
     const size_t Dp = (size_t)volume_stride((int)D);
     const bool u8 = mode == kOutU8;
     l.lc = 0;
-    l.rc = l.lc + census_span(n);
-    l.flags = l.rc + ws_align(n * 8);
+    l.rc = l.lc + ws_align(n * 8);
+    l.cost = l.rc + ws_align(n * 8);
+    l.flags = l.cost + ws_align(n * Dp);
     l.sync = l.flags + (u8 ? ws_align(n * 8) : 0);
     l.vol = l.sync + ws_align(sync_words(w, h) * 4);
     l.vol_bytes = ws_align(n * Dp * (u8 ? 1 : 2));
@@ -125,7 +112,7 @@ inline int auto_volume_mode(float p2) {
 // volume).  Blocks [first_block, first_block + nblocks) belong to a pass;
 // nblocks is a multiple of 8 so that the hardware's round-robin placement of
 // workgroups over the 8 XCDs can be undone (each XCD gets a contiguous range
-// of the pass's scanlines and shares their census rows in its L2).
+// of the pass's scanlines and shares their cost rows in its L2).
 constexpr int kMaxPasses = 8;
 struct PassDesc {
     uint8_t *vol, *hi;
@@ -152,6 +139,10 @@ struct MultiPass {
 hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
                          uint2 *out, uint2 *zero_flags, uint32_t *zero_words, size_t nwords,
                          hipStream_t s);
+// Matching-cost volume of a pair from its census (layout of
+// PassParams::cost, Dp = volume_stride(D)).
+hipError_t launch_cost(const uint2 *lc, const uint2 *rc, int w, int h, int D, uint8_t *cost,
+                       hipStream_t s);
 // Zeroes n words with a kernel (safe inside captured graphs).
 hipError_t launch_zero(uint32_t *p, size_t n, hipStream_t s);
 // Default schedule for D disparities.

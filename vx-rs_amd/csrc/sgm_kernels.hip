@@ -22,6 +22,19 @@ census_kernel(const uint8_t *__restrict__ img, int w, int h, int rx, int ry,
     for (size_t k = i; k < nwords; k += (size_t)w * h) zero_words[k] = 0;
 }
 
+// One cost tile (64 pixels of a row) per wave.
+__global__ void __launch_bounds__(256)
+cost_kernel(const uint2 *__restrict__ lc, const uint2 *__restrict__ rc, int w, int h, int D,
+            int Dp, int tiles_x, uint8_t *__restrict__ cost) {
+    __shared__ uint2 span[4][kCostSpan];
+    const int wv = threadIdx.x >> 6;
+    const int tile = blockIdx.x * 4 + wv;
+    if (tile >= tiles_x * h) return;   // wave-uniform
+    const int y = tile / tiles_x;
+    const int x0 = (tile - y * tiles_x) * kCostTile;
+    cost_tile(lc, rc, w, D, Dp, y, x0, cost, span[wv]);
+}
+
 // Field `f` of pass i (i wave-uniform): unrolled selects, so the kernel
 // argument block is read with scalar loads instead of being copied to private
 // memory for a dynamic index.
@@ -194,6 +207,15 @@ hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
     dim3 grid((w + 63) / 64, (h + 3) / 4);
     hipLaunchKernelGGL(census_kernel, grid, block, 0, s, img, w, h, rx, ry, out, zero_flags,
                        zero_words, zero_words ? nwords : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_cost(const uint2 *lc, const uint2 *rc, int w, int h, int D, uint8_t *cost,
+                       hipStream_t s) {
+    const int tiles_x = (w + kCostTile - 1) / kCostTile;
+    const long tiles = (long)tiles_x * h;
+    hipLaunchKernelGGL(cost_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, lc, rc,
+                       w, h, D, volume_stride(D), tiles_x, cost);
     return hipGetLastError();
 }
 

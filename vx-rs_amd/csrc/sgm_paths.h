@@ -104,13 +104,11 @@ __device__ __forceinline__ void bytes_of_pairs(const uint32_t *v, uint32_t sel, 
     }
 }
 
-// Census loads of one step for both scanlines of a lane's group: left census
-// of the pixel and right census at x - d for the lane's K disparities.
+// Cost loads of one step for both scanlines of a lane's group: the lane's K
+// entries of each pixel's cost slice (K bytes).
 template <int K>
 struct PairBuf {
-    uint2 lv[2];
-    uint2 rv[2][K];
-    bool edge;   // some active scanline of the wave has x - d < 0 for a lane's d
+    uint32_t c[2][(K + 3) / 4];
 };
 
 // Split scanlines (head role): the tail's stored values of one step, both
@@ -171,8 +169,8 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     const bool lane_real = d0 < D;
     const bool lane_store = d0 < Dp;
     const uint32_t p1x2 = p.p1x2, p2x2 = p.p2x2;
-    const uint2 *__restrict__ lc = p.lc;
-    const uint2 *__restrict__ rcg = (const uint2 *)((const char *)p.rc - kCensusGuard);
+    // Lanes past the volume stride read their pixel's first entries (unused).
+    const uint8_t *__restrict__ cost = p.cost + (lane_store ? d0 : 0);
 
     int sx[2] = {0, 0}, len[2] = {0, 0}, pix0[2] = {0, 0};
     // Head role: steps [vfrom, len) of a scanline check the tail's values.
@@ -247,25 +245,25 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     }
 
     auto issue = [&](PairBuf<K> &b, int t) {
-        uint64_t e = 0;
 #pragma unroll
         for (int hh = 0; hh < 2; ++hh) {
-            const bool a = t < len[hh];
-            const uint32_t pix = a ? (uint32_t)(pix0[hh] + t * pstep) : 0u;
-            b.lv[hh] = load_census(lc, pix << 3);
-            // One base, constant offsets: reads with x - d < 0 land in the
-            // previous row or the guard bytes before rc and are replaced by
-            // 64 in step().  Inactive scanlines read around pixel 0.
-            const uint32_t rb = ((pix - (uint32_t)(d0 + K - 1)) << 3) + (uint32_t)kCensusGuard;
-            static_for<K>([&](auto I) {
-                constexpr int i = decltype(I)::value;
-                b.rv[hh][i] = load_census<8 * (K - 1 - i)>(rcg, rb);
-            });
-            const int x = sx[hh] + t * dx;
-            e |= lanes_lt(a ? x : 0x7FFFFFFF, d0 + K - 1);
+            // Inactive scanlines read pixel 0.
+            const uint32_t pix = t < len[hh] ? (uint32_t)(pix0[hh] + t * pstep) : 0u;
+            const uint8_t *src = cost + (size_t)pix * (uint32_t)Dp;
+            if constexpr (K == 2) b.c[hh][0] = *(const uint16_t *)src;
+            else if constexpr (K == 4) b.c[hh][0] = *(const uint32_t *)src;
+            else if constexpr (K == 8) {
+                const uint2 q = *(const uint2 *)src;
+                b.c[hh][0] = q.x;
+                b.c[hh][1] = q.y;
+            } else {
+                const uint4 q = *(const uint4 *)src;
+                b.c[hh][0] = q.x;
+                b.c[hh][1] = q.y;
+                b.c[hh][2] = q.z;
+                b.c[hh][3] = q.w;
+            }
         }
-        // A mask, not a branch: the step that uses it runs PF steps later.
-        b.edge = e != 0;
     };
     // Head role: the tail's stored values of step t (loaded only after the
     // tail has finished; other lanes and steps read the sink).
@@ -332,23 +330,13 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
 
     auto step = [&](PairBuf<K> &b, TailBuf<K> &tb, int t) {
         const bool act0 = t < len[0], act1 = t < len[1];
+        // Costs of both scanlines, packed: byte i of each slice segment into
+        // the low (first scanline) and high (second) half.
         uint32_t C[K];
 #pragma unroll
-        for (int i = 0; i < K; ++i) {
-            const uint32_t c0 = __popc(b.lv[0].x ^ b.rv[0][i].x) + __popc(b.lv[0].y ^ b.rv[0][i].y);
-            const uint32_t c1 = __popc(b.lv[1].x ^ b.rv[1][i].x) + __popc(b.lv[1].y ^ b.rv[1][i].y);
-            C[i] = c0 | (c1 << 16);
-        }
-        if (b.edge) {
-            // Cost 64 where x - d < 0 (SGMStereo.metal:133-136); every cost is
-            // at most 64, so a packed max with 64 sets exactly those halves.
-            const int x0 = sx[0] + t * dx, x1 = sx[1] + t * dx;
-#pragma unroll
-            for (int i = 0; i < K; ++i) {
-                const uint32_t e = (x0 < d0 + i ? 64u : 0u) | (x1 < d0 + i ? (64u << 16) : 0u);
-                C[i] = pk_max(C[i], e);
-            }
-        }
+        for (int i = 0; i < K; ++i)
+            C[i] = perm(b.c[1][i / 4], b.c[0][i / 4],
+                        0x0C000C00u | ((4u + (i & 3)) << 16) | (uint32_t)(i & 3));
 
         // With prev = 0 and pm = 0 before the first step, the recurrence
         // below gives agg = C there (best = min(0, ...) = 0), the first
@@ -485,7 +473,7 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     // ring, PF steps ahead (after the wait on the tail).  A head ends as soon
     // as every scanline of the wave has met its tail or ended (checked once
     // per round).
-    constexpr int PF = K <= 2 ? 6 : (K <= 4 ? 4 : (K <= 8 ? 2 : 1));
+    constexpr int PF = K <= 4 ? 6 : (K <= 8 ? 4 : 3);
     constexpr int NB = PF + 1;
     const int rounds = __builtin_amdgcn_readfirstlane((maxlen + NB - 1) / NB);
     PairBuf<K> buf[NB];

@@ -312,7 +312,7 @@ size_t vxh_sgm_workspace_bytes(uint32_t w, uint32_t h, const vxh_sgm_config *c) 
 // sums the volumes and takes the winner: into `out` (may be null), `best`
 // (null unless the LR check follows) and, with `sum_out`, the summed u16
 // volume.  With `mark`, records it on `st` between the two launches.
-static vxh_status paths_and_wta(const uint2 *lc, const uint2 *rc, uint32_t w, uint32_t h,
+static vxh_status paths_and_wta(const uint8_t *cost, uint32_t w, uint32_t h,
                                 const vxh_sgm_config *c, char *ws, const WsLayout &L,
                                 int mode, vxh::Schedule sc, hipStream_t st, float *out,
                                 int16_t *best, uint16_t *sum_out,
@@ -320,8 +320,7 @@ static vxh_status paths_and_wta(const uint2 *lc, const uint2 *rc, uint32_t w, ui
     const int D = (int)c->num_disparities;
     const int P = (int)c->num_paths;
     vxh::PassParams base{};
-    base.lc = lc;
-    base.rc = rc;
+    base.cost = cost;
     base.flags = mode == vxh::kOutU8 ? (uint8_t *)(ws + L.flags) : nullptr;
     base.w = (int)w;
     base.h = (int)h;
@@ -379,10 +378,12 @@ static vxh_status enqueue_impl(vxh_sgm *s, const uint8_t *d_left, const uint8_t 
     VXH_HIP(vxh::launch_census(d_right, w, h, c->census_radius_x, c->census_radius_y, rc,
                                nullptr, nullptr, 0, st),
             "census");
+    uint8_t *cost = (uint8_t *)(base + L.cost);
+    VXH_HIP(vxh::launch_cost(lc, rc, (int)w, (int)h, D, cost, st), "cost");
     if (marks) VXH_HIP(hipEventRecord(marks[0], st), "hipEventRecord");
     int16_t *best = lr ? (int16_t *)(base + L.best) : nullptr;
     uint16_t *sum = lr ? (uint16_t *)(base + L.sum) : nullptr;
-    v = paths_and_wta(lc, rc, w, h, c, base, L, mode, schedule_for(s, D), st, d_out, best,
+    v = paths_and_wta(cost, w, h, c, base, L, mode, schedule_for(s, D), st, d_out, best,
                       sum, marks ? marks[1] : nullptr);
     if (v != VXH_OK) return v;
     if (lr)
@@ -620,10 +621,8 @@ vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,
         if (i < 0) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
         sc = vxh::Schedule{i, i};
     }
-    // Stream-ordered scratch (the call stays asynchronous): the right census
-    // behind the kCensusGuard readable bytes the kernels may touch in front
-    // of it (a caller's buffer need not have them), the flags and the
-    // per-direction volumes.
+    // Stream-ordered scratch (the call stays asynchronous): the cost volume,
+    // the flags and the per-direction volumes.
     hipStream_t st = (hipStream_t)stream;
     const size_t n = (size_t)w * h;
     const int mode = auto_mode(c);
@@ -632,14 +631,15 @@ vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,
     char *g = nullptr;
     hipError_t e = hipMallocAsync((void **)&g, bytes, st);
     if (e != hipSuccess) return hip_fail(e, "hipMallocAsync", bytes);
-    e = hipMemcpyAsync(g + L.rc, d_rc, n * 8, hipMemcpyDeviceToDevice, st);
+    e = vxh::launch_cost((const uint2 *)d_lc, (const uint2 *)d_rc, (int)w, (int)h, D,
+                         (uint8_t *)(g + L.cost), st);
     if (e == hipSuccess && mode == vxh::kOutU8) e = hipMemsetAsync(g + L.flags, 0, n * 8, st);
     if (e == hipSuccess)
         e = vxh::launch_zero((uint32_t *)(g + L.sync), vxh::sync_words(w, h), st);
     vxh_status v2 = e == hipSuccess
-                        ? paths_and_wta((const uint2 *)d_lc, (const uint2 *)(g + L.rc), w, h,
-                                        c, g, L, mode, sc, st, nullptr, nullptr, d_volume)
-                        : hip_fail(e, "hipMemcpyAsync");
+                        ? paths_and_wta((const uint8_t *)(g + L.cost), w, h, c, g, L, mode, sc,
+                                        st, nullptr, nullptr, d_volume)
+                        : hip_fail(e, "launch");
     hipError_t e2 = hipFreeAsync(g, st);
     if (v2 != VXH_OK) return v2;
     if (e2 != hipSuccess) return hip_fail(e2, "hipFreeAsync");

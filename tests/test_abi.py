@@ -125,12 +125,13 @@ def test_server_request_bytes_is_workspace(w, h, D, paths, p2):
     """A served request is admitted against its whole pipeline workspace: census
     of both images, the u8 matching-cost volume (Dp bytes per pixel), the
     exception flags (8 bytes per pixel), the split launch's synchronisation
-    words (ticket, error flag, one flag per tail wave), and per direction a u8
+    words (error flag, a ticket counter per XCD, two flags per scanline
+    group), and per direction a u8
     volume plus its high-byte volume, or a u16 volume; the larger of the two
     formats is reserved."""
     cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=paths, p2=p2)
     n, Dp = w * h, (D + 15) // 16 * 16
-    sync = align256(4 * (2 + 8 * ((w + h) // 2 + 1)))
+    sync = align256(4 * (10 + 16 * ((w + h) // 2 + 1)))
     common = 2 * align256(n * 8) + align256(n * Dp) + sync
     u8 = common + align256(n * 8) + 2 * paths * align256(n * Dp)
     u16 = common + paths * align256(n * Dp * 2)

@@ -341,13 +341,15 @@ def test_schedule_setting_validated(ctx):
         s.close()
 
 
-@pytest.mark.parametrize("w,h,D", [(200, 130, 64), (110, 300, 100), (256, 96, 128), (300, 40, 256),
-                                   (96, 96, 16)])
+@pytest.mark.parametrize("w,h,D", [(600, 80, 64), (110, 420, 100), (512, 96, 128), (300, 40, 256),
+                                   (257, 30, 32), (96, 96, 16)])
 @pytest.mark.parametrize("fmt", ["u8", "u16"])
 def test_split_scanlines_match_oracle(ctx, w, h, D, fmt):
-    """Scanlines of 96 pixels or more run as a head and a tail half at once;
-    the head's overwrite of the tail's first steps must leave the oracle's map,
-    with and without the LR check, and identical to the unsplit run."""
+    """Scanlines of 256 pixels or more (in the directions whose scanlines are
+    longest) run as a head and a warmed-up tail at once; the head's check and
+    any overwrite of the tail's first steps must leave the oracle's map, with
+    and without the LR check, and identical to the unsplit run (96 x 96: no
+    scanline is split)."""
     left, right, _ = vxsgm.synth_pair(w, h, D, 4, 1, 5000 + w + h + D)
     s = vxsgm.SGMStereo(ctx)
     try:
@@ -367,8 +369,10 @@ def test_split_scanlines_match_oracle(ctx, w, h, D, fmt):
 @pytest.mark.parametrize("p2", [300, 2000])
 def test_split_scanlines_with_wraps(ctx, p2):
     """u8 volumes forced with a large P2: the tail's and head's exception flags
-    and high bytes in the re-run steps must resolve to the oracle's sums."""
-    D, w, h = 64, 220, 110
+    and high bytes in the re-run steps must resolve to the oracle's sums (a
+    flagged step never counts as met, so heads run on past the early range and
+    wait for their tails to end)."""
+    D, w, h = 64, 560, 110
     left, right, _ = vxsgm.synth_pair(w, h, D, 5, 1, 6000 + p2)
     cfg = vxsgm.SGMStereoConfig(num_disparities=D, p2=p2, num_paths=8)
     s = vxsgm.SGMStereo(ctx)
@@ -379,6 +383,25 @@ def test_split_scanlines_with_wraps(ctx, p2):
     finally:
         s.close()
     assert_same_disp(got, O.sgm(left, right, D, p2=p2, num_paths=8))
+
+
+@pytest.mark.parametrize("fmt", ["u8", "u16", "auto"])
+def test_split_slow_convergence(ctx, fmt):
+    """Penalties far above the costs (P1 = 1000, P2 = 2000) keep a fresh
+    start's state apart from the true one for longer than the tail's warm-up:
+    heads then continue through the tail's early range, wait for the tail to
+    end and overwrite its values until the states meet."""
+    D, w, h = 64, 700, 64
+    left, right, _ = vxsgm.synth_pair(w, h, D, 6, 1, 6100)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, p1=1000, p2=2000, num_paths=8)
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        s.set_volume_format(D, fmt)
+        s.set_split(D, True)
+        got = s.compute_host(left, right, w, h, cfg)
+    finally:
+        s.close()
+    assert_same_disp(got, O.sgm(left, right, D, p1=1000, p2=2000, num_paths=8))
 
 
 def test_720p_d128_8path_bit_exact(ctx, sgm):

@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 batch_kernel(const PairDesc *__restrict__ pairs, const Item *__restrict__ items,
              int nitems, uint32_t *next, uint32_t *done,
              const uint32_t *__restrict__ need, uint32_t *error) {
-    __shared__ uint2 span[4][kCostSpan];
+    __shared__ uint2 span[4][kCostSlots];
     const int lane = threadIdx.x & 63;
     for (uint32_t idx = wave_take(next, lane); idx < (uint32_t)nitems;
          idx = wave_take(next, lane)) {
@@ -151,7 +151,7 @@ batch_kernel(const PairDesc *__restrict__ pairs, const Item *__restrict__ items,
             for (int y = it.first; y < it.first + it.count; ++y)
                 for (int x = lane; x < pd.w; x += 64) {
                     const size_t i = (size_t)y * pd.w + x;
-                    out[i] = census_pixel(img, pd.w, pd.h, pd.rx, pd.ry, x, y);
+                    out[i] = census_at(img, pd.w, pd.h, pd.rx, pd.ry, x, y);
                     if (fl) fl[i] = make_uint2(0, 0);
                 }
         } else if (it.kind == kCost) {

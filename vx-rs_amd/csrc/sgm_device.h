@@ -112,6 +112,43 @@ __device__ __forceinline__ void scan_start(int s, int dx, int dy, int w, int h,
     len = lx < ly ? lx : ly;
 }
 
+// Census word of pixel (x, y) with a compile-time window of
+// (2 RX + 1) x (2 RY + 1) (SGMStereo.metal:41-65): pixels at least RX / RY
+// inside the image read their window without clamping.
+template <int RX, int RY>
+__device__ __forceinline__ uint2 census_pixel_fixed(const uint8_t *__restrict__ img, int w,
+                                                    int h, int x, int y) {
+    const uint32_t c = img[(size_t)y * w + x];
+    uint64_t bits = 0;
+    if (x >= RX && x + RX < w && y >= RY && y + RY < h) {
+        const uint8_t *base = img + (size_t)(y - RY) * w + (x - RX);
+        int bit = 0;
+#pragma unroll
+        for (int dy = 0; dy <= 2 * RY; ++dy) {
+            const uint8_t *row = base + (size_t)dy * w;
+#pragma unroll
+            for (int dx = 0; dx <= 2 * RX; ++dx) {
+                if (dx == RX && dy == RY) continue;
+                bits |= (uint64_t)((uint32_t)row[dx] < c) << bit;
+                ++bit;
+            }
+        }
+    } else {
+        int bit = 0;
+#pragma unroll
+        for (int dy = -RY; dy <= RY; ++dy) {
+            const uint8_t *row = img + (size_t)min(max(y + dy, 0), h - 1) * w;
+#pragma unroll
+            for (int dx = -RX; dx <= RX; ++dx) {
+                if (dx == 0 && dy == 0) continue;
+                bits |= (uint64_t)((uint32_t)row[min(max(x + dx, 0), w - 1)] < c) << bit;
+                ++bit;
+            }
+        }
+    }
+    return make_uint2((uint32_t)bits, (uint32_t)(bits >> 32));
+}
+
 // Census word of pixel (x, y) (SGMStereo.metal:31-65).
 __device__ __forceinline__ uint2 census_pixel(const uint8_t *__restrict__ img, int w,
                                               int h, int rx, int ry, int x, int y) {
@@ -134,6 +171,13 @@ __device__ __forceinline__ uint2 census_pixel(const uint8_t *__restrict__ img, i
     return make_uint2(lo, hi);
 }
 
+// census_pixel with the default 9x7 window compiled in.
+__device__ __forceinline__ uint2 census_at(const uint8_t *__restrict__ img, int w, int h,
+                                           int rx, int ry, int x, int y) {
+    if (rx == 4 && ry == 3) return census_pixel_fixed<4, 3>(img, w, h, x, y);
+    return census_pixel(img, w, h, rx, ry, x, y);
+}
+
 // Target of the stores of lanes with nothing to store (past the end of their
 // scanline, or padding lanes past the volume stride).  Every memory operation
 // of the scan loop is issued unconditionally, so each step issues the same
@@ -148,15 +192,21 @@ __device__ __forceinline__ T *store_sink() {
 }
 
 // Matching-cost volume tiles.  A wave computes the costs of kCostTile
-// consecutive pixels of one row (one pixel per lane) for all Dp entries of
-// their slices, staging the right census row segment it reads in LDS.
+// consecutive pixels of one row for all Dp entries of their slices, one
+// 16-entry chunk per lane and iteration: the tile's slices are contiguous in
+// the volume, so every store instruction writes 1 KB in one piece.  The
+// right census words the tile reads are staged in LDS first; entry e sits in
+// slot e + 2 * (e / 16), so the lanes of one read (8 chunks of 16 entries
+// apart, of 2 neighbouring pixels per 16 lanes) hit distinct banks.
 constexpr int kCostTile = 64;
-constexpr int kCostSpan = kCostTile + 255;   // right census entries of a tile (D <= 256)
+constexpr int kCostSpan = kCostTile + 255;                   // entries (D <= 256)
+constexpr int kCostSlots = kCostSpan + 2 * ((kCostSpan + 15) / 16);
+__device__ __forceinline__ int cost_slot(int e) { return e + 2 * (e >> 4); }
 
 // Costs of pixels x0 .. x0 + 63 of row y into cost[(y*w + x)*Dp + d]: for
 // d < D the Hamming distance of the left census at x and the right census at
-// x - d, or 64 where x - d < 0 (SGMStereo.metal:125-137); 64 for the padding
-// entries d >= D.  `span` is this wave's LDS area of kCostSpan entries.
+// x - d, or 64 where x - d < 0 (SGMStereo.metal:133-142); 64 for the padding
+// entries d >= D.  `span` is this wave's LDS area of kCostSlots entries.
 // Wave-level: all 64 lanes call it with the same arguments.
 __device__ __forceinline__ void cost_tile(const uint2 *__restrict__ lc,
                                           const uint2 *__restrict__ rc, int w, int D, int Dp,
@@ -166,44 +216,46 @@ __device__ __forceinline__ void cost_tile(const uint2 *__restrict__ lc,
     const int xs0 = max(0, x0 - (D - 1));
     const int xs1 = min(w, x0 + kCostTile);
     const uint2 *rrow = rc + (size_t)y * w + xs0;
-    for (int j = lane; j < xs1 - xs0; j += 64) span[j] = rrow[j];
+    for (int j = lane; j < xs1 - xs0; j += 64) span[cost_slot(j)] = rrow[j];
     // The span's entries come from other lanes of this wave.
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int x = x0 + lane;
-    if (x < w) {
-        const uint2 lv = lc[(size_t)y * w + x];
-        uint4 *dst = (uint4 *)(cost + ((size_t)y * w + x) * Dp);
+    const int nch = Dp >> 4;   // chunks per pixel
+    const int nchunks = min(kCostTile, w - x0) * nch;
+    const uint2 *__restrict__ lrow = lc + (size_t)y * w;
+    uint4 *dst = (uint4 *)(cost + ((size_t)y * w + x0) * Dp);
+    const bool inside = x0 >= D - 1;   // no pixel of the tile has x - d < 0 for d < D
+    for (int c = lane; c < nchunks; c += 64) {
+        const int px = c / nch;
+        const int dc = (c - px * nch) * 16;
+        const int x = x0 + px;
         const int rel = x - xs0;   // span index of x - d is rel - d
-        // Every entry of the chunk is real and every x - d >= 0.
-        auto chunk = [&](int dc, auto checked) {
-            uint32_t wd[4];
+        const uint2 lv = lrow[x];
+        uint32_t wd[4];
+        auto chunk = [&](auto checked) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 uint32_t v = 0;
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
                     const int d = dc + 4 * q + b;
-                    uint32_t c;
+                    uint32_t cst;
                     if constexpr (decltype(checked)::value) {
-                        const uint2 r = span[max(rel - d, 0)];
-                        c = (d < D && d <= x) ? __popc(lv.x ^ r.x) + __popc(lv.y ^ r.y) : 64u;
+                        const uint2 r = span[cost_slot(max(rel - d, 0))];
+                        cst = (d < D && d <= x) ? __popc(lv.x ^ r.x) + __popc(lv.y ^ r.y) : 64u;
                     } else {
-                        const uint2 r = span[rel - d];
-                        c = __popc(lv.x ^ r.x) + __popc(lv.y ^ r.y);
+                        const uint2 r = span[cost_slot(rel - d)];
+                        cst = __popc(lv.x ^ r.x) + __popc(lv.y ^ r.y);
                     }
-                    v |= c << (8 * b);
+                    v |= cst << (8 * b);
                 }
                 wd[q] = v;
             }
-            dst[dc / 16] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
         };
-        const bool inside = x0 >= D - 1;   // wave-uniform: no lane has x - d < 0
-        for (int dc = 0; dc < Dp; dc += 16) {
-            if (inside && dc + 16 <= D) chunk(dc, std::false_type{});
-            else chunk(dc, std::true_type{});
-        }
+        if (inside && dc + 16 <= D) chunk(std::false_type{});
+        else chunk(std::true_type{});
+        dst[c] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
     }
 }
 

@@ -64,10 +64,14 @@ struct Schedule {
     int split;   // 1: long scanlines run as head and tail halves at once
 };
 
-// Words of the split launch's synchronisation area for a w x h pair: the
-// ticket counter, the error flag and one completion flag per tail wave (at
-// most 8 passes of ceil(scanlines / 2) waves).
-inline size_t sync_words(uint32_t w, uint32_t h) { return 2 + 8 * ((size_t)(w + h) / 2 + 1); }
+// Words of the split launch's synchronisation area for a w x h pair: an
+// unused word, the error flag, one ticket counter per XCD (8), then two flags
+// (done, early) per scanline group (at most 8 passes of ceil(scanlines / 2)
+// groups).
+constexpr size_t kSyncFlags = 10;
+inline size_t sync_words(uint32_t w, uint32_t h) {
+    return kSyncFlags + 16 * ((size_t)(w + h) / 2 + 1);
+}
 
 constexpr size_t kWsAlign = 256;
 inline size_t ws_align(size_t v) { return (v + kWsAlign - 1) & ~(kWsAlign - 1); }
@@ -119,15 +123,18 @@ struct PassDesc {
     int dir, dx, dy, nscan;
     int layout;
     int first_block, nblocks;   // unsplit launches: blocks of this pass
-    int first_ticket, nwaves;   // split launches: the pass's tickets start, waves per role
+    int nwaves;                 // scanline groups (waves per role)
+    int first_wave;             // split launches: index of the pass's first group
     int split;                  // split launches: this pass runs tail and head roles
 };
 struct MultiPass {
     PassParams base;       // shared fields; vol/hi/dir/dx/dy/nscan come per pass
     int npass;
     int split;             // some pass runs as tail and head roles (ticketed launch)
-    int total_waves;       // split: tickets over all passes
-    uint32_t *sync;        // split: [0] ticket, [1] error, [2 + t] done flag of tail ticket t
+    int total_waves;       // split: scanline groups over all passes
+    uint32_t *sync;        // split: [1] error, [2 + k] XCD k's ticket counter,
+                           // [kSyncFlags + g] done and [kSyncFlags + total_waves + g]
+                           // early flag of group g's tail
     PassDesc pass[kMaxPasses];
 };
 

@@ -17,7 +17,7 @@ census_kernel(const uint8_t *__restrict__ img, int w, int h, int rx, int ry,
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= w || y >= h) return;
     const size_t i = (size_t)y * w + x;
-    out[i] = census_pixel(img, w, h, rx, ry, x, y);
+    out[i] = census_at(img, w, h, rx, ry, x, y);
     if (zero_flags) zero_flags[i] = make_uint2(0, 0);
     for (size_t k = i; k < nwords; k += (size_t)w * h) zero_words[k] = 0;
 }
@@ -26,7 +26,7 @@ census_kernel(const uint8_t *__restrict__ img, int w, int h, int rx, int ry,
 __global__ void __launch_bounds__(256)
 cost_kernel(const uint2 *__restrict__ lc, const uint2 *__restrict__ rc, int w, int h, int D,
             int Dp, int tiles_x, uint8_t *__restrict__ cost) {
-    __shared__ uint2 span[4][kCostSpan];
+    __shared__ uint2 span[4][kCostSlots];
     const int wv = threadIdx.x >> 6;
     const int tile = blockIdx.x * 4 + wv;
     if (tile >= tiles_x * h) return;   // wave-uniform
@@ -54,15 +54,18 @@ cost_kernel(const uint2 *__restrict__ lc, const uint2 *__restrict__ rc, int w, i
 // stay in that XCD's L2.
 template <int C, int MODE, bool SPLIT>
 __device__ __forceinline__ void run_layout(const PassParams &p, int layout, int wave, int role,
-                                           const uint32_t *tail_done, uint32_t *error) {
+                                           const uint32_t *tail_done, uint32_t *tail_early,
+                                           uint32_t *error) {
     constexpr int L0 = kClassLayouts[C][0][0], K0 = kClassLayouts[C][0][1];
     constexpr int L1 = kClassLayouts[C][1][0], K1 = kClassLayouts[C][1][1];
     constexpr int L2 = kClassLayouts[C][2][0], K2 = kClassLayouts[C][2][1];
 #define VXH_RUN(LL, KK)                                                                       \
     if (p.D % KK == 0)                                                                        \
-        agg_pair_wave<LL, KK, MODE, true, SPLIT>(p, wave * (128 / LL), role, tail_done, error); \
+        agg_pair_wave<LL, KK, MODE, true, SPLIT>(p, wave * (128 / LL), role, tail_done,       \
+                                                 tail_early, error);                          \
     else                                                                                      \
-        agg_pair_wave<LL, KK, MODE, false, SPLIT>(p, wave * (128 / LL), role, tail_done, error)
+        agg_pair_wave<LL, KK, MODE, false, SPLIT>(p, wave * (128 / LL), role, tail_done,      \
+                                                  tail_early, error)
     if (layout == 0) {
         VXH_RUN(L0, K0);
     } else if (layout == 1 || kClassCount[C] < 3) {
@@ -85,40 +88,53 @@ __device__ __forceinline__ int pass_params(const MultiPass &m, int i, PassParams
     return VXH_PASS_FIELD(m, i, layout);
 }
 
-// Split launch: every wave takes a ticket in the order waves start.  Pass j
-// owns tickets [first_ticket, first_ticket + nwaves * (split ? 2 : 1)): a
-// split pass's first nwaves tickets are its tail roles and the next nwaves
-// its head roles, an unsplit pass's are whole scanlines.  A head therefore
-// starts only after its tail has started, so its wait for the tail's
-// completion flag always ends; passes are ordered longest first, so the
-// split ones (the longest) start first and their roles run side by side.
+// Split launch.  Waves take tickets from one counter per XCD (block b from
+// counter b mod 8, the XCD the hardware places it on), in the order they
+// start.  Counter k hands out the k-th eighth of every pass's scanline
+// groups, passes longest first: for a split pass first the tail roles of the
+// chunk, then its head roles, for an unsplit pass whole scanlines.  Each XCD
+// thus works on contiguous scanlines (rows of one page), and a head starts
+// only after its tail has started, so its waits on the tail's flags always
+// end.
 template <int C, int MODE>
 __global__ void __launch_bounds__(256)
 paths_split_kernel(MultiPass m) {
     const int lane = threadIdx.x & 63;
     uint32_t *const sync = m.sync;
-    const uint32_t tk = __hip_atomic_fetch_add(sync, lane == 0 ? 1u : 0u, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
+    const int k = (int)(blockIdx.x & 7);
+    const uint32_t tk = __hip_atomic_fetch_add(sync + 2 + k, lane == 0 ? 1u : 0u,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int ticket = __builtin_amdgcn_readfirstlane((int)tk);
-    if (ticket >= m.total_waves) return;
-    int i = 0;
+    int i = -1, local = 0, c0 = 0, cnt = 0, base = 0;
 #pragma unroll
-    for (int j = 1; j < kMaxPasses; ++j)
-        if (j < m.npass && ticket >= m.pass[j].first_ticket) i = j;
+    for (int j = 0; j < kMaxPasses; ++j) {
+        if (j < m.npass && i < 0) {
+            const int nw = m.pass[j].nwaves;
+            const int a = nw * k / 8, b = nw * (k + 1) / 8;
+            const int n = (b - a) * (m.pass[j].split ? 2 : 1);
+            if (ticket < base + n) {
+                i = j;
+                local = ticket - base;
+                c0 = a;
+                cnt = b - a;
+            }
+            base += n;
+        }
+    }
+    if (i < 0) return;   // wave-uniform: no work left for this XCD
     PassParams p;
     const int layout = pass_params(m, i, p);
-    const int local = ticket - VXH_PASS_FIELD(m, i, first_ticket);
-    const int nw = VXH_PASS_FIELD(m, i, nwaves);
     const bool split = VXH_PASS_FIELD(m, i, split) != 0;
-    const int role = !split ? kRoleWhole : (local < nw ? kRoleTail : kRoleHead);
-    const int wave = local < nw ? local : local - nw;
-    // The tail of this scanline group holds ticket first_ticket + wave.
-    const int tail_ticket = ticket - local + wave;
-    run_layout<C, MODE, true>(p, layout, wave, role, sync + 2 + tail_ticket, sync + 1);
+    const int role = !split ? kRoleWhole : (local < cnt ? kRoleTail : kRoleHead);
+    const int wave = c0 + (local < cnt ? local : local - cnt);
+    // Flags of this scanline group's tail: done, then early.
+    const int gw = VXH_PASS_FIELD(m, i, first_wave) + wave;
+    uint32_t *const done = sync + kSyncFlags + gw;
+    run_layout<C, MODE, true>(p, layout, wave, role, done, done + m.total_waves, sync + 1);
     if (role == kRoleTail) {
         // Every store of this tail is visible before its flag.
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(sync + 2 + ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -136,7 +152,7 @@ paths_kernel(MultiPass m) {
     PassParams p;
     const int layout = pass_params(m, i, p);
     const int wave = lb * 4 + (int)(threadIdx.x >> 6);
-    run_layout<C, MODE, false>(p, layout, wave, kRoleWhole, nullptr, nullptr);
+    run_layout<C, MODE, false>(p, layout, wave, kRoleWhole, nullptr, nullptr, nullptr);
 }
 
 __global__ void __launch_bounds__(256) zero_kernel(uint32_t *p, size_t n) {
@@ -260,20 +276,28 @@ hipError_t launch_paths(const PassParams &base, const int *dirs, uint8_t *const 
     // whose halves then run no longer than the vertical scanlines).
     int longest = 0;
     for (int j = 0; j < n; ++j) longest = std::max(longest, scan_length(m.pass[j].dir, base.w, base.h));
-    int tickets = 0, nsplit = 0;
+    int waves_all = 0, nsplit = 0;
+    int per_xcd[8] = {};   // tickets of each XCD's counter
     for (int j = 0; j < n; ++j) {
         PassDesc &d = m.pass[j];
         const int len = scan_length(d.dir, base.w, base.h);
         d.split = sc.split && len >= kSplitMin && 3 * len > 2 * longest;
-        d.first_ticket = tickets;
-        tickets += d.nwaves * (d.split ? 2 : 1);
+        d.first_wave = waves_all;
+        waves_all += d.nwaves;
         nsplit += d.split;
+        for (int k = 0; k < 8; ++k)
+            per_xcd[k] += (d.nwaves * (k + 1) / 8 - d.nwaves * k / 8) * (d.split ? 2 : 1);
     }
     for (int j = n; j < kMaxPasses; ++j) m.pass[j] = m.pass[0];
-    m.split = sync != nullptr && nsplit > 0 && (size_t)tickets + 2 <= sync_words(base.w, base.h);
-    m.total_waves = tickets;
+    m.split = sync != nullptr && nsplit > 0 &&
+              kSyncFlags + 2 * (size_t)waves_all <= sync_words(base.w, base.h);
+    m.total_waves = waves_all;
     m.sync = sync;
-    if (m.split) blocks = (tickets + 3) / 4;
+    if (m.split) {
+        // Every counter's tickets are taken: 8 equal shares of blocks.
+        const int most = *std::max_element(per_xcd, per_xcd + 8);
+        blocks = 8 * ((most + 3) / 4);
+    }
 #define VXH_PATHS_CASE(CC, MM)                                                              \
     if (C == CC && mode == MM) {                                                            \
         if (m.split)                                                                        \

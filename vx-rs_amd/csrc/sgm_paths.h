@@ -120,19 +120,26 @@ struct TailBuf {
     uint32_t flag[2];
 };
 
-// A scanline of length len is split at split_point(len): its tail role runs
-// [split, len) from the same fresh state as any path start, its head role
-// runs [0, split) and then continues into the tail's range until its state
-// equals the tail's stored one bit for bit (every later step then agrees, the
-// recurrence being deterministic), overwriting the tail's values until then.
-constexpr int kSplitMin = 96;
-__device__ __forceinline__ int split_point(int len) { return len >= kSplitMin ? len / 2 : len; }
+// A scanline of length len >= kSplitMin is split at m = split_point(len).
+// Its tail role starts kSplitWarm steps before m from the same fresh state as
+// any path start, runs those warm-up steps without storing (the state forgets
+// its start within a few dozen steps) and stores [m, len).  Its head role runs
+// [0, m) and then continues into the tail's range until its state equals the
+// tail's stored one bit for bit (every later step then agrees, the recurrence
+// being deterministic), overwriting the tail's values until then; after the
+// warm-up that is almost always the first step, m.  m balances the two:
+// m = len - m + kSplitWarm.
+constexpr int kSplitWarm = 96;
+constexpr int kSplitMin = 256;
+__device__ __forceinline__ int split_point(int len) {
+    return len >= kSplitMin ? (len + kSplitWarm) / 2 : len;
+}
 
 // Roles of a wave in a split launch.
 constexpr int kRoleWhole = 0, kRoleHead = 1, kRoleTail = 2;
 
-// Bounded wait for a tail's completion flag (seconds of polling): a head
-// that times out raises *error and treats the tail as complete.
+// Bounded wait for one of a tail's flags (seconds of polling): a head that
+// times out raises *error and treats the flag as set.
 __device__ __forceinline__ void wait_flag(const uint32_t *flag, uint32_t *error) {
     for (uint32_t n = 0; n < (1u << 22); ++n) {
         const uint32_t v = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -151,14 +158,18 @@ __device__ __forceinline__ void wait_flag(const uint32_t *flag, uint32_t *error)
 // storage MODE (kOutU8 or kOutU16; -1 = p.mode at run time).  ALIGNED: the
 // caller guarantees D % K == 0, so every lane is either wholly inside D or
 // wholly padding and only the element at D - 1 needs its d+1 term masked.
-// SPLIT: `role` is kRoleHead or kRoleTail of split scanlines (see
-// split_point); a head waits on `tail_done` (set by its tail wave once all
-// the tail's stores are visible) before it reads the tail's values.
+// SPLIT: `role` is kRoleWhole, kRoleHead or kRoleTail of split scanlines
+// (see split_point).  A tail sets *tail_early (release) once its values of
+// the first kSplitRange steps from m are stored; its caller sets *tail_done
+// once all of them are.  A head reads the tail's values of [m, m + range)
+// after *tail_early and any later ones after *tail_done, which also ends the
+// tail's stores, so the head may then overwrite them.
 // Wave-level: every lane of the wave must call it with the same arguments.
 template <int L, int K, int MODE, bool ALIGNED, bool SPLIT = false>
 __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
                                               int role = kRoleWhole,
                                               const uint32_t *tail_done = nullptr,
+                                              uint32_t *tail_early = nullptr,
                                               uint32_t *error = nullptr) {
     static_assert(K >= 2 && K <= 16 && (K & (K - 1)) == 0, "K must be 2, 4, 8 or 16");
     const int lane = threadIdx.x & 63;
@@ -175,6 +186,8 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     int sx[2] = {0, 0}, len[2] = {0, 0}, pix0[2] = {0, 0};
     // Head role: steps [vfrom, len) of a scanline check the tail's values.
     int vfrom[2] = {0x7FFFFFFF, 0x7FFFFFFF};
+    // Tail role: the steps before sfrom are the unstored warm-up.
+    int sfrom[2] = {0, 0};
 #pragma unroll
     for (int hh = 0; hh < 2; ++hh) {
         const int s = s_first + 2 * g + hh;
@@ -182,9 +195,11 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
         if (s < p.nscan) scan_start(s, dx, dy, w, h, sx[hh], sy, len[hh]);
         if (SPLIT && role == kRoleTail) {
             const int m = split_point(len[hh]);
-            sx[hh] += m * dx;
-            sy += m * dy;
-            len[hh] -= m;
+            const int start = m < len[hh] ? m - kSplitWarm : len[hh];
+            sx[hh] += start * dx;
+            sy += start * dy;
+            len[hh] -= start;
+            sfrom[hh] = m - start;
         } else if (SPLIT && role == kRoleHead) {
             const int m = split_point(len[hh]);
             if (m < len[hh]) vfrom[hh] = m;
@@ -244,6 +259,13 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
         realmask[q] = m;
     }
 
+    // Loads run PF steps ahead through a ring of NB = PF + 1 buffers (see the
+    // main loop below).
+    constexpr int PF = K <= 4 ? 6 : (K <= 8 ? 4 : 3);
+    constexpr int NB = PF + 1;
+    // Steps from m that a tail publishes early (two rounds).
+    constexpr int kRange = 2 * NB;
+
     auto issue = [&](PairBuf<K> &b, int t) {
 #pragma unroll
         for (int hh = 0; hh < 2; ++hh) {
@@ -269,7 +291,8 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     // tail has finished; other lanes and steps read the sink).
     auto issue_tail = [&](TailBuf<K> &tb, int t) {
         if (t < vfirst) return;   // wave-uniform: only the checking steps load
-        if (t == vfirst) wait_flag(tail_done, error);
+        if (t == vfirst) wait_flag(tail_early, error);
+        if (t == vfirst + kRange) wait_flag(tail_done, error);
 #pragma unroll
         for (int hh = 0; hh < 2; ++hh) {
             const bool v = t >= vfrom[hh] && t < len[hh] && lane_store;
@@ -328,7 +351,9 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
         high_lanes = 0;
     };
 
-    auto step = [&](PairBuf<K> &b, TailBuf<K> &tb, int t) {
+    // VERIFY: a head's step at or after vfirst (compares with the tail).
+    auto step = [&](PairBuf<K> &b, TailBuf<K> &tb, int t, auto verify) {
+        constexpr bool VERIFY = decltype(verify)::value;
         const bool act0 = t < len[0], act1 = t < len[1];
         // Costs of both scanlines, packed: byte i of each slice segment into
         // the low (first scanline) and high (second) half.
@@ -392,8 +417,13 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
         for (int i = 0; i < K; ++i) prev[i] = agg[i];
 
         bool st0 = act0 && lane_store, st1 = act1 && lane_store;
+        if constexpr (SPLIT) {
+            // Tail warm-up steps store nothing (sfrom is 0 for other roles).
+            st0 = st0 && t >= sfrom[0];
+            st1 = st1 && t >= sfrom[1];
+        }
         uint32_t clear_flag = 0;   // head role: u8 flags this step rewrites (bit per scanline)
-        if (SPLIT && role == kRoleHead && t >= vfirst) {
+        if (VERIFY && t >= vfirst) {
             // Compare this step's state with the tail's stored one, scanline
             // by scanline over the whole group (entries d < D): from the
             // first step where they agree, the tail's values stand.
@@ -469,40 +499,68 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     // every lane: their loads read in-range words, their stores go to the
     // sink).  With no branch around any load or store, every step issues the
     // same memory operations and waits only for its own PF-old loads.
-    // The head role's checks of the tail's values load them in the same
-    // ring, PF steps ahead (after the wait on the tail).  A head ends as soon
-    // as every scanline of the wave has met its tail or ended (checked once
-    // per round).
-    constexpr int PF = K <= 4 ? 6 : (K <= 8 ? 4 : 3);
-    constexpr int NB = PF + 1;
+    // Split roles run the same loop: a tail publishes its early range once
+    // (at round publish_round), a head leaves it at the round holding vfirst
+    // and continues in a second loop that also loads and checks the tail's
+    // values; it ends as soon as every scanline of the wave has met its tail
+    // or ended (checked once per round).
+    auto publish = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(tail_early, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
     const int rounds = __builtin_amdgcn_readfirstlane((maxlen + NB - 1) / NB);
+    // A tail's values of step kSplitWarm + kRange - 1 are flushed (high bytes
+    // included) once step kSplitWarm + kRange has run, at the end of this
+    // round: every step it publishes early is then in memory before the flag.
+    // A tail too short for that publishes after its last flush instead.
+    const int publish_round =
+        SPLIT && role == kRoleTail && (kSplitWarm + kRange) / NB < rounds - 1
+            ? (kSplitWarm + kRange) / NB
+            : -1;
+    const int rounds_main =
+        SPLIT && role == kRoleHead ? min(rounds, __builtin_amdgcn_readfirstlane(vfirst) / NB)
+                                   : rounds;
     PairBuf<K> buf[NB];
     TailBuf<K> tbuf[SPLIT ? NB : 1];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
         issue(buf[j], j);
-        if constexpr (SPLIT) issue_tail(tbuf[j], j);
         __builtin_amdgcn_sched_barrier(0);
     }
     int r = 0;
-    for (; r < rounds; ++r) {
+    for (; r < rounds_main; ++r) {
         const int t = r * NB;
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             issue(buf[(u + PF) % NB], t + u + PF);
-            if constexpr (SPLIT) issue_tail(tbuf[(u + PF) % NB], t + u + PF);
-            step(buf[u], tbuf[SPLIT ? u : 0], t + u);
+            step(buf[u], tbuf[0], t + u, std::false_type{});
         }
-        if (SPLIT && role == kRoleHead && t + NB > vfirst) {
-            const int tn = t + NB;
-            const bool live = (tn < len[0] && !met[0]) || (tn < len[1] && !met[1]);
-            if (__builtin_amdgcn_uicmp((uint32_t)live, 0u, 33) == 0) {
-                ++r;
-                break;
+        if (SPLIT && r == publish_round) publish();
+    }
+    if constexpr (SPLIT) {
+        if (role == kRoleHead && r < rounds) {
+            // The tail's values of the steps whose costs the ring holds.
+#pragma unroll
+            for (int j = 0; j < PF; ++j) issue_tail(tbuf[j], r * NB + j);
+            for (; r < rounds; ++r) {
+                const int t = r * NB;
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    issue(buf[(u + PF) % NB], t + u + PF);
+                    issue_tail(tbuf[(u + PF) % NB], t + u + PF);
+                    step(buf[u], tbuf[u], t + u, std::true_type{});
+                }
+                const int tn = t + NB;
+                const bool live = (tn < len[0] && !met[0]) || (tn < len[1] && !met[1]);
+                if (tn > vfirst && __builtin_amdgcn_uicmp((uint32_t)live, 0u, 33) == 0) {
+                    ++r;
+                    break;
+                }
             }
         }
     }
     if (mode == kOutU8) flush_high(r * NB - 1);
+    if (SPLIT && role == kRoleTail && publish_round < 0) publish();
     // Loads still in flight (the ring runs PF steps ahead) complete before
     // the wave ends.
     __builtin_amdgcn_s_waitcnt(0);
@@ -564,6 +622,9 @@ __device__ __forceinline__ void wta_slice(const u16x2 (&v)[2][4], int nc, int j,
 // 16 bytes, for u16 entries; E = uint2, 8 bytes, for u8), fl the pixel's
 // exception flags (u8 only).  Split from the sums so that a caller can keep
 // the loads of the next pixels in flight while it sums the current ones.
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+
 template <int MODE>
 struct WtaIn {
     using E = typename std::conditional<MODE == kOutU16, uint4, uint2>::type;
@@ -591,7 +652,15 @@ __device__ __forceinline__ void wta_load(WtaIn<MODE> &in, const uint8_t *__restr
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             if (k >= nv) break;
-            in.q[c][k] = *(const E *)(vol0 + k * vstride + (base + d0) * sizeof(E) / 8);
+            // Read once: non-temporal loads (measured 25% faster here).
+            const E *src = (const E *)(vol0 + k * vstride + (base + d0) * sizeof(E) / 8);
+            if constexpr (sizeof(E) == 8) {
+                const v2u v = __builtin_nontemporal_load((const v2u *)src);
+                in.q[c][k] = E{v.x, v.y};
+            } else {
+                const v4u v = __builtin_nontemporal_load((const v4u *)src);
+                in.q[c][k] = E{v.x, v.y, v.z, v.w};
+            }
         }
     }
 }

@@ -1,6 +1,7 @@
 /*
  * vxh batched SGM: many stereo pairs of uneven resolution and disparity range
- * in one persistent-kernel launch, captured in a hipGraph.
+ * replayed from one hipGraph that runs the per-pair pipeline of every pair
+ * over several streams at once.
  *
  * No single reference entry point corresponds: the reference computes one
  * pair per SGMStereo::compute_disparity call
@@ -25,9 +26,8 @@ extern "C" {
  * out w*h floats, all resident on the batch's device), its config, the
  * aggregation layouts as for vxh_sgm_set_schedule ((lanes, per_lane) for the
  * non-horizontal directions, (lanes_h, per_lane_h) for the horizontal ones,
- * 0, 0 = default; the batched kernel runs 8 disparities per lane: 8x8,
- * 16x8 or 32x8 for D <= 64, <= 128, <= 256, and rejects other layouts), and
- * volume_format as for vxh_sgm_set_volume_format (0 = automatic). */
+ * 0, 0 = default), volume_format as for vxh_sgm_set_volume_format
+ * (0 = automatic) and split as for vxh_sgm_set_split (0 = off). */
 typedef struct {
     const uint8_t *d_left;
     const uint8_t *d_right;
@@ -40,35 +40,36 @@ typedef struct {
     uint32_t lanes_h;
     uint32_t per_lane_h;
     uint32_t volume_format;
+    uint32_t split;
 } vxh_sgm_pair_desc;
 
 typedef struct vxh_sgm_batch vxh_sgm_batch;
 
-/* Schedule of the persistent launch (the autotuner's knobs; 0 = default):
- *   blocks_per_cu  resident 256-thread workgroups per CU, 1 or 2 (default 2);
- *   wta_lag        pairs of aggregation items placed between a pair's last
- *                  aggregation item and its sum + winner items (default 2);
- *   wta_pixels     pixels per sum + winner item, a multiple of 4 (default
- *                  1024). */
+/* Launch options (the autotuner's knob; 0 = default): `streams`, 1..16
+ * (default 4), is the number of streams the pairs are dealt to, largest pairs
+ * first; pairs on one stream run in turn and share one workspace. */
 typedef struct {
-    uint32_t blocks_per_cu;
-    uint32_t wta_lag;
-    uint32_t wta_pixels;
+    uint32_t streams;
 } vxh_sgm_batch_options;
 
 /* Per-pair plan summary. */
 typedef struct {
-    uint32_t items;      /* work items of this pair in the batch's list */
-    uint32_t lanes;      /* schedule used for the non-horizontal directions */
+    uint32_t stream;          /* stream the pair runs on */
+    uint32_t lanes;           /* layout of the non-horizontal directions */
     uint32_t per_lane;
+    uint32_t lanes_h;         /* layout of the horizontal directions */
+    uint32_t per_lane_h;
+    uint32_t volume_format;   /* 1 = u8, 2 = u16 */
+    uint64_t workspace_bytes; /* scratch the pair needs */
 } vxh_sgm_batch_stats;
 
-/* Validates every pair, allocates the per-pair scratch (census, direction
- * volumes; vxh_sgm_workspace_bytes per pair), builds the work list and captures the persistent launch in a
- * hipGraph.  Buffers named in `pairs` are bound at creation. */
+/* Validates every pair, allocates one workspace per stream (the largest
+ * vxh_sgm_workspace_bytes of the pairs dealt to it) and captures the
+ * pipelines of all pairs in a hipGraph.  Buffers named in `pairs` are bound
+ * at creation. */
 vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs,
                                 uint32_t n, vxh_sgm_batch **out);
-/* vxh_sgm_batch_create with an explicit launch schedule (NULL = defaults). */
+/* vxh_sgm_batch_create with explicit options (NULL = defaults). */
 vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs,
                                    uint32_t n, const vxh_sgm_batch_options *options,
                                    vxh_sgm_batch **out);
@@ -76,11 +77,12 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs,
 vxh_status vxh_sgm_batch_launch(vxh_sgm_batch *batch, void *stream);
 /* Launch on the batch's own stream, wait, and check the error flag. */
 vxh_status vxh_sgm_batch_run(vxh_sgm_batch *batch);
-/* After a launch has completed: VXH_ERR_GPU if any item's bounded dependency
- * wait expired (the batch's maps are then incomplete), else VXH_OK. */
+/* After a launch has completed: VXH_ERR_GPU if any pair's bounded wait of a
+ * split scanline expired (its map is then incomplete), else VXH_OK. */
 vxh_status vxh_sgm_batch_check(vxh_sgm_batch *batch);
 vxh_status vxh_sgm_batch_stats_get(const vxh_sgm_batch *batch, uint32_t pair,
                                    vxh_sgm_batch_stats *out);
+/* Kernel launches in the batch's graph. */
 uint32_t vxh_sgm_batch_items(const vxh_sgm_batch *batch);
 void vxh_sgm_batch_destroy(vxh_sgm_batch *batch);
 /* Message of the last batch failure on this thread. */

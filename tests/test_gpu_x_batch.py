@@ -1,6 +1,7 @@
-"""GPU parity of the batched form: every pair of a persistent, graph-captured
-batch equals the plain per-pair path and the oracle bit for bit, for mixed
-sizes, disparity ranges, path counts, schedules and repeated graph replays."""
+"""GPU parity of the batched form: every pair of a graph-captured batch (the
+per-pair pipelines over several streams) equals the plain per-pair path and
+the oracle bit for bit, for mixed sizes, disparity ranges, path counts,
+schedules, split settings, stream counts and repeated graph replays."""
 import numpy as np
 import pytest
 
@@ -12,7 +13,7 @@ from vxsgm import hip
 pytestmark = pytest.mark.gpu
 
 
-def run_batch(pairs, scheds=None, replays=1, formats=None):
+def run_batch(pairs, scheds=None, replays=1, formats=None, splits=None, options=None):
     """pairs: [(left, right, cfg)] host arrays.  Returns list of host maps."""
     bufs, descs = [], []
     for i, (l, r, cfg) in enumerate(pairs):
@@ -22,8 +23,9 @@ def run_batch(pairs, scheds=None, replays=1, formats=None):
         bufs.append((dl, dr, do))
         sched = scheds[i] if scheds else None
         fmt = formats[i] if formats else "auto"
-        descs.append((dl.ptr, dr.ptr, do.ptr, w, h, cfg, sched, fmt))
-    b = vxsgm.SGMBatch(descs)
+        split = splits[i] if splits else False
+        descs.append((dl.ptr, dr.ptr, do.ptr, w, h, cfg, sched, fmt, split))
+    b = vxsgm.SGMBatch(descs, options=options)
     outs = []
     for _ in range(replays):
         for _, _, do in bufs:
@@ -67,10 +69,10 @@ def test_batch_small_mix_matches_oracle():
     assert not bad, "\n".join(bad)
 
 
-@pytest.mark.parametrize("opts", [None, (1, 1, 256), (2, 4, 4096)])
+@pytest.mark.parametrize("opts", [None, (1,), (3,), (16,)])
 def test_batch_launch_schedules_equal_plain(ctx, opts):
-    """Every launch schedule the tuner tries gives the plain path's maps, and
-    the plain path gives the same maps under each of its lane layouts."""
+    """Every stream count gives the plain path's maps, and the plain path
+    gives the same maps under each of its lane layouts."""
     pairs = [vxsgm.mixture_pair(256, 96, 128, s)[:2] for s in range(3)]
     cfg = vxsgm.SGMStereoConfig(num_disparities=128, num_paths=8)
     bufs, descs = [], []
@@ -116,8 +118,34 @@ def test_batch_rejects_bad_pair():
     with pytest.raises(vxsgm.InvalidConfig):
         vxsgm.SGMBatch([(l.ptr, l.ptr, o.ptr, 64, 8, bad, None)])
     ok = vxsgm.SGMStereoConfig(num_disparities=64)
-    for sched in ((16, 2), (32, 2), ((8, 8), (16, 4))):
+    for sched in ((16, 2), (64, 2), ((8, 8), (4, 16))):
         with pytest.raises(vxsgm.InvalidConfig):
             vxsgm.SGMBatch([(l.ptr, l.ptr, o.ptr, 64, 8, ok, sched)])
     with pytest.raises(vxsgm.InvalidConfig):
-        vxsgm.SGMBatch([(l.ptr, l.ptr, o.ptr, 64, 8, ok, None)], options=(3, 2, 1024))
+        vxsgm.SGMBatch([(l.ptr, l.ptr, o.ptr, 64, 8, ok, None)], options=(17,))
+
+
+def test_batch_schedules_and_splits_per_pair():
+    """Each pair keeps its own lane layouts and split setting; streams share
+    workspaces between pairs of different sizes, formats and path counts."""
+    pairs, scheds, splits, formats = [], [], [], []
+    layouts = {64: [(32, 2), (16, 4), (8, 8)], 128: [(64, 2), (32, 4), (16, 8)],
+               256: [(64, 4), (32, 8)]}
+    for i, (w, h, d, paths) in enumerate([(600, 60, 64, 8), (300, 280, 128, 8), (90, 40, 64, 4),
+                                          (520, 50, 256, 8), (270, 260, 128, 2),
+                                          (64, 300, 32, 8), (400, 30, 96, 8)]):
+        l, r = random_pair(w, h, seed=300 + i)
+        cfg = vxsgm.SGMStereoConfig(num_disparities=d, num_paths=paths, lr_check=(i % 3 == 0))
+        pairs.append((l, r, cfg))
+        cls = 64 if d <= 64 else (128 if d <= 128 else 256)
+        lay = layouts[cls]
+        scheds.append((lay[i % len(lay)], lay[(i + 1) % len(lay)]))
+        splits.append(i % 2 == 0)
+        formats.append(["auto", "u16", "u8"][i % 3])
+    outs = run_batch(pairs, scheds=scheds, replays=2, formats=formats, splits=splits,
+                     options=(3,))
+    for rep in outs:
+        for (l, r, cfg), got in zip(pairs, rep):
+            want = O.sgm(l, r, cfg.num_disparities, num_paths=cfg.num_paths,
+                         lr_check=cfg.lr_check).ravel()
+            assert (bits(got) == bits(want)).all()

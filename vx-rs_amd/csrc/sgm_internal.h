@@ -150,6 +150,8 @@ hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
 // PassParams::cost, Dp = volume_stride(D)).
 hipError_t launch_cost(const uint2 *lc, const uint2 *rc, int w, int h, int D, uint8_t *cost,
                        hipStream_t s);
+// *dst |= *src with a kernel (stream-ordered, safe inside captured graphs).
+hipError_t launch_or_word(uint32_t *dst, const uint32_t *src, hipStream_t s);
 // Zeroes n words with a kernel (safe inside captured graphs).
 hipError_t launch_zero(uint32_t *p, size_t n, hipStream_t s);
 // Default schedule for D disparities.

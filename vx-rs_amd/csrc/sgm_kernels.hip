@@ -155,6 +155,10 @@ paths_kernel(MultiPass m) {
     run_layout<C, MODE, false>(p, layout, wave, kRoleWhole, nullptr, nullptr, nullptr);
 }
 
+__global__ void or_word_kernel(uint32_t *dst, const uint32_t *src) {
+    if (threadIdx.x == 0) *dst |= *src;
+}
+
 __global__ void __launch_bounds__(256) zero_kernel(uint32_t *p, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -223,6 +227,11 @@ hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
     dim3 grid((w + 63) / 64, (h + 3) / 4);
     hipLaunchKernelGGL(census_kernel, grid, block, 0, s, img, w, h, rx, ry, out, zero_flags,
                        zero_words, zero_words ? nwords : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_or_word(uint32_t *dst, const uint32_t *src, hipStream_t s) {
+    hipLaunchKernelGGL(or_word_kernel, dim3(1), dim3(64), 0, s, dst, src);
     return hipGetLastError();
 }
 

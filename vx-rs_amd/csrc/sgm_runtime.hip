@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "sgm_internal.h"
+#include "sgm_pipeline.h"
 #include "vxh/sgm.h"
 
 namespace {
@@ -348,8 +349,49 @@ static vxh_status paths_and_wta(const uint8_t *cost, uint32_t w, uint32_t h,
     return VXH_OK;
 }
 
-// The whole pipeline on `st`.  With `marks`, records marks[0] after the
-// census pass, marks[1] after the path aggregation and marks[2] at the end.
+}  // extern "C"
+
+namespace vxh {
+
+// The whole pipeline of one pair on `st` (inputs validated by the caller;
+// `ws` holds workspace_layout(w, h, D, paths, mode, lr) bytes).  With
+// `marks`, records marks[0] after the census and cost passes, marks[1] after
+// the path aggregation and marks[2] at the end.
+vxh_status run_pipeline(const uint8_t *d_left, const uint8_t *d_right, uint32_t w, uint32_t h,
+                        const vxh_sgm_config *c, float *d_out, char *base, int mode,
+                        Schedule sc, hipStream_t st, hipEvent_t const *marks) {
+    const int D = (int)c->num_disparities;
+    const bool lr = c->lr_check != 0;
+    const WsLayout L = workspace_layout(w, h, D, (int)c->num_paths, mode, lr);
+    uint2 *lc = (uint2 *)(base + L.lc);
+    uint2 *rc = (uint2 *)(base + L.rc);
+    uint2 *flags = mode == kOutU8 ? (uint2 *)(base + L.flags) : nullptr;
+    VXH_HIP(launch_census(d_left, w, h, c->census_radius_x, c->census_radius_y, lc, flags,
+                          (uint32_t *)(base + L.sync), sync_words(w, h), st),
+            "census");
+    VXH_HIP(launch_census(d_right, w, h, c->census_radius_x, c->census_radius_y, rc, nullptr,
+                          nullptr, 0, st),
+            "census");
+    uint8_t *cost = (uint8_t *)(base + L.cost);
+    VXH_HIP(launch_cost(lc, rc, (int)w, (int)h, D, cost, st), "cost");
+    if (marks) VXH_HIP(hipEventRecord(marks[0], st), "hipEventRecord");
+    int16_t *best = lr ? (int16_t *)(base + L.best) : nullptr;
+    uint16_t *sum = lr ? (uint16_t *)(base + L.sum) : nullptr;
+    vxh_status v = paths_and_wta(cost, w, h, c, base, L, mode, sc, st, d_out, best, sum,
+                                 marks ? marks[1] : nullptr);
+    if (v != VXH_OK) return v;
+    if (lr)
+        VXH_HIP(launch_lr_check(sum, w, h, D, volume_stride(D), (int)c->lr_tolerance, best,
+                                d_out, st),
+                "lr_check");
+    if (marks) VXH_HIP(hipEventRecord(marks[2], st), "hipEventRecord");
+    return VXH_OK;
+}
+
+}  // namespace vxh
+
+extern "C" {
+
 static vxh_status enqueue_impl(vxh_sgm *s, const uint8_t *d_left, const uint8_t *d_right,
                                uint32_t w, uint32_t h, const vxh_sgm_config *c,
                                float *d_out, void *d_ws, size_t ws_bytes,
@@ -360,38 +402,15 @@ static vxh_status enqueue_impl(vxh_sgm *s, const uint8_t *d_left, const uint8_t 
     if (!d_left || !d_right || !d_out || !d_ws)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null device buffer");
     const int mode = mode_for(s, c);
-    const bool lr = c->lr_check != 0;
-    const WsLayout L = workspace_layout(w, h, D, (int)c->num_paths, mode, lr);
+    const WsLayout L = workspace_layout(w, h, D, (int)c->num_paths, mode, c->lr_check != 0);
     if (ws_bytes < L.total) {
         char buf[128];
         snprintf(buf, sizeof buf, "invalid config: workspace %zu < %zu bytes",
                  ws_bytes, L.total);
         return fail(VXH_ERR_INVALID_CONFIG, buf);
     }
-    char *base = (char *)d_ws;
-    uint2 *lc = (uint2 *)(base + L.lc);
-    uint2 *rc = (uint2 *)(base + L.rc);
-    uint2 *flags = mode == vxh::kOutU8 ? (uint2 *)(base + L.flags) : nullptr;
-    VXH_HIP(vxh::launch_census(d_left, w, h, c->census_radius_x, c->census_radius_y, lc,
-                               flags, (uint32_t *)(base + L.sync), vxh::sync_words(w, h), st),
-            "census");
-    VXH_HIP(vxh::launch_census(d_right, w, h, c->census_radius_x, c->census_radius_y, rc,
-                               nullptr, nullptr, 0, st),
-            "census");
-    uint8_t *cost = (uint8_t *)(base + L.cost);
-    VXH_HIP(vxh::launch_cost(lc, rc, (int)w, (int)h, D, cost, st), "cost");
-    if (marks) VXH_HIP(hipEventRecord(marks[0], st), "hipEventRecord");
-    int16_t *best = lr ? (int16_t *)(base + L.best) : nullptr;
-    uint16_t *sum = lr ? (uint16_t *)(base + L.sum) : nullptr;
-    v = paths_and_wta(cost, w, h, c, base, L, mode, schedule_for(s, D), st, d_out, best,
-                      sum, marks ? marks[1] : nullptr);
-    if (v != VXH_OK) return v;
-    if (lr)
-        VXH_HIP(vxh::launch_lr_check(sum, w, h, D, vxh::volume_stride(D),
-                                     (int)c->lr_tolerance, best, d_out, st),
-                "lr_check");
-    if (marks) VXH_HIP(hipEventRecord(marks[2], st), "hipEventRecord");
-    return VXH_OK;
+    return vxh::run_pipeline(d_left, d_right, w, h, c, d_out, (char *)d_ws, mode,
+                             schedule_for(s, D), st, marks);
 }
 
 vxh_status vxh_sgm_enqueue(vxh_sgm *s, const uint8_t *d_left, const uint8_t *d_right,

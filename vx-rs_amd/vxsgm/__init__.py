@@ -360,15 +360,16 @@ class vxh_sgm_pair_desc(ctypes.Structure):
     _fields_ = [("d_left", _vp), ("d_right", _vp), ("d_out", _vp), ("width", _u32),
                 ("height", _u32), ("cfg", vxh_sgm_config), ("lanes", _u32),
                 ("per_lane", _u32), ("lanes_h", _u32), ("per_lane_h", _u32),
-                ("volume_format", _u32)]
+                ("volume_format", _u32), ("split", _u32)]
 
 
 class vxh_sgm_batch_options(ctypes.Structure):
-    _fields_ = [("blocks_per_cu", _u32), ("wta_lag", _u32), ("wta_pixels", _u32)]
+    _fields_ = [("streams", _u32)]
 
 
 class vxh_sgm_batch_stats(ctypes.Structure):
-    _fields_ = [("items", _u32), ("lanes", _u32), ("per_lane", _u32)]
+    _fields_ = [("stream", _u32), ("lanes", _u32), ("per_lane", _u32), ("lanes_h", _u32),
+                ("per_lane_h", _u32), ("volume_format", _u32), ("workspace_bytes", ctypes.c_uint64)]
 
 
 _sig("vxh_sgm_batch_create", ctypes.c_int, ctypes.c_int, ctypes.POINTER(vxh_sgm_pair_desc),
@@ -391,19 +392,20 @@ def _check_batch(status):
 
 
 class SGMBatch:
-    """Many pairs in one persistent launch replayed from a hipGraph.
+    """Many pairs replayed from one hipGraph over several streams.
 
     `pairs` is a list of (d_left, d_right, d_out, width, height, config,
-    schedule[, volume_format]) with device addresses; schedule is
+    schedule[, volume_format[, split]]) with device addresses; schedule is
     (lanes, per_lane) for every direction, ((lanes_h, per_lane_h),
-    (lanes, per_lane)), or None; volume_format "auto", "u8" or "u16".
-    `options` is (blocks_per_cu, wta_lag, wta_pixels) or None (defaults)."""
+    (lanes, per_lane)), or None; volume_format "auto", "u8" or "u16"; split
+    a bool.  `options` is (streams,) or None (default: 4 streams)."""
 
     def __init__(self, pairs, device=0, options=None):
         arr = (vxh_sgm_pair_desc * len(pairs))()
         for i, desc in enumerate(pairs):
             l, r, o, w, h, cfg, sched = desc[:7]
             fmt = {"auto": 0, "u8": 1, "u16": 2}[desc[7] if len(desc) > 7 else "auto"]
+            split = int(bool(desc[8])) if len(desc) > 8 else 0
             if not sched:
                 hs = os = (0, 0)
             elif isinstance(sched[0], (tuple, list)):
@@ -411,7 +413,7 @@ class SGMBatch:
             else:
                 hs = os = sched
             arr[i] = vxh_sgm_pair_desc(l, r, o, w, h, cfg.to_c(), os[0], os[1], hs[0],
-                                       hs[1], fmt)
+                                       hs[1], fmt, split)
         h = _vp()
         opt = ctypes.byref(vxh_sgm_batch_options(*options)) if options else None
         _check_batch(lib.vxh_sgm_batch_create_ex(device, arr, len(pairs), opt, ctypes.byref(h)))
@@ -427,7 +429,9 @@ class SGMBatch:
     def stats(self, i):
         s = vxh_sgm_batch_stats()
         _check_batch(lib.vxh_sgm_batch_stats_get(self._h, i, ctypes.byref(s)))
-        return {"items": s.items, "lanes": s.lanes, "per_lane": s.per_lane}
+        return {"stream": s.stream, "schedule": ((s.lanes_h, s.per_lane_h), (s.lanes, s.per_lane)),
+                "volume_format": {1: "u8", 2: "u16"}[s.volume_format],
+                "workspace_bytes": s.workspace_bytes}
 
     @property
     def items(self):

@@ -11,11 +11,9 @@ speed does, and differently for the two forms:
       directions (the longest scanlines; wide groups cut the critical path)
       and one to the others, and every pair of layouts is timed through the
       per-pair pipeline.
-  tune_batch(pairs)  a batch is bound by instruction throughput, so its
-      persistent kernel always runs 8 disparities per lane; what is tuned is
-      the launch schedule: resident workgroups per CU, how far behind the
-      aggregation each pair's sum + winner items are placed, and the pixels
-      per such item.  Timed on the batch itself, replayed from its hipGraph.
+  tune_batch(pairs)  a batch runs every pair's pipeline with its own
+      schedule over several streams at once; what is tuned is the number of
+      streams.  Timed on the batch itself, replayed from its hipGraph.
 
 Timings use HIP events on the launch stream; the median of `reps` replays
 after one warm-up wins.  Winners are cached in a JSON file keyed by form and
@@ -30,10 +28,8 @@ from vxsgm import hip
 
 CLASS_LAYOUTS = [[(32, 2), (16, 4), (8, 8)], [(64, 2), (32, 4), (16, 8)],
                  [(64, 4), (32, 8)]]
-# Persistent-launch schedules tried for a batch: (blocks_per_cu, wta_lag,
-# wta_pixels).
-BATCH_OPTIONS = [(2, 2, 1024), (1, 2, 1024), (2, 1, 1024), (2, 4, 1024), (2, 2, 4096),
-                 (2, 2, 256)]
+# Batch launch options tried: (streams,).
+BATCH_OPTIONS = [(1,), (2,), (3,), (4,), (6,)]
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 
@@ -96,15 +92,16 @@ class Tuner:
         return statistics.median(ms)
 
     def tune_batch(self, descs, key="batch", reps=3, force=False):
-        """Best launch schedule (blocks_per_cu, wta_lag, wta_pixels) for the
-        batch of pair descriptors `descs` (as for SGMBatch)."""
+        """Best launch options (streams,) for the batch of pair descriptors
+        `descs` (as for SGMBatch)."""
         if not force and key in self.table:
             return tuple(self.table[key]["options"])
         stream = hip.Stream()
         times = {}
         for opt in BATCH_OPTIONS:
             b = vxsgm.SGMBatch(descs, options=opt)
-            times["%d/%d/%d" % opt] = self._time(lambda: b.launch(stream.handle), stream, reps)
+            times["/".join(str(v) for v in opt)] = self._time(lambda: b.launch(stream.handle),
+                                                             stream, reps)
             b.close()
         best = min(times, key=times.get)
         opt = tuple(int(v) for v in best.split("/"))

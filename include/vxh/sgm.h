@@ -72,6 +72,8 @@ typedef struct vxh_context vxh_context;
  * VXH_ERR_SHADER_MISSING when the gfx950 code object lacks a kernel. */
 vxh_status vxh_context_create(int device, vxh_context **out);
 void vxh_context_destroy(vxh_context *ctx);
+/* The context's HIP device ordinal (-1 for NULL). */
+int vxh_context_device(const vxh_context *ctx);
 /* Number of HIP devices visible to the process (0 when none). */
 int vxh_device_count(void);
 
@@ -85,6 +87,12 @@ vxh_status vxh_texture_gray8(vxh_context *ctx, const uint8_t *pixels,
 /* Context::texture_output_r32float (context.rs:64-66, texture.rs:324-349). */
 vxh_status vxh_texture_output_r32float(vxh_context *ctx, uint32_t w,
                                        uint32_t h, vxh_texture **out);
+/* Context::texture_r32float -> Texture::from_r32float (context.rs:44-46,
+ * texture.rs:96-151): uploads w*h floats; len != w*h -> INVALID_CONFIG. */
+vxh_status vxh_texture_r32float(vxh_context *ctx, const float *data, size_t len,
+                                uint32_t w, uint32_t h, vxh_texture **out);
+/* Pixel format of a texture: 1 = gray8 (R8Unorm), 2 = R32Float, 0 = none. */
+uint32_t vxh_texture_format(const vxh_texture *tex);
 /* Texture::read_r32float (texture.rs:404-427): copies w*h floats to dst;
  * len != w*h or a non-R32Float texture -> INVALID_CONFIG. */
 vxh_status vxh_texture_read_r32float(const vxh_texture *tex, float *dst,

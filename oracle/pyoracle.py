@@ -50,6 +50,44 @@ _sig("oracle_hazard_dir", None, _vp, _vp, _u32, _u32, _u32, _u32, _u32,
      ctypes.c_int, ctypes.c_int, ctypes.POINTER(HazardStats))
 
 
+_sig("oracle_depth_bilateral", None, _vp, _u32, _u32, ctypes.c_int, ctypes.c_float,
+     ctypes.c_float, _vp)
+_sig("oracle_depth_median", None, _vp, _u32, _u32, ctypes.c_int, _vp)
+_sig("oracle_disparity_to_depth", None, _vp, ctypes.c_size_t, ctypes.c_float, ctypes.c_float,
+     _vp)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def depth_bilateral(depth, radius=3, sigma_spatial=5.0, sigma_depth=0.05):
+    """DepthFilter::apply_bilateral (DepthFilter.metal:17-60) on an (h, w) map."""
+    d = _f32(depth)
+    h, w = d.shape
+    out = np.empty_like(d)
+    lib.oracle_depth_bilateral(d.ctypes.data, w, h, radius, sigma_spatial, sigma_depth,
+                               out.ctypes.data)
+    return out
+
+
+def depth_median(depth, radius=1):
+    """DepthFilter::apply_median (DepthFilter.metal:66-120) on an (h, w) map."""
+    d = _f32(depth)
+    h, w = d.shape
+    out = np.empty_like(d)
+    lib.oracle_depth_median(d.ctypes.data, w, h, radius, out.ctypes.data)
+    return out
+
+
+def disparity_to_depth(disp, fx, baseline):
+    """(fx * baseline) / d for d > 0, else 0 (sgm.rs:284-286)."""
+    d = _f32(disp)
+    out = np.empty_like(d)
+    lib.oracle_disparity_to_depth(d.ctypes.data, d.size, fx, baseline, out.ctypes.data)
+    return out
+
+
 def _u8(a):
     return np.ascontiguousarray(a, dtype=np.uint8)
 

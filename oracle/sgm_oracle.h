@@ -16,6 +16,7 @@
 #ifndef VX_SGM_ORACLE_H
 #define VX_SGM_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -90,6 +91,13 @@ typedef struct {
 void oracle_hazard_dir(const uint32_t *lc, const uint32_t *rc, uint32_t w,
                        uint32_t h, uint32_t nd, uint32_t p1, uint32_t p2,
                        int dx, int dy, oracle_hazard_stats *st);
+
+/* Depth filters and disparity-to-depth (depth_oracle.cpp). */
+void oracle_depth_bilateral(const float *in, uint32_t w, uint32_t h, int radius, float ss,
+                            float sd, float *out);
+void oracle_depth_median(const float *in, uint32_t w, uint32_t h, int radius, float *out);
+void oracle_disparity_to_depth(const float *disp, size_t n, float fx, float baseline,
+                               float *out);
 
 #ifdef __cplusplus
 }

@@ -9,7 +9,12 @@
 #include "sgm_internal.h"
 #include "vxh/sgm.h"
 
+#include <string>
+
 namespace vxh {
+
+// Sets the calling thread's vxh_last_error() message; returns st.
+vxh_status report_error(vxh_status st, const std::string &msg);
 
 // Enqueues the pipeline of one pair on `st`.  The config must be valid for
 // (w, h) and `ws` must hold workspace_layout(w, h, D, num_paths, mode,

@@ -209,6 +209,8 @@ vxh_status vxh_context_create(int device, vxh_context **out) {
 
 void vxh_context_destroy(vxh_context *ctx) { delete ctx; }
 
+int vxh_context_device(const vxh_context *ctx) { return ctx ? ctx->device : -1; }
+
 static vxh_status tex_alloc(vxh_context *ctx, uint32_t w, uint32_t h, int fmt,
                             vxh_texture **out) {
     if (!ctx || !out) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null argument");
@@ -249,6 +251,29 @@ vxh_status vxh_texture_output_r32float(vxh_context *ctx, uint32_t w, uint32_t h,
                                        vxh_texture **out) {
     return tex_alloc(ctx, w, h, kR32F, out);
 }
+
+vxh_status vxh_texture_r32float(vxh_context *ctx, const float *data, size_t len, uint32_t w,
+                                uint32_t h, vxh_texture **out) {
+    if (out) *out = nullptr;
+    if (!data || len != (size_t)w * h) {
+        char buf[160];
+        snprintf(buf, sizeof buf,
+                 "invalid config: expected %zu floats for %ux%u R32Float texture, got %zu",
+                 (size_t)w * h, w, h, len);
+        return fail(VXH_ERR_INVALID_CONFIG, buf);
+    }
+    vxh_status st = tex_alloc(ctx, w, h, kR32F, out);
+    if (st != VXH_OK) return st;
+    hipError_t e = hipMemcpy((*out)->dptr, data, len * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        vxh_texture_destroy(*out);
+        *out = nullptr;
+        return hip_fail(e, "hipMemcpy");
+    }
+    return VXH_OK;
+}
+
+uint32_t vxh_texture_format(const vxh_texture *t) { return t ? (uint32_t)t->format : 0u; }
 
 vxh_status vxh_texture_write_gray8(vxh_texture *t, const uint8_t *pixels, size_t len) {
     if (!t || t->format != kGray8 || len != (size_t)t->w * t->h)
@@ -352,6 +377,8 @@ static vxh_status paths_and_wta(const uint8_t *cost, uint32_t w, uint32_t h,
 }  // extern "C"
 
 namespace vxh {
+
+vxh_status report_error(vxh_status st, const std::string &msg) { return fail(st, msg); }
 
 // The whole pipeline of one pair on `st` (inputs validated by the caller;
 // `ws` holds workspace_layout(w, h, D, paths, mode, lr) bytes).  With

@@ -71,6 +71,9 @@ _sig("vxh_context_create", ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp))
 _sig("vxh_context_destroy", None, _vp)
 _sig("vxh_texture_gray8", ctypes.c_int, _vp, _vp, _sz, _u32, _u32, ctypes.POINTER(_vp))
 _sig("vxh_texture_output_r32float", ctypes.c_int, _vp, _u32, _u32, ctypes.POINTER(_vp))
+_sig("vxh_texture_r32float", ctypes.c_int, _vp, _vp, _sz, _u32, _u32, ctypes.POINTER(_vp))
+_sig("vxh_texture_format", _u32, _vp)
+_sig("vxh_context_device", ctypes.c_int, _vp)
 _sig("vxh_texture_read_r32float", ctypes.c_int, _vp, _vp, _sz)
 _sig("vxh_texture_write_gray8", ctypes.c_int, _vp, _vp, _sz)
 _sig("vxh_texture_width", _u32, _vp)
@@ -243,6 +246,14 @@ class Context:
     def texture_output_r32float(self, width, height):
         h = _vp()
         check(lib.vxh_texture_output_r32float(self._h, width, height, ctypes.byref(h)))
+        return Texture(h.value)
+
+    def texture_r32float(self, data, width, height):
+        """Context::texture_r32float (vx-vision/src/context.rs:44-46)."""
+        d = np.ascontiguousarray(data, dtype=np.float32).ravel()
+        h = _vp()
+        check(lib.vxh_texture_r32float(self._h, d.ctypes.data, d.size, width, height,
+                                       ctypes.byref(h)))
         return Texture(h.value)
 
     def close(self):
@@ -541,6 +552,98 @@ class SGMServer:
             self.close()
         except Exception:
             pass
+
+
+class vxh_depth_filter_config(ctypes.Structure):
+    _fields_ = [("radius", ctypes.c_int32), ("sigma_spatial", ctypes.c_float),
+                ("sigma_depth", ctypes.c_float)]
+
+
+class vxh_depth_median_config(ctypes.Structure):
+    _fields_ = [("radius", ctypes.c_int32)]
+
+
+_sig("vxh_depth_filter_config_default", None, ctypes.POINTER(vxh_depth_filter_config))
+_sig("vxh_depth_median_config_default", None, ctypes.POINTER(vxh_depth_median_config))
+_sig("vxh_depth_filter_create", ctypes.c_int, _vp, ctypes.POINTER(_vp))
+_sig("vxh_depth_filter_destroy", None, _vp)
+_sig("vxh_depth_filter_apply_bilateral", ctypes.c_int, _vp, _vp, _vp, _vp,
+     ctypes.POINTER(vxh_depth_filter_config))
+_sig("vxh_depth_filter_apply_median", ctypes.c_int, _vp, _vp, _vp, _vp,
+     ctypes.POINTER(vxh_depth_median_config))
+_sig("vxh_depth_bilateral_enqueue", ctypes.c_int, _vp, _vp, _u32, _u32,
+     ctypes.POINTER(vxh_depth_filter_config), _vp)
+_sig("vxh_depth_median_enqueue", ctypes.c_int, _vp, _vp, _u32, _u32,
+     ctypes.POINTER(vxh_depth_median_config), _vp)
+_sig("vxh_disparity_to_depth", ctypes.c_int, _vp, _vp, _u32, _u32, ctypes.c_float,
+     ctypes.c_float, _vp)
+
+
+@dataclass
+class DepthFilterConfig:
+    """DepthFilterConfig (vx-vision/src/kernels/depth_filter.rs:22-50)."""
+    radius: int = 3
+    sigma_spatial: float = 5.0
+    sigma_depth: float = 0.05
+
+    def to_c(self):
+        return vxh_depth_filter_config(self.radius, self.sigma_spatial, self.sigma_depth)
+
+
+@dataclass
+class DepthMedianConfig:
+    """DepthMedianConfig (vx-vision/src/kernels/depth_filter.rs:54-63)."""
+    radius: int = 1
+
+    def to_c(self):
+        return vxh_depth_median_config(self.radius)
+
+
+class DepthFilter:
+    """DepthFilter (vx-vision/src/kernels/depth_filter.rs:66-249): bilateral
+    and median filters of R32Float depth maps."""
+
+    def __init__(self, ctx):
+        h = _vp()
+        check(lib.vxh_depth_filter_create(ctx._h, ctypes.byref(h)))
+        self._h = h
+
+    def apply_bilateral(self, ctx, input, output, config):
+        c = config.to_c()
+        check(lib.vxh_depth_filter_apply_bilateral(self._h, ctx._h, input._h, output._h,
+                                                   ctypes.byref(c)))
+
+    def apply_median(self, ctx, input, output, config):
+        c = config.to_c()
+        check(lib.vxh_depth_filter_apply_median(self._h, ctx._h, input._h, output._h,
+                                                ctypes.byref(c)))
+
+    @staticmethod
+    def encode_bilateral(d_in, d_out, width, height, config, stream=None):
+        c = config.to_c()
+        check(lib.vxh_depth_bilateral_enqueue(d_in, d_out, width, height, ctypes.byref(c),
+                                              stream))
+
+    @staticmethod
+    def encode_median(d_in, d_out, width, height, config, stream=None):
+        c = config.to_c()
+        check(lib.vxh_depth_median_enqueue(d_in, d_out, width, height, ctypes.byref(c), stream))
+
+    def close(self):
+        if self._h:
+            lib.vxh_depth_filter_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def disparity_to_depth(d_disp, d_depth, width, height, fx, baseline, stream=None):
+    """(fx * baseline) / d where d > 0, else 0, on device buffers."""
+    check(lib.vxh_disparity_to_depth(d_disp, d_depth, width, height, fx, baseline, stream))
 
 
 MIX_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080)]

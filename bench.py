@@ -3,8 +3,9 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms plain,batched,served]
 
 Headline (`value`): the plain per-pair path.  A step is one pass of the SGM
-pipeline (census of both images, path aggregation over every direction,
-winner-take-all) over one synthetic rectified pair already resident in HBM,
+pipeline (census of both images, matching costs, path aggregation over every
+direction, sum + winner-take-all) over one synthetic rectified pair already
+resident in HBM,
 on the workload 720p_d128_8path (1280x720, D=128, 8 paths, P1=10, P2=120,
 census 9x7).  value = W*H*D*K / t / 1e9 Gpx*disp/s over K timed steps.
 
@@ -15,14 +16,18 @@ inputs, the f32 output), split between its two kernels: the aggregation's
 share is W*H*(2*D + 2) (the summed volume written once, both inputs read), the
 winner-take-all's W*H*(2*D + 4).  The aggregation's share is divided by its
 average HIP-event time over the timed steps, recorded on the launch stream
-between the census and winner-take-all launches.  `cpu_baseline` times the
+between the cost and winner-take-all launches.  `traffic` is the HBM bytes
+per aggregation launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes of
+this command (scaled by a copy kernel's readings), taken from the committed
+summary profiles/paths_kernel_720p_d128_8path.md when its schedule is the
+run's, else null.  `cpu_baseline` times the
 scalar C++ oracle (the checker) on the same pair with the host's threads and
 confirms that the timed GPU map is bit-identical to it.
 
 `batched`: the 64-pair mixture of sizes {320x240, 640x480, 1280x720,
-1920x1080} x D {64, 128, 256}, 8 paths, as one persistent launch replayed from
-a hipGraph, against a loop of plain per-pair calls; all maps compared bit for
-bit.  `served`: Poisson arrivals at 200/500/1000 req/s of the same mixture
+1920x1080} x D {64, 128, 256}, 8 paths, as per-pair pipelines over several
+streams replayed from one hipGraph, against a loop of plain per-pair calls on
+one stream; all maps compared bit for bit.  `served`: Poisson arrivals at 200/500/1000 req/s of the same mixture
 against the serving loop and against a FIFO queue into the plain path;
 p50/p99 latency and Gpx*disp/s for each.
 """
@@ -67,6 +72,26 @@ def parse():
                     help="streams of the batched form (default: tuned, or the built-in "
                          "default with --no-tune)")
     return ap.parse_args()
+
+
+def sched_label(sched, split):
+    (a, b), (c, d) = sched
+    return f"{a}x{b}/{c}x{d}" + ("/split" if split else "")
+
+
+def committed_traffic(schedule, workload):
+    """HBM bytes per aggregation launch from the committed rocprofv3 summary
+    of this workload, if it was measured with the same schedule."""
+    path = os.path.join(ROOT, "profiles", f"paths_kernel_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    got, sched = None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            got = int(line.split(":")[1].split()[0])
+        if line.startswith("schedule:"):
+            sched = line.split(":", 1)[1].strip()
+    return got if sched == schedule else None
 
 
 def log(msg):
@@ -254,7 +279,8 @@ def main():
     stage = [sum(m[i].elapsed_ms(m[i + 1]) for m in marks) / args.steps for i in range(3)]
     ms_per_step = wall * 1e3 / args.steps
     gpu_disp = d_out.download(np.float32).reshape(h, w)
-    stages = {"census_x2": stage[0], "aggregate_all_paths": stage[1], "sum_wta": stage[2]}
+    stages = {"census_and_cost": stage[0], "aggregate_all_paths": stage[1],
+              "sum_wta": stage[2]}
     log(f"plain: {ms_per_step:.3f} ms/step, stages {stages}")
 
     pxd = w * h * D
@@ -283,7 +309,9 @@ def main():
                    "volume_format": args.volume_format},
         "roofline": {"bound": "hbm", "kernel": "paths_kernel (aggregation, all directions)",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": committed_traffic(sched_label(plain_sched, plain_split),
+                                                  args.workload),
                      "alg_bytes_per_launch": agg_bytes,
                      "launch_ms_event": round(stage[1], 4),
                      "pipeline_alg_bytes": alg_bytes,

@@ -206,9 +206,12 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
         }
         pix0[hh] = sy * w + sx[hh];
     }
-    int maxlen = max(len[0], len[1]);
+    int maxlen = max(len[0], len[1]), minlen = min(len[0], len[1]);
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, o));
+    for (int o = 32; o >= 1; o >>= 1) {
+        maxlen = max(maxlen, __shfl_xor(maxlen, o));
+        minlen = min(minlen, __shfl_xor(minlen, o));
+    }
     if (maxlen == 0) return;
     const int pstep = dy * w + dx;
     const int mode = MODE >= 0 ? MODE : p.mode;
@@ -266,11 +269,13 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     // Steps from m that a tail publishes early (two rounds).
     constexpr int kRange = 2 * NB;
 
-    auto issue = [&](PairBuf<K> &b, int t) {
+    // FULL: step t lies inside every scanline of the wave (no clamp needed).
+    auto issue = [&](PairBuf<K> &b, int t, auto full) {
+        constexpr bool FULL = decltype(full)::value;
 #pragma unroll
         for (int hh = 0; hh < 2; ++hh) {
             // Inactive scanlines read pixel 0.
-            const uint32_t pix = t < len[hh] ? (uint32_t)(pix0[hh] + t * pstep) : 0u;
+            const uint32_t pix = FULL || t < len[hh] ? (uint32_t)(pix0[hh] + t * pstep) : 0u;
             const uint8_t *src = cost + (size_t)pix * (uint32_t)Dp;
             if constexpr (K == 2) b.c[hh][0] = *(const uint16_t *)src;
             else if constexpr (K == 4) b.c[hh][0] = *(const uint32_t *)src;
@@ -352,9 +357,12 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     };
 
     // VERIFY: a head's step at or after vfirst (compares with the tail).
-    auto step = [&](PairBuf<K> &b, TailBuf<K> &tb, int t, auto verify) {
+    // FULL: every scanline of the wave is active at step t and every lane
+    // stores (no tail warm-up), so no predicate depends on t.
+    auto step = [&](PairBuf<K> &b, TailBuf<K> &tb, int t, auto verify, auto full) {
         constexpr bool VERIFY = decltype(verify)::value;
-        const bool act0 = t < len[0], act1 = t < len[1];
+        constexpr bool FULL = decltype(full)::value;
+        const bool act0 = FULL || t < len[0], act1 = FULL || t < len[1];
         // Costs of both scanlines, packed: byte i of each slice segment into
         // the low (first scanline) and high (second) half.
         uint32_t C[K];
@@ -416,8 +424,8 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
 #pragma unroll
         for (int i = 0; i < K; ++i) prev[i] = agg[i];
 
-        bool st0 = act0 && lane_store, st1 = act1 && lane_store;
-        if constexpr (SPLIT) {
+        bool st0 = FULL || (act0 && lane_store), st1 = FULL || (act1 && lane_store);
+        if constexpr (SPLIT && !FULL) {
             // Tail warm-up steps store nothing (sfrom is 0 for other roles).
             st0 = st0 && t >= sfrom[0];
             st1 = st1 && t >= sfrom[1];
@@ -520,20 +528,34 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     const int rounds_main =
         SPLIT && role == kRoleHead ? min(rounds, __builtin_amdgcn_readfirstlane(vfirst) / NB)
                                    : rounds;
+    // Rounds whose steps and prefetches all lie inside every scanline of the
+    // wave run the FULL step (every lane of the wave stores when L * K <= Dp;
+    // a tail's warm-up does not store).
+    const bool full_ok = L * K <= Dp && !(SPLIT && role == kRoleTail);
+    const int full_rounds = full_ok ? min(rounds_main, max(0, (minlen - PF) / NB)) : 0;
     PairBuf<K> buf[NB];
     TailBuf<K> tbuf[SPLIT ? NB : 1];
 #pragma unroll
     for (int j = 0; j < PF; ++j) {
-        issue(buf[j], j);
+        issue(buf[j], j, std::false_type{});
         __builtin_amdgcn_sched_barrier(0);
     }
     int r = 0;
+    for (; r < full_rounds; ++r) {
+        const int t = r * NB;
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            issue(buf[(u + PF) % NB], t + u + PF, std::true_type{});
+            step(buf[u], tbuf[0], t + u, std::false_type{}, std::true_type{});
+        }
+        if (SPLIT && r == publish_round) publish();
+    }
     for (; r < rounds_main; ++r) {
         const int t = r * NB;
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
-            issue(buf[(u + PF) % NB], t + u + PF);
-            step(buf[u], tbuf[0], t + u, std::false_type{});
+            issue(buf[(u + PF) % NB], t + u + PF, std::false_type{});
+            step(buf[u], tbuf[0], t + u, std::false_type{}, std::false_type{});
         }
         if (SPLIT && r == publish_round) publish();
     }
@@ -546,9 +568,9 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
                 const int t = r * NB;
 #pragma unroll
                 for (int u = 0; u < NB; ++u) {
-                    issue(buf[(u + PF) % NB], t + u + PF);
+                    issue(buf[(u + PF) % NB], t + u + PF, std::false_type{});
                     issue_tail(tbuf[(u + PF) % NB], t + u + PF);
-                    step(buf[u], tbuf[u], t + u, std::true_type{});
+                    step(buf[u], tbuf[u], t + u, std::true_type{}, std::false_type{});
                 }
                 const int tn = t + NB;
                 const bool live = (tn < len[0] && !met[0]) || (tn < len[1] && !met[1]);

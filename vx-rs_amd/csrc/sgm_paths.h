@@ -532,7 +532,8 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     // wave run the FULL step (every lane of the wave stores when L * K <= Dp;
     // a tail's warm-up does not store).
     const bool full_ok = L * K <= Dp && !(SPLIT && role == kRoleTail);
-    const int full_rounds = full_ok ? min(rounds_main, max(0, (minlen - PF) / NB)) : 0;
+    const int full_rounds =
+        full_ok ? min(rounds_main, max(0, (__builtin_amdgcn_readfirstlane(minlen) - PF) / NB)) : 0;
     PairBuf<K> buf[NB];
     TailBuf<K> tbuf[SPLIT ? NB : 1];
 #pragma unroll

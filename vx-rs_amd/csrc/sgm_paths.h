@@ -235,9 +235,15 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     }
     // Aligned: a padding lane's minimum is masked once per step.
     const uint32_t lane_pad = lane_real ? 0u : 0xFFFFFFFFu;
-    uint32_t prev[K];
+    // prevq[i] = prev[i] + P1, kept from the chain of the step that produced
+    // prev (the d+1 term of the next step reads it, as the d-1 term of the
+    // chain did), so no step adds P1 to prev again.
+    uint32_t prev[K], prevq[K];
 #pragma unroll
-    for (int i = 0; i < K; ++i) prev[i] = 0;
+    for (int i = 0; i < K; ++i) {
+        prev[i] = 0;
+        prevq[i] = p1x2;
+    }
     uint32_t pm = 0;
 
     // Head role: the first step at which any scanline of the wave reads the
@@ -375,23 +381,26 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
         // below gives agg = C there (best = min(0, ...) = 0), the first
         // pixel's copy of the cost (SGMStereo.metal:152), so no step is
         // special-cased.
-        uint32_t agg[K];
+        uint32_t agg[K], q[K];   // q[i] = agg[i] + P1
         {
             const uint32_t o3 = pk_add(pm, p2x2);
-            const uint32_t nb = from_upper_lane<L>(prev[0]);
+            const uint32_t nbq = from_upper_lane<L>(prevq[0]);
             uint32_t A[K], B[K];
 #pragma unroll
             for (int i = 0; i < K; ++i) {
-                const uint32_t nx = (i + 1 < K) ? prev[(i + 1) % K] : nb;
-                uint32_t o2 = pk_add(nx, p1x2);
+                uint32_t o2 = (i + 1 < K) ? prevq[(i + 1) % K] : nbq;
                 if (!ALIGNED || i == K - 1) o2 |= om[i];
                 A[i] = pk_min(pk_min(prev[i], o2), o3);
                 B[i] = pk_sub(C[i], pm);
             }
             // Speculative pass: no carry-in from the lane below.
             agg[0] = pk_add(B[0], A[0]);
+            q[0] = pk_add(agg[0], p1x2);
 #pragma unroll
-            for (int i = 1; i < K; ++i) agg[i] = pk_add(B[i], pk_min(A[i], pk_add(agg[i - 1], p1x2)));
+            for (int i = 1; i < K; ++i) {
+                agg[i] = pk_add(B[i], pk_min(A[i], q[i - 1]));
+                q[i] = pk_add(agg[i], p1x2);
+            }
             // Fixed-point rounds over the carry-in.  Halves whose scanline has
             // ended, the group's first lane and padding lanes take no carry.
             const uint32_t take = (sub > 0 && lane_real)
@@ -399,16 +408,18 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
                                       : 0u;
             uint32_t e_used = A[0];
             for (int r = 0; r < L - 1; ++r) {
-                const uint32_t up = from_lower_lane<L>(agg[K - 1]);
-                const uint32_t cand = pk_min(A[0], pk_add(up, p1x2));
+                const uint32_t cand = pk_min(A[0], from_lower_lane<L>(q[K - 1]));
                 const uint32_t e = (cand & take) | (e_used & ~take);
                 if (lanes_ne(e, e_used) == 0) break;
                 if (e != e_used) {
                     e_used = e;
                     agg[0] = pk_add(B[0], e);
+                    q[0] = pk_add(agg[0], p1x2);
 #pragma unroll
-                    for (int i = 1; i < K; ++i)
-                        agg[i] = pk_add(B[i], pk_min(A[i], pk_add(agg[i - 1], p1x2)));
+                    for (int i = 1; i < K; ++i) {
+                        agg[i] = pk_add(B[i], pk_min(A[i], q[i - 1]));
+                        q[i] = pk_add(agg[i], p1x2);
+                    }
                 }
             }
         }
@@ -422,7 +433,10 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
         // before prev is overwritten.
         if (mode == kOutU8) flush_high(t - 1);
 #pragma unroll
-        for (int i = 0; i < K; ++i) prev[i] = agg[i];
+        for (int i = 0; i < K; ++i) {
+            prev[i] = agg[i];
+            prevq[i] = q[i];
+        }
 
         bool st0 = FULL || (act0 && lane_store), st1 = FULL || (act1 && lane_store);
         if constexpr (SPLIT && !FULL) {

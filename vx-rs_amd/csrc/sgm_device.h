@@ -191,74 +191,6 @@ __device__ __forceinline__ T *store_sink() {
     return (T *)&g_store_sink[(threadIdx.x & 63) * 2];
 }
 
-// Matching-cost volume tiles.  A wave computes the costs of kCostTile
-// consecutive pixels of one row for all Dp entries of their slices, one
-// 16-entry chunk per lane and iteration: the tile's slices are contiguous in
-// the volume, so every store instruction writes 1 KB in one piece.  The
-// right census words the tile reads are staged in LDS first; entry e sits in
-// slot e + 2 * (e / 16), so the lanes of one read (8 chunks of 16 entries
-// apart, of 2 neighbouring pixels per 16 lanes) hit distinct banks.
-constexpr int kCostTile = 64;
-constexpr int kCostSpan = kCostTile + 255;                   // entries (D <= 256)
-constexpr int kCostSlots = kCostSpan + 2 * ((kCostSpan + 15) / 16);
-__device__ __forceinline__ int cost_slot(int e) { return e + 2 * (e >> 4); }
-
-// Costs of pixels x0 .. x0 + 63 of row y into cost[(y*w + x)*Dp + d]: for
-// d < D the Hamming distance of the left census at x and the right census at
-// x - d, or 64 where x - d < 0 (SGMStereo.metal:133-142); 64 for the padding
-// entries d >= D.  `span` is this wave's LDS area of kCostSlots entries.
-// Wave-level: all 64 lanes call it with the same arguments.
-__device__ __forceinline__ void cost_tile(const uint2 *__restrict__ lc,
-                                          const uint2 *__restrict__ rc, int w, int D, int Dp,
-                                          int y, int x0, uint8_t *__restrict__ cost,
-                                          uint2 *span) {
-    const int lane = threadIdx.x & 63;
-    const int xs0 = max(0, x0 - (D - 1));
-    const int xs1 = min(w, x0 + kCostTile);
-    const uint2 *rrow = rc + (size_t)y * w + xs0;
-    for (int j = lane; j < xs1 - xs0; j += 64) span[cost_slot(j)] = rrow[j];
-    // The span's entries come from other lanes of this wave.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int nch = Dp >> 4;   // chunks per pixel
-    const int nchunks = min(kCostTile, w - x0) * nch;
-    const uint2 *__restrict__ lrow = lc + (size_t)y * w;
-    uint4 *dst = (uint4 *)(cost + ((size_t)y * w + x0) * Dp);
-    const bool inside = x0 >= D - 1;   // no pixel of the tile has x - d < 0 for d < D
-    for (int c = lane; c < nchunks; c += 64) {
-        const int px = c / nch;
-        const int dc = (c - px * nch) * 16;
-        const int x = x0 + px;
-        const int rel = x - xs0;   // span index of x - d is rel - d
-        const uint2 lv = lrow[x];
-        uint32_t wd[4];
-        auto chunk = [&](auto checked) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                uint32_t v = 0;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int d = dc + 4 * q + b;
-                    uint32_t cst;
-                    if constexpr (decltype(checked)::value) {
-                        const uint2 r = span[cost_slot(max(rel - d, 0))];
-                        cst = (d < D && d <= x) ? __popc(lv.x ^ r.x) + __popc(lv.y ^ r.y) : 64u;
-                    } else {
-                        const uint2 r = span[cost_slot(rel - d)];
-                        cst = __popc(lv.x ^ r.x) + __popc(lv.y ^ r.y);
-                    }
-                    v |= cst << (8 * b);
-                }
-                wd[q] = v;
-            }
-        };
-        if (inside && dc + 16 <= D) chunk(std::false_type{});
-        else chunk(std::true_type{});
-        dst[c] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
-    }
-}
-
 // Left-right check of pixel i: D_R(x') is the lowest d minimising
 // S(x'+d, y, d) over x'+d < w; the pixel is kept iff x - best >= 0 and
 // |best - D_R(x - best)| <= tol, otherwise it becomes -1.0f.

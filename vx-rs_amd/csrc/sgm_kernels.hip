@@ -22,17 +22,101 @@ census_kernel(const uint8_t *__restrict__ img, int w, int h, int rx, int ry,
     for (size_t k = i; k < nwords; k += (size_t)w * h) zero_words[k] = 0;
 }
 
-// One cost tile (64 pixels of a row) per wave.
+// Matching-cost volume of a pair from its census (SGMStereo.metal:133-142),
+// computed once and read by every direction pass.  A block (4 waves) owns
+// 256 pixels of one row: it stages the left census words of the tile and the
+// right census words the tile reads (the tile and the D - 1 pixels before
+// it) in LDS, then each wave computes the cost slices of its 64 pixels, one
+// pixel per lane (a lane reads 16 consecutive right-census words per chunk
+// at constant offsets), into an LDS tile that it copies out as contiguous
+// 16-byte chunks.  Slices wider than 128 entries go in two 128-entry passes,
+// each a whole 128-byte line per pixel.
+constexpr int kCcTile = 256;                      // pixels per block
+constexpr int kCcSeg = 128;                       // slice entries per pass
+constexpr int kCcOutRow = kCcSeg + 16;            // LDS row of one pixel (bank spread)
+constexpr int kCcSpanR = kCcTile + 256;           // right census words (D - 1 <= 255 before)
+constexpr size_t kCcLds = (size_t)(kCcSpanR + kCcTile) * 8 + 4 * 64 * kCcOutRow;
+
 __global__ void __launch_bounds__(256)
 cost_kernel(const uint2 *__restrict__ lc, const uint2 *__restrict__ rc, int w, int h, int D,
-            int Dp, int tiles_x, uint8_t *__restrict__ cost) {
-    __shared__ uint2 span[4][kCostSlots];
-    const int wv = threadIdx.x >> 6;
-    const int tile = blockIdx.x * 4 + wv;
-    if (tile >= tiles_x * h) return;   // wave-uniform
-    const int y = tile / tiles_x;
-    const int x0 = (tile - y * tiles_x) * kCostTile;
-    cost_tile(lc, rc, w, D, Dp, y, x0, cost, span[wv]);
+            int Dp, uint8_t *__restrict__ cost) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    uint2 *cen_r = (uint2 *)lds;                            // kCcSpanR words
+    uint2 *cen_l = cen_r + kCcSpanR;                        // kCcTile words
+    uint8_t *out_all = (uint8_t *)(cen_l + kCcTile);        // 4 x 64 x kCcOutRow, 16-aligned
+    const int t = threadIdx.x;
+    const int y = blockIdx.y;
+    const int x0 = blockIdx.x * kCcTile;
+    const int xs0 = max(0, x0 - (D - 1));                   // first right-census pixel
+    const int xs1 = min(w, x0 + kCcTile);
+    const int nr = xs1 - xs0, nl = xs1 - x0;
+    {
+        const size_t row = (size_t)y * w;
+        for (int i = t; i < nr + nl; i += 256) {
+            if (i < nr) cen_r[i] = rc[row + xs0 + i];
+            else cen_l[i - nr] = lc[row + x0 + i - nr];
+        }
+    }
+    __syncthreads();
+    const int wv = t >> 6, lane = t & 63;
+    const int p0 = wv * 64;                                 // first pixel of this wave
+    const int npx = min(64, nl - p0);
+    if (npx <= 0) return;                                   // wave-uniform
+    uint8_t *out = out_all + wv * 64 * kCcOutRow;
+    const int x = x0 + p0 + lane;
+    const bool live = lane < npx;
+    const uint2 lv = live ? cen_l[p0 + lane] : make_uint2(0, 0);
+    const int rel = x - xs0;                                // cen_r index of x - d is rel - d
+    const bool inside = x0 + p0 >= D - 1;                   // no lane has x - d < 0 for d < D
+    uint8_t *dst = cost + ((size_t)y * w + x0 + p0) * Dp;
+    for (int seg = 0; seg < Dp; seg += kCcSeg) {
+        const int segw = min(kCcSeg, Dp - seg);
+        if (live) {
+            for (int dc = 0; dc < segw; dc += 16) {
+                const int d0 = seg + dc;
+                uint32_t wd[4];
+                auto chunk = [&](auto checked) {
+                    // Words rel - d0 - 15 .. rel - d0, read at constant offsets.
+                    const uint2 *rb = cen_r + (decltype(checked)::value ? 0 : rel - d0 - 15);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        uint32_t v = 0;
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) {
+                            const int i = 4 * q + b, d = d0 + i;
+                            uint32_t c;
+                            if constexpr (decltype(checked)::value) {
+                                const uint2 r = cen_r[max(rel - d, 0)];
+                                c = (d < D && d <= x) ? __popc(lv.x ^ r.x) + __popc(lv.y ^ r.y)
+                                                      : 64u;
+                            } else {
+                                const uint2 r = rb[15 - i];
+                                c = __popc(lv.x ^ r.x) + __popc(lv.y ^ r.y);
+                            }
+                            v |= c << (8 * b);
+                        }
+                        wd[q] = v;
+                    }
+                };
+                if (inside && d0 + 16 <= D) chunk(std::false_type{});
+                else chunk(std::true_type{});
+                *(uint4 *)(out + lane * kCcOutRow + dc) = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+            }
+        }
+        // The tile's rows come from other lanes of this wave.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int nch = segw / 16;
+        for (int i = lane; i < npx * nch; i += 64) {
+            const int px = i / nch, c = i - px * nch;
+            *(uint4 *)(dst + (size_t)px * Dp + seg + c * 16) =
+                *(const uint4 *)(out + px * kCcOutRow + c * 16);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 }
 
 // Field `f` of pass i (i wave-uniform): unrolled selects, so the kernel
@@ -237,10 +321,9 @@ hipError_t launch_or_word(uint32_t *dst, const uint32_t *src, hipStream_t s) {
 
 hipError_t launch_cost(const uint2 *lc, const uint2 *rc, int w, int h, int D, uint8_t *cost,
                        hipStream_t s) {
-    const int tiles_x = (w + kCostTile - 1) / kCostTile;
-    const long tiles = (long)tiles_x * h;
-    hipLaunchKernelGGL(cost_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, lc, rc,
-                       w, h, D, volume_stride(D), tiles_x, cost);
+    const dim3 grid((unsigned)((w + kCcTile - 1) / kCcTile), (unsigned)h);
+    hipLaunchKernelGGL(cost_kernel, grid, dim3(256), kCcLds, s, lc, rc, w, h, D,
+                       volume_stride(D), cost);
     return hipGetLastError();
 }
 

@@ -91,7 +91,24 @@ vxh_status vxh_texture_output_r32float(vxh_context *ctx, uint32_t w,
  * texture.rs:96-151): uploads w*h floats; len != w*h -> INVALID_CONFIG. */
 vxh_status vxh_texture_r32float(vxh_context *ctx, const float *data, size_t len,
                                 uint32_t w, uint32_t h, vxh_texture **out);
-/* Pixel format of a texture: 1 = gray8 (R8Unorm), 2 = R32Float, 0 = none. */
+/* Context::texture_output_gray8 (context.rs:49-51, texture.rs:154-180) and
+ * texture_intermediate_gray8 (context.rs:69-71): an uninitialised gray8
+ * texture (every texture here is device memory, so both are the same). */
+vxh_status vxh_texture_output_gray8(vxh_context *ctx, uint32_t w, uint32_t h,
+                                    vxh_texture **out);
+/* Context::texture_rgba8 -> Texture::from_rgba8 (context.rs:54-56,
+ * texture.rs:238-294): uploads w*h*4 bytes; len != w*h*4 -> INVALID_CONFIG. */
+vxh_status vxh_texture_rgba8(vxh_context *ctx, const uint8_t *pixels, size_t len,
+                             uint32_t w, uint32_t h, vxh_texture **out);
+/* Context::texture_output_rgba8 (context.rs:59-61, texture.rs:296-322). */
+vxh_status vxh_texture_output_rgba8(vxh_context *ctx, uint32_t w, uint32_t h,
+                                    vxh_texture **out);
+/* Texture::read_gray8 / read_rgba8 (texture.rs:352-402): copies w*h (gray8)
+ * or w*h*4 (RGBA8) bytes to dst; another length or format -> INVALID_CONFIG. */
+vxh_status vxh_texture_read_gray8(const vxh_texture *tex, uint8_t *dst, size_t len);
+vxh_status vxh_texture_read_rgba8(const vxh_texture *tex, uint8_t *dst, size_t len);
+/* Pixel format of a texture: 1 = gray8 (R8Unorm), 2 = R32Float,
+ * 3 = RGBA8Unorm, 0 = none. */
 uint32_t vxh_texture_format(const vxh_texture *tex);
 /* Texture::read_r32float (texture.rs:404-427): copies w*h floats to dst;
  * len != w*h or a non-R32Float texture -> INVALID_CONFIG. */
@@ -100,6 +117,8 @@ vxh_status vxh_texture_read_r32float(const vxh_texture *tex, float *dst,
 /* Replaces the bytes of an existing gray8 texture (same size). */
 vxh_status vxh_texture_write_gray8(vxh_texture *tex, const uint8_t *pixels,
                                    size_t len);
+/* HIP device ordinal the texture lives on (MTLTexture.device); -1 for NULL. */
+int vxh_texture_device(const vxh_texture *tex);
 uint32_t vxh_texture_width(const vxh_texture *tex);
 uint32_t vxh_texture_height(const vxh_texture *tex);
 /* Device address of the texture's pixels (row-major, w*h elements). */

@@ -47,7 +47,7 @@ def test_median_matches_oracle(shape, r):
     if r >= 5 and w * h > 4000:
         pytest.skip("large windows on the small maps only")
     d = noisy_map(w, h, 1000 + w + r)
-    got = run(vxsgm.DepthFilter.encode_median, d, vxsgm.DepthMedianConfig(r))
+    got = run(vxsgm.DepthFilter.enqueue_median, d, vxsgm.DepthMedianConfig(r))
     assert (bits(got) == bits(O.depth_median(d, r))).all()
 
 
@@ -57,7 +57,7 @@ def test_median_matches_oracle(shape, r):
 def test_bilateral_matches_oracle(shape, cfg):
     w, h = shape
     d = noisy_map(w, h, 2000 + w + cfg[0])
-    got = run(vxsgm.DepthFilter.encode_bilateral, d, vxsgm.DepthFilterConfig(*cfg))
+    got = run(vxsgm.DepthFilter.enqueue_bilateral, d, vxsgm.DepthFilterConfig(*cfg))
     want = O.depth_bilateral(d, *cfg)
     invalid = d <= 0   # a NaN centre is not skipped by the shader's test: it stays NaN
     assert (got[invalid] == 0).all() and (want[invalid] == 0).all()

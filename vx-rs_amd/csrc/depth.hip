@@ -303,4 +303,31 @@ vxh_status vxh_depth_filter_apply_median(vxh_depth_filter *f, vxh_context *ctx,
                                               vxh_texture_height(input), cfg, f->stream));
 }
 
+vxh_status vxh_depth_filter_encode_bilateral(vxh_depth_filter *f, vxh_pipeline *p,
+                                             const vxh_texture *input, vxh_texture *output,
+                                             const vxh_depth_filter_config *cfg) {
+    if (!f || !p) return vxh::report_error(VXH_ERR_INVALID_CONFIG, "invalid config: null handle");
+    vxh_status st = check_textures(input, output);
+    if (st != VXH_OK) return st;
+    hipStream_t s = nullptr;
+    if ((st = vxh::pipeline_stream(p, &s)) != VXH_OK) return st;
+    return vxh_depth_bilateral_enqueue((const float *)vxh_texture_device_ptr(input),
+                                       (float *)vxh_texture_device_ptr(output),
+                                       vxh_texture_width(input), vxh_texture_height(input), cfg,
+                                       s);
+}
+
+vxh_status vxh_depth_filter_encode_median(vxh_depth_filter *f, vxh_pipeline *p,
+                                          const vxh_texture *input, vxh_texture *output,
+                                          const vxh_depth_median_config *cfg) {
+    if (!f || !p) return vxh::report_error(VXH_ERR_INVALID_CONFIG, "invalid config: null handle");
+    vxh_status st = check_textures(input, output);
+    if (st != VXH_OK) return st;
+    hipStream_t s = nullptr;
+    if ((st = vxh::pipeline_stream(p, &s)) != VXH_OK) return st;
+    return vxh_depth_median_enqueue((const float *)vxh_texture_device_ptr(input),
+                                    (float *)vxh_texture_device_ptr(output),
+                                    vxh_texture_width(input), vxh_texture_height(input), cfg, s);
+}
+
 }  // extern "C"

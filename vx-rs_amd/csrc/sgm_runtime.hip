@@ -2,8 +2,11 @@
 // handle with its scratch-slot pool, and the stream-ordered pipeline.
 //
 // Reference counterparts:
-//   vxh_context      vx-vision/src/context.rs:17-36, vx-core/src/device.rs:11-56
-//   vxh_texture      vx-vision/src/texture.rs:29-462 (R8Unorm in, R32Float out)
+//   vxh_context      vx-vision/src/context.rs:17-36, vx-core/src/device.rs:11-56;
+//                    also owns the device pool (scratch blocks and streams)
+//                    that pipelines draw from
+//   vxh_texture      vx-vision/src/texture.rs:29-462 (R8Unorm, R32Float,
+//                    RGBA8Unorm)
 //   vxh_sgm          vx-vision/src/kernels/sgm.rs:61-288
 //   device buffers   vx-core/src/buffer.rs:14-122 (UnifiedBuffer/GpuGuard): here
 //                    explicit hipMalloc allocations owned by textures and by
@@ -13,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -28,6 +32,11 @@
 namespace {
 
 thread_local std::string g_last_error;
+
+// Pipelines between begin and commit, process-wide.  While one is open, a
+// blocking copy on the legacy null stream is refused by the runtime (it would
+// depend on the capturing stream), so texture copies take a pooled stream.
+std::atomic<int> g_open_captures{0};
 
 vxh_status fail(vxh_status st, const std::string &msg) {
     g_last_error = msg;
@@ -54,7 +63,9 @@ vxh_status hip_fail(hipError_t e, const char *what, size_t bytes = 0) {
     } while (0)
 
 
-enum TexFormat { kGray8 = 1, kR32F = 2 };
+enum TexFormat { kGray8 = 1, kR32F = 2, kRGBA8 = 3 };
+
+size_t texel_bytes(int fmt) { return fmt == kGray8 ? 1 : 4; }
 
 using vxh::WsLayout;
 using vxh::workspace_layout;
@@ -88,7 +99,16 @@ int layout_index(int D, uint32_t lanes, uint32_t per_lane) {
 }  // namespace
 
 struct vxh_context {
-    int device;
+    struct Block {
+        void *ptr;
+        size_t bytes;
+        bool busy;
+    };
+    int device = 0;
+    std::mutex mu;
+    std::vector<Block> blocks;
+    std::vector<hipStream_t> idle_streams;
+    std::vector<hipStream_t> all_streams;
 };
 
 struct vxh_texture {
@@ -108,6 +128,32 @@ struct vxh_sgm {
     bool sched_set[257] = {};
     uint8_t format[257] = {};        // per D: 0 auto, 1 u8, 2 u16
 };
+
+namespace vxh {
+
+void capture_opened() { g_open_captures.fetch_add(1); }
+void capture_closed() { g_open_captures.fetch_sub(1); }
+
+}  // namespace vxh
+
+// A blocking host<->device copy of texture bytes: hipMemcpy, or, while a
+// pipeline is being encoded, hipMemcpyWithStream on a stream of ctx's pool.
+// The latter is not ordered after work on the null stream.
+static vxh_status copy_blocking(vxh_context *ctx, void *dst, const void *src, size_t n,
+                                hipMemcpyKind kind) {
+    vxh::CaptureRelaxed relaxed;
+    VXH_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    if (g_open_captures.load() == 0) {
+        VXH_HIP(hipMemcpy(dst, src, n, kind), "hipMemcpy");
+        return VXH_OK;
+    }
+    hipStream_t st = nullptr;
+    vxh_status v = vxh::ctx_take_stream(ctx, &st);
+    if (v != VXH_OK) return v;
+    const hipError_t e = hipMemcpyWithStream(dst, src, n, kind, st);
+    vxh::ctx_give_stream(ctx, st);
+    return e == hipSuccess ? VXH_OK : hip_fail(e, "hipMemcpyWithStream");
+}
 
 static vxh::Schedule schedule_for(vxh_sgm *s, int D) {
     std::lock_guard<std::mutex> g(s->mu);
@@ -203,11 +249,58 @@ vxh_status vxh_context_create(int device, vxh_context **out) {
     if (device < 0 || device >= n)
         return fail(VXH_ERR_DEVICE_NOT_FOUND, "no HIP device found");
     VXH_HIP(hipSetDevice(device), "hipSetDevice");
-    *out = new vxh_context{device};
+    vxh_context *c = new vxh_context();
+    c->device = device;
+    *out = c;
     return VXH_OK;
 }
 
-void vxh_context_destroy(vxh_context *ctx) { delete ctx; }
+void vxh_context_destroy(vxh_context *ctx) {
+    if (!ctx) return;
+    vxh::CaptureRelaxed relaxed;
+    (void)hipSetDevice(ctx->device);
+    for (hipStream_t st : ctx->all_streams) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    for (const vxh_context::Block &b : ctx->blocks) (void)hipFree(b.ptr);
+    delete ctx;
+}
+
+vxh_status vxh_context_pool_stats(vxh_context *ctx, size_t *out3) {
+    if (!ctx || !out3) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null argument");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    size_t held = 0, used = 0;
+    for (const vxh_context::Block &b : ctx->blocks) {
+        held += b.bytes;
+        if (b.busy) used += b.bytes;
+    }
+    out3[0] = ctx->blocks.size();
+    out3[1] = held;
+    out3[2] = used;
+    return VXH_OK;
+}
+
+void vxh_context_pool_trim(vxh_context *ctx) {
+    if (!ctx) return;
+    vxh::CaptureRelaxed relaxed;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    std::vector<vxh_context::Block> keep;
+    for (const vxh_context::Block &b : ctx->blocks) {
+        if (b.busy)
+            keep.push_back(b);
+        else
+            (void)hipFree(b.ptr);
+    }
+    ctx->blocks.swap(keep);
+    for (hipStream_t st : ctx->idle_streams) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+        ctx->all_streams.erase(std::find(ctx->all_streams.begin(), ctx->all_streams.end(), st));
+    }
+    ctx->idle_streams.clear();
+}
 
 int vxh_context_device(const vxh_context *ctx) { return ctx ? ctx->device : -1; }
 
@@ -217,7 +310,8 @@ static vxh_status tex_alloc(vxh_context *ctx, uint32_t w, uint32_t h, int fmt,
     *out = nullptr;
     if (w == 0 || h == 0)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: empty texture");
-    size_t bytes = (size_t)w * h * (fmt == kGray8 ? 1 : 4);
+    size_t bytes = (size_t)w * h * texel_bytes(fmt);
+    vxh::CaptureRelaxed relaxed;
     VXH_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
@@ -226,25 +320,46 @@ static vxh_status tex_alloc(vxh_context *ctx, uint32_t w, uint32_t h, int fmt,
     return VXH_OK;
 }
 
-vxh_status vxh_texture_gray8(vxh_context *ctx, const uint8_t *pixels, size_t len,
-                             uint32_t w, uint32_t h, vxh_texture **out) {
+// Texture::from_gray8 / from_rgba8 (texture.rs:38-93, :238-294): a new
+// texture of `fmt` holding `len` bytes of host pixels.
+static vxh_status tex_upload(vxh_context *ctx, const uint8_t *pixels, size_t len, uint32_t w,
+                             uint32_t h, int fmt, const char *name, vxh_texture **out) {
     if (out) *out = nullptr;
-    if (len != (size_t)w * h) {
+    const size_t want = (size_t)w * h * texel_bytes(fmt);
+    if (!pixels || len != want) {
         char buf[160];
         snprintf(buf, sizeof buf,
-                 "invalid config: expected %zu bytes for %ux%u R8 texture, got %zu",
-                 (size_t)w * h, w, h, len);
+                 "invalid config: expected %zu bytes for %ux%u %s texture, got %zu", want, w,
+                 h, name, len);
         return fail(VXH_ERR_INVALID_CONFIG, buf);
     }
-    vxh_status st = tex_alloc(ctx, w, h, kGray8, out);
+    vxh_status st = tex_alloc(ctx, w, h, fmt, out);
     if (st != VXH_OK) return st;
-    hipError_t e = hipMemcpy((*out)->dptr, pixels, len, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
+    if ((st = copy_blocking(ctx, (*out)->dptr, pixels, len, hipMemcpyHostToDevice)) != VXH_OK) {
         vxh_texture_destroy(*out);
         *out = nullptr;
-        return hip_fail(e, "hipMemcpy");
     }
-    return VXH_OK;
+    return st;
+}
+
+vxh_status vxh_texture_gray8(vxh_context *ctx, const uint8_t *pixels, size_t len,
+                             uint32_t w, uint32_t h, vxh_texture **out) {
+    return tex_upload(ctx, pixels, len, w, h, kGray8, "R8", out);
+}
+
+vxh_status vxh_texture_rgba8(vxh_context *ctx, const uint8_t *pixels, size_t len,
+                             uint32_t w, uint32_t h, vxh_texture **out) {
+    return tex_upload(ctx, pixels, len, w, h, kRGBA8, "RGBA8", out);
+}
+
+vxh_status vxh_texture_output_gray8(vxh_context *ctx, uint32_t w, uint32_t h,
+                                    vxh_texture **out) {
+    return tex_alloc(ctx, w, h, kGray8, out);
+}
+
+vxh_status vxh_texture_output_rgba8(vxh_context *ctx, uint32_t w, uint32_t h,
+                                    vxh_texture **out) {
+    return tex_alloc(ctx, w, h, kRGBA8, out);
 }
 
 vxh_status vxh_texture_output_r32float(vxh_context *ctx, uint32_t w, uint32_t h,
@@ -264,13 +379,12 @@ vxh_status vxh_texture_r32float(vxh_context *ctx, const float *data, size_t len,
     }
     vxh_status st = tex_alloc(ctx, w, h, kR32F, out);
     if (st != VXH_OK) return st;
-    hipError_t e = hipMemcpy((*out)->dptr, data, len * 4, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
+    if ((st = copy_blocking(ctx, (*out)->dptr, data, len * 4, hipMemcpyHostToDevice)) !=
+        VXH_OK) {
         vxh_texture_destroy(*out);
         *out = nullptr;
-        return hip_fail(e, "hipMemcpy");
     }
-    return VXH_OK;
+    return st;
 }
 
 uint32_t vxh_texture_format(const vxh_texture *t) { return t ? (uint32_t)t->format : 0u; }
@@ -278,9 +392,7 @@ uint32_t vxh_texture_format(const vxh_texture *t) { return t ? (uint32_t)t->form
 vxh_status vxh_texture_write_gray8(vxh_texture *t, const uint8_t *pixels, size_t len) {
     if (!t || t->format != kGray8 || len != (size_t)t->w * t->h)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: gray8 write size");
-    VXH_HIP(hipSetDevice(t->ctx->device), "hipSetDevice");
-    VXH_HIP(hipMemcpy(t->dptr, pixels, len, hipMemcpyHostToDevice), "hipMemcpy");
-    return VXH_OK;
+    return copy_blocking(t->ctx, t->dptr, pixels, len, hipMemcpyHostToDevice);
 }
 
 vxh_status vxh_texture_read_r32float(const vxh_texture *t, float *dst, size_t len) {
@@ -288,17 +400,33 @@ vxh_status vxh_texture_read_r32float(const vxh_texture *t, float *dst, size_t le
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: not an R32Float texture");
     if (len != (size_t)t->w * t->h)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: read length");
-    VXH_HIP(hipSetDevice(t->ctx->device), "hipSetDevice");
-    VXH_HIP(hipMemcpy(dst, t->dptr, len * 4, hipMemcpyDeviceToHost), "hipMemcpy");
-    return VXH_OK;
+    return copy_blocking(t->ctx, dst, t->dptr, len * 4, hipMemcpyDeviceToHost);
 }
 
+// Texture::read_gray8 / read_rgba8 (texture.rs:352-402).
+static vxh_status tex_read_bytes(const vxh_texture *t, int fmt, uint8_t *dst, size_t len) {
+    if (!t || t->format != fmt || !dst)
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: texture format");
+    if (len != t->bytes) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: read length");
+    return copy_blocking(t->ctx, dst, t->dptr, len, hipMemcpyDeviceToHost);
+}
+
+vxh_status vxh_texture_read_gray8(const vxh_texture *t, uint8_t *dst, size_t len) {
+    return tex_read_bytes(t, kGray8, dst, len);
+}
+
+vxh_status vxh_texture_read_rgba8(const vxh_texture *t, uint8_t *dst, size_t len) {
+    return tex_read_bytes(t, kRGBA8, dst, len);
+}
+
+int vxh_texture_device(const vxh_texture *t) { return t ? t->ctx->device : -1; }
 uint32_t vxh_texture_width(const vxh_texture *t) { return t ? t->w : 0; }
 uint32_t vxh_texture_height(const vxh_texture *t) { return t ? t->h : 0; }
 void *vxh_texture_device_ptr(const vxh_texture *t) { return t ? t->dptr : nullptr; }
 
 void vxh_texture_destroy(vxh_texture *t) {
     if (!t) return;
+    vxh::CaptureRelaxed relaxed;
     (void)hipSetDevice(t->ctx->device);
     (void)hipFree(t->dptr);
     delete t;
@@ -379,6 +507,62 @@ static vxh_status paths_and_wta(const uint8_t *cost, uint32_t w, uint32_t h,
 namespace vxh {
 
 vxh_status report_error(vxh_status st, const std::string &msg) { return fail(st, msg); }
+
+vxh_status ctx_take_block(vxh_context *ctx, size_t bytes, void **ptr, size_t *got) {
+    bytes = (bytes + 4095) & ~(size_t)4095;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        vxh_context::Block *best = nullptr;
+        for (vxh_context::Block &b : ctx->blocks)
+            if (!b.busy && b.bytes >= bytes && (!best || b.bytes < best->bytes)) best = &b;
+        if (best) {
+            best->busy = true;
+            *ptr = best->ptr;
+            if (got) *got = best->bytes;
+            return VXH_OK;
+        }
+    }
+    CaptureRelaxed relaxed;
+    VXH_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    void *p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc", bytes);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->blocks.push_back(vxh_context::Block{p, bytes, true});
+    *ptr = p;
+    if (got) *got = bytes;
+    return VXH_OK;
+}
+
+void ctx_give_block(vxh_context *ctx, void *ptr) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    for (vxh_context::Block &b : ctx->blocks)
+        if (b.ptr == ptr) b.busy = false;
+}
+
+vxh_status ctx_take_stream(vxh_context *ctx, hipStream_t *st) {
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        if (!ctx->idle_streams.empty()) {
+            *st = ctx->idle_streams.back();
+            ctx->idle_streams.pop_back();
+            return VXH_OK;
+        }
+    }
+    CaptureRelaxed relaxed;
+    VXH_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t s = nullptr;
+    VXH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->all_streams.push_back(s);
+    *st = s;
+    return VXH_OK;
+}
+
+void ctx_give_stream(vxh_context *ctx, hipStream_t st) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->idle_streams.push_back(st);
+}
 
 // The whole pipeline of one pair on `st` (inputs validated by the caller;
 // `ws` holds workspace_layout(w, h, D, paths, mode, lr) bytes).  With
@@ -501,21 +685,56 @@ static void release_slot(vxh_sgm *s, Slot *sl) {
     s->free_slots.push_back(sl);
 }
 
-vxh_status vxh_sgm_compute_disparity(vxh_sgm *s, vxh_context *ctx,
-                                     const vxh_texture *left,
-                                     const vxh_texture *right, vxh_texture *out,
-                                     const vxh_sgm_config *c) {
+// Texture checks of the synchronous and the encode form: gray8 inputs and an
+// R32Float output of one size, else INVALID_CONFIG / TEXTURE_SIZE_MISMATCH
+// (the reference's compute_disparity, sgm.rs:108-119, checks none and takes
+// the sizes from the left texture).
+static vxh_status check_pair_textures(vxh_sgm *s, const vxh_texture *left,
+                                      const vxh_texture *right, const vxh_texture *out) {
     if (!s || !left || !right || !out)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null argument");
-    (void)ctx;
     if (left->format != kGray8 || right->format != kGray8 || out->format != kR32F)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: texture format");
     if (left->w != right->w || left->h != right->h || left->w != out->w ||
         left->h != out->h)
         return fail(VXH_ERR_TEXTURE_SIZE_MISMATCH, "texture size mismatch");
-    const uint32_t w = left->w, h = left->h;
-    vxh_status v = vxh_sgm_config_validate(c, w, h);
+    return VXH_OK;
+}
+
+vxh_status vxh_sgm_encode(vxh_sgm *s, vxh_pipeline *p, const vxh_texture *left,
+                          const vxh_texture *right, vxh_texture *out,
+                          const vxh_sgm_config *c) {
+    if (!p) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null pipeline");
+    vxh_status v = check_pair_textures(s, left, right, out);
     if (v != VXH_OK) return v;
+    const uint32_t w = left->w, h = left->h;
+    if ((v = vxh_sgm_config_validate(c, w, h)) != VXH_OK) return v;
+    hipStream_t st = nullptr;
+    if ((v = vxh::pipeline_stream(p, &st)) != VXH_OK) return v;
+    const int mode = mode_for(s, c);
+    const WsLayout L = workspace_layout(w, h, c->num_disparities, (int)c->num_paths, mode,
+                                        c->lr_check != 0);
+    void *ws = nullptr;
+    if ((v = vxh::pipeline_scratch(p, L.total, &ws)) != VXH_OK) return v;
+    v = vxh::run_pipeline((const uint8_t *)left->dptr, (const uint8_t *)right->dptr, w, h, c,
+                          (float *)out->dptr, (char *)ws, mode,
+                          schedule_for(s, (int)c->num_disparities), st);
+    if (v != VXH_OK) return v;
+    // The split launch's error word (a head whose wait for its tail timed out).
+    return vxh::pipeline_watch_word(p, (const uint32_t *)((char *)ws + L.sync) + 1);
+}
+
+vxh_status vxh_sgm_compute_disparity(vxh_sgm *s, vxh_context *ctx,
+                                     const vxh_texture *left,
+                                     const vxh_texture *right, vxh_texture *out,
+                                     const vxh_sgm_config *c) {
+    (void)ctx;
+    vxh_status v = check_pair_textures(s, left, right, out);
+    if (v != VXH_OK) return v;
+    const uint32_t w = left->w, h = left->h;
+    v = vxh_sgm_config_validate(c, w, h);
+    if (v != VXH_OK) return v;
+    vxh::CaptureRelaxed relaxed;   // callable while a pipeline is being encoded
     VXH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
     Slot *sl = nullptr;
     v = acquire_slot(s, workspace_total(w, h, c), &sl);
@@ -528,7 +747,8 @@ vxh_status vxh_sgm_compute_disparity(vxh_sgm *s, vxh_context *ctx,
     if (v == VXH_OK && e == hipSuccess) {
         const WsLayout L = workspace_layout(w, h, c->num_disparities, (int)c->num_paths,
                                             mode_for(s, c), c->lr_check != 0);
-        e = hipMemcpy(&err, (char *)sl->ws + L.sync + 4, 4, hipMemcpyDeviceToHost);
+        e = hipMemcpyWithStream(&err, (char *)sl->ws + L.sync + 4, 4, hipMemcpyDeviceToHost,
+                                sl->stream);
     }
     release_slot(s, sl);
     if (v != VXH_OK) return v;

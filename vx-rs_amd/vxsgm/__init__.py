@@ -73,6 +73,12 @@ _sig("vxh_texture_gray8", ctypes.c_int, _vp, _vp, _sz, _u32, _u32, ctypes.POINTE
 _sig("vxh_texture_output_r32float", ctypes.c_int, _vp, _u32, _u32, ctypes.POINTER(_vp))
 _sig("vxh_texture_r32float", ctypes.c_int, _vp, _vp, _sz, _u32, _u32, ctypes.POINTER(_vp))
 _sig("vxh_texture_format", _u32, _vp)
+_sig("vxh_texture_output_gray8", ctypes.c_int, _vp, _u32, _u32, ctypes.POINTER(_vp))
+_sig("vxh_texture_rgba8", ctypes.c_int, _vp, _vp, _sz, _u32, _u32, ctypes.POINTER(_vp))
+_sig("vxh_texture_output_rgba8", ctypes.c_int, _vp, _u32, _u32, ctypes.POINTER(_vp))
+_sig("vxh_texture_read_gray8", ctypes.c_int, _vp, _vp, _sz)
+_sig("vxh_texture_read_rgba8", ctypes.c_int, _vp, _vp, _sz)
+_sig("vxh_texture_device", ctypes.c_int, _vp)
 _sig("vxh_context_device", ctypes.c_int, _vp)
 _sig("vxh_texture_read_r32float", ctypes.c_int, _vp, _vp, _sz)
 _sig("vxh_texture_write_gray8", ctypes.c_int, _vp, _vp, _sz)
@@ -206,10 +212,34 @@ class Texture:
     def device_ptr(self):
         return lib.vxh_texture_device_ptr(self._h)
 
+    @property
+    def format(self):
+        """TextureFormat value: GRAY8 (1), R32FLOAT (2) or RGBA8 (3)."""
+        return lib.vxh_texture_format(self._h)
+
     def read_r32float(self):
         out = np.empty(self.width * self.height, dtype=np.float32)
         check(lib.vxh_texture_read_r32float(self._h, out.ctypes.data, out.size))
         return out
+
+    def read_gray8(self):
+        """Texture::read_gray8 (vx-vision/src/texture.rs:352-376)."""
+        out = np.empty(self.width * self.height, dtype=np.uint8)
+        check(lib.vxh_texture_read_gray8(self._h, out.ctypes.data, out.size))
+        return out
+
+    def read_rgba8(self):
+        """Texture::read_rgba8 (vx-vision/src/texture.rs:378-402)."""
+        out = np.empty(self.width * self.height * 4, dtype=np.uint8)
+        check(lib.vxh_texture_read_rgba8(self._h, out.ctypes.data, out.size))
+        return out
+
+    def _release(self):
+        """Gives up ownership of the handle (moved into a pipeline or a pool)."""
+        h, self._h = self._h, _vp()
+        if not h:
+            raise InvalidConfig("invalid config: texture already moved")
+        return h
 
     def write_gray8(self, pixels):
         p = _as_u8(pixels).ravel()
@@ -247,6 +277,41 @@ class Context:
         h = _vp()
         check(lib.vxh_texture_output_r32float(self._h, width, height, ctypes.byref(h)))
         return Texture(h.value)
+
+    def texture_output_gray8(self, width, height):
+        """Context::texture_output_gray8 (vx-vision/src/context.rs:49-51)."""
+        h = _vp()
+        check(lib.vxh_texture_output_gray8(self._h, width, height, ctypes.byref(h)))
+        return Texture(h.value)
+
+    # Context::texture_intermediate_* (context.rs:69-76): device memory either way.
+    texture_intermediate_gray8 = texture_output_gray8
+
+    def texture_intermediate_r32float(self, width, height):
+        return self.texture_output_r32float(width, height)
+
+    def texture_rgba8(self, pixels, width, height):
+        """Context::texture_rgba8 (vx-vision/src/context.rs:54-56)."""
+        p = _as_u8(pixels).ravel()
+        h = _vp()
+        check(lib.vxh_texture_rgba8(self._h, p.ctypes.data, p.size, width, height,
+                                    ctypes.byref(h)))
+        return Texture(h.value)
+
+    def texture_output_rgba8(self, width, height):
+        """Context::texture_output_rgba8 (vx-vision/src/context.rs:59-61)."""
+        h = _vp()
+        check(lib.vxh_texture_output_rgba8(self._h, width, height, ctypes.byref(h)))
+        return Texture(h.value)
+
+    def pool_stats(self):
+        """(blocks, bytes held, bytes in use) of the context's scratch pool."""
+        out = (_sz * 3)()
+        check(lib.vxh_context_pool_stats(self._h, out))
+        return tuple(out)
+
+    def pool_trim(self):
+        lib.vxh_context_pool_trim(self._h)
 
     def texture_r32float(self, data, width, height):
         """Context::texture_r32float (vx-vision/src/context.rs:44-46)."""
@@ -293,6 +358,13 @@ class SGMStereo:
         c = config.to_c()
         check(lib.vxh_sgm_compute_disparity(self._h, ctx._h, left._h, right._h,
                                             output._h, ctypes.byref(c)))
+
+    def encode(self, pipe, left, right, output, config):
+        """The encode() form of compute_disparity: recorded into a Pipeline,
+        run on its commit, same result."""
+        c = config.to_c()
+        check(lib.vxh_sgm_encode(self._h, pipe._h, left._h, right._h, output._h,
+                                 ctypes.byref(c)))
 
     def compute(self, ctx, left, right, config, intrinsics, baseline):
         c = config.to_c()
@@ -618,14 +690,28 @@ class DepthFilter:
         check(lib.vxh_depth_filter_apply_median(self._h, ctx._h, input._h, output._h,
                                                 ctypes.byref(c)))
 
+    def encode_bilateral(self, pipe, input, output, config):
+        """DepthFilter::encode_bilateral (depth_filter.rs:183-214) into a Pipeline."""
+        c = config.to_c()
+        check(lib.vxh_depth_filter_encode_bilateral(self._h, pipe._h, input._h, output._h,
+                                                    ctypes.byref(c)))
+
+    def encode_median(self, pipe, input, output, config):
+        """DepthFilter::encode_median (depth_filter.rs:217-246) into a Pipeline."""
+        c = config.to_c()
+        check(lib.vxh_depth_filter_encode_median(self._h, pipe._h, input._h, output._h,
+                                                 ctypes.byref(c)))
+
     @staticmethod
-    def encode_bilateral(d_in, d_out, width, height, config, stream=None):
+    def enqueue_bilateral(d_in, d_out, width, height, config, stream=None):
+        """Stream-ordered bilateral filter on device buffers of width*height floats."""
         c = config.to_c()
         check(lib.vxh_depth_bilateral_enqueue(d_in, d_out, width, height, ctypes.byref(c),
                                               stream))
 
     @staticmethod
-    def encode_median(d_in, d_out, width, height, config, stream=None):
+    def enqueue_median(d_in, d_out, width, height, config, stream=None):
+        """Stream-ordered median filter on device buffers of width*height floats."""
         c = config.to_c()
         check(lib.vxh_depth_median_enqueue(d_in, d_out, width, height, ctypes.byref(c), stream))
 
@@ -644,6 +730,177 @@ class DepthFilter:
 def disparity_to_depth(d_disp, d_depth, width, height, fx, baseline, stream=None):
     """(fx * baseline) / d where d > 0, else 0, on device buffers."""
     check(lib.vxh_disparity_to_depth(d_disp, d_depth, width, height, fx, baseline, stream))
+
+
+# ---- pipelines and texture pools (include/vxh/pipeline.h) ----
+_texp = ctypes.POINTER(_vp)
+_sig("vxh_pipeline_begin", ctypes.c_int, _vp, _texp)
+_sig("vxh_pipeline_stream", _vp, _vp)
+_sig("vxh_pipeline_retain", ctypes.c_int, _vp, _vp)
+_sig("vxh_pipeline_commit", ctypes.c_int, _vp)
+_sig("vxh_pipeline_wait", ctypes.c_int, _vp)
+_sig("vxh_pipeline_commit_and_wait", ctypes.c_int, _vp)
+_sig("vxh_pipeline_take_retained", _sz, _vp, _texp, _sz)
+_sig("vxh_pipeline_is_committed", ctypes.c_int, _vp)
+_sig("vxh_pipeline_node_count", _sz, _vp)
+_sig("vxh_pipeline_relaunch", ctypes.c_int, _vp)
+_sig("vxh_pipeline_destroy", None, _vp)
+_sig("vxh_sgm_encode", ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _cfgp)
+_sig("vxh_depth_filter_encode_bilateral", ctypes.c_int, _vp, _vp, _vp, _vp,
+     ctypes.POINTER(vxh_depth_filter_config))
+_sig("vxh_depth_filter_encode_median", ctypes.c_int, _vp, _vp, _vp, _vp,
+     ctypes.POINTER(vxh_depth_median_config))
+_sig("vxh_context_pool_stats", ctypes.c_int, _vp, ctypes.POINTER(_sz))
+_sig("vxh_context_pool_trim", None, _vp)
+_sig("vxh_texture_pool_create", ctypes.c_int, _sz, _texp)
+_sig("vxh_texture_pool_destroy", None, _vp)
+_sig("vxh_texture_pool_acquire", ctypes.c_int, _vp, _vp, _u32, _u32, _u32, _texp)
+_sig("vxh_texture_pool_release", None, _vp, _vp)
+_sig("vxh_texture_pool_clear", None, _vp)
+_sig("vxh_texture_pool_cached_count", _sz, _vp)
+_sig("vxh_texture_pool_total_acquires", ctypes.c_uint64, _vp)
+_sig("vxh_texture_pool_total_hits", ctypes.c_uint64, _vp)
+_sig("vxh_texture_pool_hit_rate", ctypes.c_double, _vp)
+
+
+class TextureFormat:
+    """TextureFormat (vx-vision/src/texture.rs:14-24)."""
+    R8UNORM = 1
+    R32FLOAT = 2
+    RGBA8UNORM = 3
+
+
+class Pipeline:
+    """Pipeline (vx-vision/src/pipeline.rs:12-80): kernels encoded into one
+    unit of GPU work, run on commit.  Encoding records into a HIP stream
+    capture; commit launches the captured graph."""
+
+    def __init__(self, handle):
+        self._h = _vp(handle)
+
+    @classmethod
+    def begin(cls, ctx):
+        h = _vp()
+        check(lib.vxh_pipeline_begin(ctx._h, ctypes.byref(h)))
+        return cls(h.value)
+
+    def cmd_buf(self):
+        """The capturing hipStream_t (None once committed)."""
+        return lib.vxh_pipeline_stream(self._h)
+
+    stream = property(cmd_buf)
+
+    def retain(self, tex):
+        """Moves `tex` into the pipeline until it completes (pipeline.rs:39-41)."""
+        h = tex._release()
+        st = lib.vxh_pipeline_retain(self._h, h)
+        if st != 0:
+            tex._h = _vp(h)
+        check(st)
+
+    def retain_all(self, textures):
+        for t in textures:
+            self.retain(t)
+
+    def commit(self):
+        check(lib.vxh_pipeline_commit(self._h))
+
+    def wait(self):
+        check(lib.vxh_pipeline_wait(self._h))
+
+    def commit_and_wait(self):
+        """Commits, waits, and returns the retained textures (pipeline.rs:49-54)."""
+        check(lib.vxh_pipeline_commit_and_wait(self._h))
+        return self.take_retained()
+
+    def take_retained(self):
+        n = lib.vxh_pipeline_take_retained(self._h, None, 0)
+        arr = (_vp * max(n, 1))()
+        got = lib.vxh_pipeline_take_retained(self._h, arr, n)
+        return [Texture(arr[i]) for i in range(got)]
+
+    def is_committed(self):
+        return bool(lib.vxh_pipeline_is_committed(self._h))
+
+    def node_count(self):
+        return lib.vxh_pipeline_node_count(self._h)
+
+    def relaunch(self):
+        """Runs the committed graph again on the textures' current contents."""
+        check(lib.vxh_pipeline_relaunch(self._h))
+
+    def close(self):
+        if self._h:
+            lib.vxh_pipeline_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TexturePool:
+    """TexturePool (vx-vision/src/pool.rs:17-147): textures cached per
+    (width, height, format), at most max_per_bucket of each."""
+
+    def __init__(self, max_per_bucket=8):
+        h = _vp()
+        check(lib.vxh_texture_pool_create(max_per_bucket, ctypes.byref(h)))
+        self._h = h
+
+    @classmethod
+    def with_capacity(cls, max_per_bucket):
+        return cls(max_per_bucket)
+
+    def acquire(self, ctx, width, height, fmt):
+        h = _vp()
+        check(lib.vxh_texture_pool_acquire(self._h, ctx._h, width, height, fmt, ctypes.byref(h)))
+        return Texture(h.value)
+
+    def acquire_gray8(self, ctx, width, height):
+        return self.acquire(ctx, width, height, TextureFormat.R8UNORM)
+
+    def acquire_r32float(self, ctx, width, height):
+        return self.acquire(ctx, width, height, TextureFormat.R32FLOAT)
+
+    def acquire_rgba8(self, ctx, width, height):
+        return self.acquire(ctx, width, height, TextureFormat.RGBA8UNORM)
+
+    def release(self, tex):
+        """Moves `tex` back into the pool (dropped if its bucket is full)."""
+        lib.vxh_texture_pool_release(self._h, tex._release())
+
+    def release_all(self, textures):
+        for t in textures:
+            self.release(t)
+
+    def clear(self):
+        lib.vxh_texture_pool_clear(self._h)
+
+    def cached_count(self):
+        return lib.vxh_texture_pool_cached_count(self._h)
+
+    def total_acquires(self):
+        return lib.vxh_texture_pool_total_acquires(self._h)
+
+    def total_hits(self):
+        return lib.vxh_texture_pool_total_hits(self._h)
+
+    def hit_rate(self):
+        return lib.vxh_texture_pool_hit_rate(self._h)
+
+    def close(self):
+        if self._h:
+            lib.vxh_texture_pool_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 MIX_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080)]

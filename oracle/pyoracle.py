@@ -56,9 +56,49 @@ _sig("oracle_depth_median", None, _vp, _u32, _u32, ctypes.c_int, _vp)
 _sig("oracle_disparity_to_depth", None, _vp, ctypes.c_size_t, ctypes.c_float, ctypes.c_float,
      _vp)
 
+_sig("oracle_remap", None, _vp, _u32, _u32, _u32, _vp, _vp, _vp, _u32, _u32, _u32)
+_sig("oracle_rectify_maps", ctypes.c_int, _vp, _u32, _u32, _vp, _vp, _vp)
+
 
 def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def remap(img, map_x, map_y, out_gray8=None):
+    """Undistorter remap (undistort.metal:4-20) of an (h_in, w_in) uint8 or
+    float32 image through (h, w) float32 maps.  The output is gray8 when
+    out_gray8 is True (default: same format as the input)."""
+    img = np.ascontiguousarray(img)
+    in_fmt = 1 if img.dtype == np.uint8 else 2
+    if in_fmt == 2:
+        img = _f32(img)
+    mx, my = _f32(map_x), _f32(map_y)
+    assert mx.shape == my.shape
+    h, w = mx.shape
+    gray_out = (in_fmt == 1) if out_gray8 is None else out_gray8
+    out = np.empty((h, w), np.uint8 if gray_out else np.float32)
+    lib.oracle_remap(img.ctypes.data, in_fmt, img.shape[1], img.shape[0], mx.ctypes.data,
+                     my.ctypes.data, out.ctypes.data, 1 if gray_out else 2, w, h)
+    return out
+
+
+RECTIFY_FIELDS = ("fx", "fy", "cx", "cy", "k1", "k2", "p1", "p2", "k3")
+
+
+def rectify_maps(params, w, h):
+    """Rectify maps (rectify_oracle.cpp) for params with the keys of
+    RECTIFY_FIELDS plus R (9 values) and new_fx, new_fy, new_cx, new_cy.
+    Returns (map_x, map_y, m) or raises ValueError for a singular Knew R."""
+    v = [float(params[k]) for k in RECTIFY_FIELDS] + [float(x) for x in params["R"]] + \
+        [float(params[k]) for k in ("new_fx", "new_fy", "new_cx", "new_cy")]
+    p = np.array(v, np.float64)
+    mx = np.empty((h, w), np.float32)
+    my = np.empty((h, w), np.float32)
+    m = np.empty(9, np.float32)
+    if lib.oracle_rectify_maps(p.ctypes.data, w, h, mx.ctypes.data, my.ctypes.data,
+                               m.ctypes.data) != 0:
+        raise ValueError("singular Knew * R")
+    return mx, my, m
 
 
 def depth_bilateral(depth, radius=3, sigma_spatial=5.0, sigma_depth=0.05):

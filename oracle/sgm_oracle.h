@@ -99,6 +99,15 @@ void oracle_depth_median(const float *in, uint32_t w, uint32_t h, int radius, fl
 void oracle_disparity_to_depth(const float *disp, size_t n, float fx, float baseline,
                                float *out);
 
+/* Undistorter remap (rectify_oracle.cpp): formats 1 = gray8, 2 = R32Float. */
+void oracle_remap(const void *in, uint32_t in_fmt, uint32_t in_w, uint32_t in_h,
+                  const float *map_x, const float *map_y, void *out, uint32_t out_fmt,
+                  uint32_t out_w, uint32_t out_h);
+/* Rectify maps from 22 doubles (fx fy cx cy k1 k2 p1 p2 k3 R[9] nfx nfy ncx
+ * ncy); m_out (may be NULL) receives the float inverse matrix.  0 or 6. */
+int oracle_rectify_maps(const double *p, uint32_t w, uint32_t h, float *map_x, float *map_y,
+                        float *m_out);
+
 #ifdef __cplusplus
 }
 #endif

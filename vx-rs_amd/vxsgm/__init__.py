@@ -903,6 +903,119 @@ class TexturePool:
             pass
 
 
+# ---- rectification (include/vxh/rectify.h) ----
+class vxh_rectify_params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("fx", "fy", "cx", "cy", "k1", "k2", "p1", "p2",
+                                                "k3")] + \
+               [("R", ctypes.c_double * 9)] + \
+               [(n, ctypes.c_double) for n in ("new_fx", "new_fy", "new_cx", "new_cy")]
+
+
+_rpp = ctypes.POINTER(vxh_rectify_params)
+_sig("vxh_undistorter_create", ctypes.c_int, _vp, _texp)
+_sig("vxh_undistorter_destroy", None, _vp)
+_sig("vxh_undistorter_apply", ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("vxh_undistorter_encode", ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("vxh_remap_enqueue", ctypes.c_int, _vp, _u32, _u32, _u32, _vp, _vp, _vp, _u32, _u32, _u32,
+     _vp)
+_sig("vxh_rectify_params_default", None, _rpp)
+_sig("vxh_rectify_inverse", ctypes.c_int, _rpp, ctypes.POINTER(ctypes.c_float))
+_sig("vxh_rectify_maps_enqueue", ctypes.c_int, _rpp, _u32, _u32, _vp, _vp, _vp)
+_sig("vxh_rectify_maps", ctypes.c_int, _vp, _rpp, _vp, _vp)
+
+
+@dataclass
+class RectifyParams:
+    """Camera model of the rectify maps: K and distortion of the input camera,
+    the rectifying rotation R (row-major) and the output projection."""
+    fx: float = 1.0
+    fy: float = 1.0
+    cx: float = 0.0
+    cy: float = 0.0
+    k1: float = 0.0
+    k2: float = 0.0
+    p1: float = 0.0
+    p2: float = 0.0
+    k3: float = 0.0
+    R: tuple = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+    new_fx: float = None
+    new_fy: float = None
+    new_cx: float = None
+    new_cy: float = None
+
+    def to_c(self):
+        c = vxh_rectify_params()
+        for n in ("fx", "fy", "cx", "cy", "k1", "k2", "p1", "p2", "k3"):
+            setattr(c, n, getattr(self, n))
+        c.R[:] = [float(v) for v in self.R]
+        for n, base in (("new_fx", "fx"), ("new_fy", "fy"), ("new_cx", "cx"), ("new_cy", "cy")):
+            v = getattr(self, n)
+            setattr(c, n, getattr(self, base) if v is None else v)
+        return c
+
+    def as_dict(self):
+        c = self.to_c()
+        d = {n: getattr(c, n) for n in ("fx", "fy", "cx", "cy", "k1", "k2", "p1", "p2", "k3",
+                                        "new_fx", "new_fy", "new_cx", "new_cy")}
+        d["R"] = list(c.R)
+        return d
+
+
+def rectify_inverse(params):
+    """inverse(Knew R) as the maps use it: 9 floats, row-major."""
+    c = params.to_c()
+    m = (ctypes.c_float * 9)()
+    check(lib.vxh_rectify_inverse(ctypes.byref(c), m))
+    return np.array(m, np.float32)
+
+
+def rectify_maps(ctx, params, map_x, map_y):
+    """Fills two R32Float textures with the rectify maps (synchronous)."""
+    c = params.to_c()
+    check(lib.vxh_rectify_maps(ctx._h, ctypes.byref(c), map_x._h, map_y._h))
+
+
+def rectify_maps_enqueue(params, width, height, d_map_x, d_map_y, stream=None):
+    c = params.to_c()
+    check(lib.vxh_rectify_maps_enqueue(ctypes.byref(c), width, height, d_map_x, d_map_y, stream))
+
+
+def remap_enqueue(d_in, in_format, in_w, in_h, d_map_x, d_map_y, d_out, out_format, out_w,
+                  out_h, stream=None):
+    """Stream-ordered remap on device buffers (formats 1 gray8, 2 R32Float)."""
+    check(lib.vxh_remap_enqueue(d_in, in_format, in_w, in_h, d_map_x, d_map_y, d_out,
+                                out_format, out_w, out_h, stream))
+
+
+class Undistorter:
+    """Undistorter (vx-vision/src/kernels/undistort.rs:14-137): remap through
+    per-pixel coordinate maps, bilinear, clamp to edge."""
+
+    def __init__(self, ctx):
+        h = _vp()
+        check(lib.vxh_undistorter_create(ctx._h, ctypes.byref(h)))
+        self._h = h
+
+    def apply(self, ctx, input, map_x, map_y, output):
+        check(lib.vxh_undistorter_apply(self._h, ctx._h, input._h, map_x._h, map_y._h,
+                                        output._h))
+
+    def encode(self, pipe, input, map_x, map_y, output):
+        check(lib.vxh_undistorter_encode(self._h, pipe._h, input._h, map_x._h, map_y._h,
+                                         output._h))
+
+    def close(self):
+        if self._h:
+            lib.vxh_undistorter_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 MIX_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080)]
 MIX_DISPARITIES = [64, 128, 256]
 

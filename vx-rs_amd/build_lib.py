@@ -17,7 +17,7 @@ OUT = os.path.join(HERE, "vxsgm", "libvxsgm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["sgm_kernels.hip", "sgm_runtime.hip", "sgm_batch.hip", "sgm_server.cpp",
-               "depth.hip", "pipeline.hip", "rectify.hip"]
+               "depth.hip", "pipeline.hip", "rectify.hip", "datasets.cpp"]
 CXX_SOURCES = ["synth.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I", os.path.join(ROOT, "include"),
@@ -46,7 +46,7 @@ def build(verbose=True):
         _run(["g++", *COMMON, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     tmp = OUT + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"])
     os.replace(tmp, OUT)
     return OUT
 

@@ -1016,6 +1016,131 @@ class Undistorter:
             pass
 
 
+# ---- datasets (include/vxh/datasets.h) ----
+class vxh_stereo_frame(ctypes.Structure):
+    _fields_ = [("timestamp", ctypes.c_double), ("left_path", ctypes.c_char_p),
+                ("right_path", ctypes.c_char_p), ("has_pose", ctypes.c_int32),
+                ("rotation", ctypes.c_float * 9), ("translation", ctypes.c_float * 3)]
+
+
+class vxh_stereo_calib(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("fx", "fy", "cx", "cy", "baseline")] + \
+               [("valid", ctypes.c_int32)]
+
+
+_sig("vxh_kitti_stereo_load", ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, _texp)
+_sig("vxh_kitti_stereo_destroy", None, _vp)
+_sig("vxh_kitti_stereo_len", _sz, _vp)
+_sig("vxh_kitti_stereo_is_color", ctypes.c_int32, _vp)
+_sig("vxh_kitti_stereo_frame", ctypes.c_int, _vp, _sz, ctypes.POINTER(vxh_stereo_frame))
+_sig("vxh_kitti_stereo_calib", ctypes.c_int, _vp, ctypes.POINTER(vxh_stereo_calib))
+_sig("vxh_kitti_stereo_read_pair", ctypes.c_int, _vp, _sz, _vp, _vp, _sz)
+_sig("vxh_png_info", ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(_u32), ctypes.POINTER(_u32),
+     ctypes.POINTER(_u32))
+_sig("vxh_png_read", ctypes.c_int, ctypes.c_char_p, _vp, _sz)
+_sig("vxh_png_read_luma8", ctypes.c_int, ctypes.c_char_p, _vp, _sz)
+
+
+def _path(p):
+    return os.fsencode(p)
+
+
+def png_info(path):
+    """(width, height, channels) of an 8-bit PNG."""
+    w, h, c = _u32(), _u32(), _u32()
+    check(lib.vxh_png_info(_path(path), ctypes.byref(w), ctypes.byref(h), ctypes.byref(c)))
+    return w.value, h.value, c.value
+
+
+def png_read(path):
+    """Decoded samples as an (h, w) or (h, w, channels) uint8 array."""
+    w, h, c = png_info(path)
+    out = np.empty((h, w, c) if c > 1 else (h, w), np.uint8)
+    check(lib.vxh_png_read(_path(path), out.ctypes.data, out.size))
+    return out
+
+
+def png_read_luma8(path):
+    """DynamicImage::to_luma8 of the file, as the reference's examples load frames."""
+    w, h, _ = png_info(path)
+    out = np.empty((h, w), np.uint8)
+    check(lib.vxh_png_read_luma8(_path(path), out.ctypes.data, out.size))
+    return out
+
+
+@dataclass
+class StereoFrame:
+    """DatasetFrame (vx-vision/src/datasets/mod.rs:12-24) for a camera pair."""
+    timestamp: float
+    left_path: str
+    right_path: str
+    pose: tuple = None     # (rotation 3x3 row-major, translation 3) or None
+
+
+@dataclass
+class StereoCalib:
+    fx: float
+    fy: float
+    cx: float
+    cy: float
+    baseline: float
+
+
+class KittiStereo:
+    """KittiDataset (vx-vision/src/datasets/kitti.rs:11-107) for both cameras
+    of a sequence: iterable over StereoFrame, len(), calib, read_pair."""
+
+    def __init__(self, sequence_dir, poses_file=None):
+        h = _vp()
+        check(lib.vxh_kitti_stereo_load(_path(sequence_dir),
+                                        None if poses_file is None else _path(poses_file),
+                                        ctypes.byref(h)))
+        self._h = h
+
+    def __len__(self):
+        return lib.vxh_kitti_stereo_len(self._h)
+
+    @property
+    def is_color(self):
+        return bool(lib.vxh_kitti_stereo_is_color(self._h))
+
+    def frame(self, i):
+        f = vxh_stereo_frame()
+        check(lib.vxh_kitti_stereo_frame(self._h, i, ctypes.byref(f)))
+        pose = (tuple(f.rotation), tuple(f.translation)) if f.has_pose else None
+        return StereoFrame(f.timestamp, os.fsdecode(f.left_path), os.fsdecode(f.right_path),
+                           pose)
+
+    def __iter__(self):
+        return (self.frame(i) for i in range(len(self)))
+
+    @property
+    def calib(self):
+        c = vxh_stereo_calib()
+        check(lib.vxh_kitti_stereo_calib(self._h, ctypes.byref(c)))
+        return StereoCalib(c.fx, c.fy, c.cx, c.cy, c.baseline) if c.valid else None
+
+    def read_pair(self, i):
+        """Frame i's images as (h, w) luma arrays, ready for SGM."""
+        w, h, _ = png_info(self.frame(i).left_path)
+        left = np.empty((h, w), np.uint8)
+        right = np.empty((h, w), np.uint8)
+        check(lib.vxh_kitti_stereo_read_pair(self._h, i, left.ctypes.data, right.ctypes.data,
+                                             left.size))
+        return left, right
+
+    def close(self):
+        if self._h:
+            lib.vxh_kitti_stereo_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 MIX_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080)]
 MIX_DISPARITIES = [64, 128, 256]
 

@@ -82,6 +82,45 @@ def remap(img, map_x, map_y, out_gray8=None):
     return out
 
 
+_sig("oracle_stereo_matrix", None, _vp, _vp, _vp, _vp, _u32, _u32, _vp, _vp)
+_sig("oracle_stereo_match", _u32, _vp, _vp, _vp, _vp, _u32, _u32, _vp, _u32, _vp)
+
+CORNER_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("response", "<f4"),
+                         ("pyramid_level", "<u4")])
+MATCH_DTYPE = np.dtype([("left_idx", "<u4"), ("right_idx", "<u4"), ("disparity", "<f4"),
+                        ("point_3d", "<f4", 3), ("pad", "<f4")])
+
+
+def _stereo_args(lk, rk, ld, rd, cfg):
+    lk = np.ascontiguousarray(lk, CORNER_DTYPE)
+    rk = np.ascontiguousarray(rk, CORNER_DTYPE)
+    ld = np.ascontiguousarray(ld, np.uint32).reshape(-1, 8)
+    rd = np.ascontiguousarray(rd, np.uint32).reshape(-1, 8)
+    p = np.array([cfg["max_epipolar"], cfg["min_disparity"], cfg["max_disparity"],
+                  cfg["ratio_thresh"], cfg["fx"], cfg["fy"], cfg["cx"], cfg["cy"],
+                  cfg["baseline"]], np.float32)
+    return lk, rk, ld, rd, p
+
+
+def stereo_matrix(lk, rk, ld, rd, cfg):
+    """StereoMatch.metal:32-62: the n_left x n_right u16 distance matrix."""
+    lk, rk, ld, rd, p = _stereo_args(lk, rk, ld, rd, cfg)
+    out = np.empty((len(lk), len(rk)), np.uint16)
+    lib.oracle_stereo_matrix(lk.ctypes.data, rk.ctypes.data, ld.ctypes.data, rd.ctypes.data,
+                             len(lk), len(rk), p.ctypes.data, out.ctypes.data)
+    return out
+
+
+def stereo_match(lk, rk, ld, rd, cfg):
+    """StereoMatch.metal:68-115 in ascending left index: a MATCH_DTYPE array."""
+    lk, rk, ld, rd, p = _stereo_args(lk, rk, ld, rd, cfg)
+    out = np.zeros(max(len(lk), 1), MATCH_DTYPE)
+    n = lib.oracle_stereo_match(lk.ctypes.data, rk.ctypes.data, ld.ctypes.data, rd.ctypes.data,
+                                len(lk), len(rk), p.ctypes.data, int(cfg["max_hamming"]),
+                                out.ctypes.data)
+    return out[:n]
+
+
 RECTIFY_FIELDS = ("fx", "fy", "cx", "cy", "k1", "k2", "p1", "p2", "k3")
 
 

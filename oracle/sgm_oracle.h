@@ -108,6 +108,16 @@ void oracle_remap(const void *in, uint32_t in_fmt, uint32_t in_w, uint32_t in_h,
 int oracle_rectify_maps(const double *p, uint32_t w, uint32_t h, float *map_x, float *map_y,
                         float *m_out);
 
+/* Sparse stereo matcher (stereo_match_oracle.cpp): keypoints 16 bytes each,
+ * descriptors 8 words each, p = max_epipolar, min_disparity, max_disparity,
+ * ratio_thresh, fx, fy, cx, cy, baseline. */
+void oracle_stereo_matrix(const void *lk, const void *rk, const uint32_t *ld,
+                          const uint32_t *rd, uint32_t nl, uint32_t nr, const float *p,
+                          uint16_t *out);
+uint32_t oracle_stereo_match(const void *lk, const void *rk, const uint32_t *ld,
+                             const uint32_t *rd, uint32_t nl, uint32_t nr, const float *p,
+                             uint32_t max_hamming, uint32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

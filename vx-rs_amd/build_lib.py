@@ -17,7 +17,7 @@ OUT = os.path.join(HERE, "vxsgm", "libvxsgm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["sgm_kernels.hip", "sgm_runtime.hip", "sgm_batch.hip", "sgm_server.cpp",
-               "depth.hip", "pipeline.hip", "rectify.hip", "datasets.cpp"]
+               "depth.hip", "pipeline.hip", "rectify.hip", "datasets.cpp", "stereo_match.hip"]
 CXX_SOURCES = ["synth.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I", os.path.join(ROOT, "include"),

@@ -1141,6 +1141,108 @@ class KittiStereo:
             pass
 
 
+# ---- sparse stereo (include/vxh/stereo_match.h) ----
+CORNER_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("response", "<f4"),
+                         ("pyramid_level", "<u4")])
+ORB_DTYPE = np.dtype([("desc", "<u4", 8), ("angle", "<f4")])
+MATCH_DTYPE = np.dtype([("left_idx", "<u4"), ("right_idx", "<u4"), ("disparity", "<f4"),
+                        ("point_3d", "<f4", 3), ("pad", "<f4")])
+
+
+class vxh_stereo_config(ctypes.Structure):
+    _fields_ = [("max_epipolar", ctypes.c_float), ("min_disparity", ctypes.c_float),
+                ("max_disparity", ctypes.c_float), ("max_hamming", ctypes.c_uint32),
+                ("ratio_thresh", ctypes.c_float), ("fx", ctypes.c_float),
+                ("fy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float),
+                ("baseline", ctypes.c_float)]
+
+
+_sig("vxh_stereo_config_default", None, ctypes.POINTER(vxh_stereo_config))
+_sig("vxh_stereo_matcher_create", ctypes.c_int, _vp, _texp)
+_sig("vxh_stereo_matcher_destroy", None, _vp)
+_sig("vxh_stereo_matcher_run", ctypes.c_int, _vp, _vp, _vp, _sz, _vp, _sz, _vp, _vp,
+     ctypes.POINTER(vxh_stereo_config), _vp, ctypes.POINTER(_sz))
+_sig("vxh_stereo_match_enqueue", ctypes.c_int, _vp, _vp, _vp, _vp, _u32, _u32,
+     ctypes.POINTER(vxh_stereo_config), _vp, _vp, _vp)
+_sig("vxh_stereo_hamming_matrix_enqueue", ctypes.c_int, _vp, _vp, _vp, _vp, _u32, _u32,
+     ctypes.POINTER(vxh_stereo_config), _vp, _vp)
+
+
+@dataclass
+class StereoConfig:
+    """StereoConfig (vx-vision/src/kernels/stereomatch.rs:19-69)."""
+    max_epipolar: float = 2.0
+    min_disparity: float = 1.0
+    max_disparity: float = 120.0
+    max_hamming: int = 50
+    ratio_thresh: float = 0.8
+    fx: float = 600.0
+    fy: float = 600.0
+    cx: float = 320.0
+    cy: float = 240.0
+    baseline: float = 0.12
+
+    def to_c(self):
+        return vxh_stereo_config(self.max_epipolar, self.min_disparity, self.max_disparity,
+                                 self.max_hamming, self.ratio_thresh, self.fx, self.fy, self.cx,
+                                 self.cy, self.baseline)
+
+    def as_dict(self):
+        return dict(self.__dict__)
+
+
+class StereoMatcher:
+    """StereoMatcher (vx-vision/src/kernels/stereomatch.rs:72-339)."""
+
+    def __init__(self, ctx):
+        h = _vp()
+        check(lib.vxh_stereo_matcher_create(ctx._h, ctypes.byref(h)))
+        self._h = h
+
+    def run(self, ctx, left_kpts, right_kpts, left_descs, right_descs, config):
+        """Keypoints as CORNER_DTYPE arrays, descriptors as ORB_DTYPE arrays;
+        returns a MATCH_DTYPE array in ascending left index."""
+        lk = np.ascontiguousarray(left_kpts, CORNER_DTYPE)
+        rk = np.ascontiguousarray(right_kpts, CORNER_DTYPE)
+        ld = np.ascontiguousarray(left_descs, ORB_DTYPE)
+        rd = np.ascontiguousarray(right_descs, ORB_DTYPE)
+        if len(lk) != len(ld) or len(rk) != len(rd):
+            raise InvalidConfig("invalid config: keypoint and descriptor counts differ")
+        out = np.zeros(max(len(lk), 1), MATCH_DTYPE)
+        n = _sz()
+        c = config.to_c()
+        check(lib.vxh_stereo_matcher_run(self._h, ctx._h, lk.ctypes.data, len(lk),
+                                         rk.ctypes.data, len(rk), ld.ctypes.data,
+                                         rd.ctypes.data, ctypes.byref(c), out.ctypes.data,
+                                         ctypes.byref(n)))
+        return out[: n.value]
+
+    def close(self):
+        if self._h:
+            lib.vxh_stereo_matcher_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stereo_match_enqueue(d_lk, d_rk, d_ld, d_rd, n_left, n_right, config, d_matches, d_count,
+                         stream=None):
+    c = config.to_c()
+    check(lib.vxh_stereo_match_enqueue(d_lk, d_rk, d_ld, d_rd, n_left, n_right, ctypes.byref(c),
+                                       d_matches, d_count, stream))
+
+
+def stereo_hamming_matrix_enqueue(d_lk, d_rk, d_ld, d_rd, n_left, n_right, config, d_matrix,
+                                  stream=None):
+    c = config.to_c()
+    check(lib.vxh_stereo_hamming_matrix_enqueue(d_lk, d_rk, d_ld, d_rd, n_left, n_right,
+                                                ctypes.byref(c), d_matrix, stream))
+
+
 MIX_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080)]
 MIX_DISPARITIES = [64, 128, 256]
 

@@ -33,12 +33,14 @@ constexpr int kOutU16 = 1;   // full u16 path cost
 
 // One direction pass of the packed-pair aggregation.
 struct PassParams {
-    const uint8_t *cost;   // matching costs, entry pix*Dp + d (u8, 64 where x - d < 0)
+    const uint8_t *cost;   // matching costs + bias, entry pix*Dp + d (u8, 64 + bias where
+                           // x - d < 0)
     uint8_t *vol;          // this direction's volume
     uint8_t *hi;           // kOutU8: high bytes of flagged slices (1 byte per entry)
     uint8_t *flags;        // kOutU8: 8 bytes per pixel, byte dir = 1 when flagged
     int w, h, D, Dp;
     uint32_t p1x2, p2x2;   // penalties truncated to u16, in both halves
+    uint32_t xqx2;         // P1 - the cost volume's bias, in both halves
     int dir, dx, dy, nscan;
     int mode;              // kOutU8 / kOutU16 (read only where the kernel takes it at run time)
 };
@@ -147,9 +149,13 @@ hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
                          uint2 *out, uint2 *zero_flags, uint32_t *zero_words, size_t nwords,
                          hipStream_t s);
 // Matching-cost volume of a pair from its census (layout of
-// PassParams::cost, Dp = volume_stride(D)).
-hipError_t launch_cost(const uint2 *lc, const uint2 *rc, int w, int h, int D, uint8_t *cost,
-                       hipStream_t s);
+// PassParams::cost, Dp = volume_stride(D)), every entry plus `bias`
+// (cost_bias(p1)).
+hipError_t launch_cost(const uint2 *lc, const uint2 *rc, int w, int h, int D, uint32_t bias,
+                       uint8_t *cost, hipStream_t s);
+// The bias the cost volume carries: P1 itself when cost + P1 fits a byte
+// (costs are at most 64), else 0.  The recurrence adds the rest of P1.
+inline uint32_t cost_bias(uint32_t p1_u16) { return p1_u16 <= 191u ? p1_u16 : 0u; }
 // *dst |= *src with a kernel (stream-ordered, safe inside captured graphs).
 hipError_t launch_or_word(uint32_t *dst, const uint32_t *src, hipStream_t s);
 // Zeroes n words with a kernel (safe inside captured graphs).

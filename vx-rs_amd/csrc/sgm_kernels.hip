@@ -39,7 +39,7 @@ constexpr size_t kCcLds = (size_t)(kCcSpanR + kCcTile) * 8 + 4 * 64 * kCcOutRow;
 
 __global__ void __launch_bounds__(256)
 cost_kernel(const uint2 *__restrict__ lc, const uint2 *__restrict__ rc, int w, int h, int D,
-            int Dp, uint8_t *__restrict__ cost) {
+            int Dp, uint32_t bias, uint8_t *__restrict__ cost) {
     extern __shared__ __align__(16) uint8_t lds[];
     uint2 *cen_r = (uint2 *)lds;                            // kCcSpanR words
     uint2 *cen_l = cen_r + kCcSpanR;                        // kCcTile words
@@ -93,7 +93,7 @@ cost_kernel(const uint2 *__restrict__ lc, const uint2 *__restrict__ rc, int w, i
                                 const uint2 r = rb[15 - i];
                                 c = __popc(lv.x ^ r.x) + __popc(lv.y ^ r.y);
                             }
-                            v |= c << (8 * b);
+                            v |= (c + bias) << (8 * b);
                         }
                         wd[q] = v;
                     }
@@ -319,11 +319,11 @@ hipError_t launch_or_word(uint32_t *dst, const uint32_t *src, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_cost(const uint2 *lc, const uint2 *rc, int w, int h, int D, uint8_t *cost,
-                       hipStream_t s) {
+hipError_t launch_cost(const uint2 *lc, const uint2 *rc, int w, int h, int D, uint32_t bias,
+                       uint8_t *cost, hipStream_t s) {
     const dim3 grid((unsigned)((w + kCcTile - 1) / kCcTile), (unsigned)h);
     hipLaunchKernelGGL(cost_kernel, grid, dim3(256), kCcLds, s, lc, rc, w, h, D,
-                       volume_stride(D), cost);
+                       volume_stride(D), bias, cost);
     return hipGetLastError();
 }
 

@@ -179,7 +179,7 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     const int d0 = sub * K;
     const bool lane_real = d0 < D;
     const bool lane_store = d0 < Dp;
-    const uint32_t p1x2 = p.p1x2, p2x2 = p.p2x2;
+    const uint32_t p1x2 = p.p1x2, p2x2 = p.p2x2, xqx2 = p.xqx2;
     // Lanes past the volume stride read their pixel's first entries (unused).
     const uint8_t *__restrict__ cost = p.cost + (lane_store ? d0 : 0);
 
@@ -381,9 +381,14 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
         // below gives agg = C there (best = min(0, ...) = 0), the first
         // pixel's copy of the cost (SGMStereo.metal:152), so no step is
         // special-cased.
-        uint32_t agg[K], q[K];   // q[i] = agg[i] + P1
+        // The chain runs on q[i] = agg[i] + P1 (the d-1 term of d + 1):
+        // q[i] = C[i] + P1 - pm + min(A[i], q[i-1]).  The cost volume holds
+        // C + bias, so B[i] = that cost - (pm - (P1 - bias)) is C - pm + P1
+        // with one subtraction per entry; agg = q - P1 leaves the chain.
+        uint32_t agg[K], q[K];
         {
             const uint32_t o3 = pk_add(pm, p2x2);
+            const uint32_t pmx = pk_sub(pm, xqx2);
             const uint32_t nbq = from_upper_lane<L>(prevq[0]);
             uint32_t A[K], B[K];
 #pragma unroll
@@ -391,16 +396,12 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
                 uint32_t o2 = (i + 1 < K) ? prevq[(i + 1) % K] : nbq;
                 if (!ALIGNED || i == K - 1) o2 |= om[i];
                 A[i] = pk_min(pk_min(prev[i], o2), o3);
-                B[i] = pk_sub(C[i], pm);
+                B[i] = pk_sub(C[i], pmx);
             }
             // Speculative pass: no carry-in from the lane below.
-            agg[0] = pk_add(B[0], A[0]);
-            q[0] = pk_add(agg[0], p1x2);
+            q[0] = pk_add(B[0], A[0]);
 #pragma unroll
-            for (int i = 1; i < K; ++i) {
-                agg[i] = pk_add(B[i], pk_min(A[i], q[i - 1]));
-                q[i] = pk_add(agg[i], p1x2);
-            }
+            for (int i = 1; i < K; ++i) q[i] = pk_add(B[i], pk_min(A[i], q[i - 1]));
             // Fixed-point rounds over the carry-in.  Halves whose scanline has
             // ended, the group's first lane and padding lanes take no carry.
             const uint32_t take = (sub > 0 && lane_real)
@@ -413,15 +414,13 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
                 if (lanes_ne(e, e_used) == 0) break;
                 if (e != e_used) {
                     e_used = e;
-                    agg[0] = pk_add(B[0], e);
-                    q[0] = pk_add(agg[0], p1x2);
+                    q[0] = pk_add(B[0], e);
 #pragma unroll
-                    for (int i = 1; i < K; ++i) {
-                        agg[i] = pk_add(B[i], pk_min(A[i], q[i - 1]));
-                        q[i] = pk_add(agg[i], p1x2);
-                    }
+                    for (int i = 1; i < K; ++i) q[i] = pk_add(B[i], pk_min(A[i], q[i - 1]));
                 }
             }
+#pragma unroll
+            for (int i = 0; i < K; ++i) agg[i] = pk_sub(q[i], p1x2);
         }
 
         uint32_t m = 0xFFFFFFFFu;

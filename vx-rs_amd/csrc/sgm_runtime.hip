@@ -483,6 +483,8 @@ static vxh_status paths_and_wta(const uint8_t *cost, uint32_t w, uint32_t h,
     const uint32_t p1 = (uint32_t)c->p1 & 0xFFFFu, p2 = (uint32_t)c->p2 & 0xFFFFu;
     base.p1x2 = p1 | (p1 << 16);
     base.p2x2 = p2 | (p2 << 16);
+    const uint32_t xq = p1 - vxh::cost_bias(p1);
+    base.xqx2 = xq | (xq << 16);
     int dirs[vxh::kMaxPasses];
     uint8_t *vols[vxh::kMaxPasses], *his[vxh::kMaxPasses];
     for (int i = 0; i < P; ++i) {
@@ -584,7 +586,9 @@ vxh_status run_pipeline(const uint8_t *d_left, const uint8_t *d_right, uint32_t 
                           nullptr, 0, st),
             "census");
     uint8_t *cost = (uint8_t *)(base + L.cost);
-    VXH_HIP(launch_cost(lc, rc, (int)w, (int)h, D, cost, st), "cost");
+    VXH_HIP(launch_cost(lc, rc, (int)w, (int)h, D, cost_bias((uint32_t)c->p1 & 0xFFFFu), cost,
+                        st),
+            "cost");
     if (marks) VXH_HIP(hipEventRecord(marks[0], st), "hipEventRecord");
     int16_t *best = lr ? (int16_t *)(base + L.best) : nullptr;
     uint16_t *sum = lr ? (uint16_t *)(base + L.sum) : nullptr;
@@ -898,6 +902,7 @@ vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,
     hipError_t e = hipMallocAsync((void **)&g, bytes, st);
     if (e != hipSuccess) return hip_fail(e, "hipMallocAsync", bytes);
     e = vxh::launch_cost((const uint2 *)d_lc, (const uint2 *)d_rc, (int)w, (int)h, D,
+                         vxh::cost_bias((uint32_t)c->p1 & 0xFFFFu),
                          (uint8_t *)(g + L.cost), st);
     if (e == hipSuccess && mode == vxh::kOutU8) e = hipMemsetAsync(g + L.flags, 0, n * 8, st);
     if (e == hipSuccess)

@@ -275,14 +275,24 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     // Steps from m that a tail publishes early (two rounds).
     constexpr int kRange = 2 * NB;
 
-    // FULL: step t lies inside every scanline of the wave (no clamp needed).
+    // FULL: step t lies inside every scanline of the wave (no clamp needed);
+    // the loads then come from fptr, which the full rounds advance by one
+    // pixel step per issue instead of recomputing pix * Dp.
+    const uint8_t *fptr[2];
+    const ptrdiff_t cstep = (ptrdiff_t)pstep * Dp;
     auto issue = [&](PairBuf<K> &b, int t, auto full) {
         constexpr bool FULL = decltype(full)::value;
 #pragma unroll
         for (int hh = 0; hh < 2; ++hh) {
             // Inactive scanlines read pixel 0.
-            const uint32_t pix = FULL || t < len[hh] ? (uint32_t)(pix0[hh] + t * pstep) : 0u;
-            const uint8_t *src = cost + (size_t)pix * (uint32_t)Dp;
+            const uint8_t *src;
+            if constexpr (FULL) {
+                src = fptr[hh];
+                fptr[hh] += cstep;
+            } else {
+                const uint32_t pix = t < len[hh] ? (uint32_t)(pix0[hh] + t * pstep) : 0u;
+                src = cost + (size_t)pix * (uint32_t)Dp;
+            }
             if constexpr (K == 2) b.c[hh][0] = *(const uint16_t *)src;
             else if constexpr (K == 4) b.c[hh][0] = *(const uint32_t *)src;
             else if constexpr (K == 8) {
@@ -555,6 +565,9 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
         __builtin_amdgcn_sched_barrier(0);
     }
     int r = 0;
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh)
+        fptr[hh] = cost + (ptrdiff_t)(pix0[hh] + PF * pstep) * Dp;
     for (; r < full_rounds; ++r) {
         const int t = r * NB;
 #pragma unroll

@@ -9,15 +9,48 @@ namespace {
 
 using namespace dev;
 
+// Census of 64 x 4 pixels per block (blockDim (64, 4)).  The default 9 x 7
+// window reads its 10 x 72 tile of clamped image rows once into LDS, which
+// reproduces clamp-to-edge exactly, so every pixel then takes the same path:
+// 63 LDS byte reads at constant offsets, and each bit shifted in with one
+// subtraction (n - c has bit 31 set iff n < c) and one alignbit, from the last
+// neighbour down so that neighbour k ends at bit k.  Other windows read the
+// image directly.
+constexpr int kCtW = 64 + 8, kCtH = 4 + 6;
+
 __global__ void __launch_bounds__(256)
 census_kernel(const uint8_t *__restrict__ img, int w, int h, int rx, int ry,
               uint2 *__restrict__ out, uint2 *__restrict__ zero_flags,
               uint32_t *__restrict__ zero_words, size_t nwords) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= w || y >= h) return;
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    uint2 word;
+    if (rx == 4 && ry == 3) {   // block-uniform
+        __shared__ uint8_t tile[kCtH * kCtW];
+        const int x0 = blockIdx.x * 64 - 4, y0 = blockIdx.y * 4 - 3;
+        for (int t = threadIdx.y * 64 + threadIdx.x; t < kCtH * kCtW; t += 256) {
+            const int r = t / kCtW, c = t - r * kCtW;
+            tile[t] = img[(size_t)min(max(y0 + r, 0), h - 1) * w + min(max(x0 + c, 0), w - 1)];
+        }
+        __syncthreads();
+        if (x >= w || y >= h) return;
+        const uint8_t *win = tile + threadIdx.y * kCtW + threadIdx.x;   // window's top-left
+        const uint32_t c = win[3 * kCtW + 4];
+        uint32_t lo = 0, hi = 0;
+#pragma unroll
+        for (int k = 61; k >= 0; --k) {
+            const int j = k < 31 ? k : k + 1;   // window index, the centre (31) skipped
+            const uint32_t d = (uint32_t)win[(j / 9) * kCtW + j % 9] - c;
+            if (k >= 32) hi = __builtin_amdgcn_alignbit(hi, d, 31);
+            else lo = __builtin_amdgcn_alignbit(lo, d, 31);
+        }
+        word = make_uint2(lo, hi);
+    } else {
+        if (x >= w || y >= h) return;
+        word = census_at(img, w, h, rx, ry, x, y);
+    }
     const size_t i = (size_t)y * w + x;
-    out[i] = census_at(img, w, h, rx, ry, x, y);
+    out[i] = word;
     if (zero_flags) zero_flags[i] = make_uint2(0, 0);
     for (size_t k = i; k < nwords; k += (size_t)w * h) zero_words[k] = 0;
 }

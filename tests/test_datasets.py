@@ -124,6 +124,13 @@ def test_png_rejects_bad_files(tmp_path):
         vxsgm.png_read(p)
     with pytest.raises(vxsgm.InvalidConfig):
         vxsgm.png_read(tmp_path / "missing.png")
+    # A header alone cannot make the reader allocate more than 1 GiB.
+    ihdr = struct.pack(">IIBBBBB", 65535, 65535, 8, 6, 0, 0, 0)
+    big = (b"\x89PNG\r\n\x1a\n" + struct.pack(">I", 13) + b"IHDR" + ihdr +
+           struct.pack(">I", zlib.crc32(b"IHDR" + ihdr) & 0xFFFFFFFF))
+    p.write_bytes(big)
+    with pytest.raises(vxsgm.InvalidConfig, match="1 GiB"):
+        vxsgm.png_read(p)
 
 
 def make_sequence(root, n=4, color=False, size=(24, 10), calib=True, skip_right=None):

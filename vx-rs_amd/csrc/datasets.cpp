@@ -22,6 +22,7 @@
 #include <cstring>
 #include <fstream>
 #include <iterator>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -97,6 +98,10 @@ bool parse_f32(const std::string &tok, float *v) {
 
 // ---- PNG ----
 
+// Largest decoded image accepted (a header cannot make the reader allocate
+// more than this).
+constexpr uint64_t kMaxImageBytes = 1ull << 30;
+
 struct Image {
     uint32_t w = 0, h = 0, channels = 0;
     std::vector<uint8_t> pix;
@@ -149,6 +154,8 @@ vxh_status png_load(const std::string &path, Image *img, bool header_only) {
                 case 6: img->channels = 4; break;
                 default: return invalid(path + ": palette PNG not supported");
             }
+            if ((uint64_t)img->w * img->h * img->channels > kMaxImageBytes)
+                return invalid(path + ": PNG image over 1 GiB");
             have_header = true;
             if (header_only) return VXH_OK;
         } else if (t == "IDAT") {
@@ -158,13 +165,19 @@ vxh_status png_load(const std::string &path, Image *img, bool header_only) {
         }
         pos += 12 + (size_t)len;
     }
+    if (!have_header) return invalid(path + ": PNG without IHDR");
     const size_t row = (size_t)img->w * img->channels;
     const size_t expect = (row + 1) * img->h;
-    std::vector<uint8_t> raw(expect);
+    std::vector<uint8_t> raw;
+    try {
+        raw.resize(expect);
+        img->pix.assign(row * img->h, 0);
+    } catch (const std::bad_alloc &) {
+        return vxh::report_error(VXH_ERR_BUFFER_ALLOC, "buffer allocation failed (PNG " + path + ")");
+    }
     uLongf got = (uLongf)expect;
     if (uncompress(raw.data(), &got, zdata.data(), (uLong)zdata.size()) != Z_OK || got != expect)
         return invalid(path + ": corrupt PNG image data");
-    img->pix.assign(row * img->h, 0);
     const int bpp = (int)img->channels;
     for (uint32_t y = 0; y < img->h; ++y) {
         const uint8_t f = raw[y * (row + 1)];

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -248,7 +249,12 @@ vxh_status vxh_stereo_matcher_run(vxh_stereo_matcher *m, vxh_context *ctx,
     const size_t o_m = o_rd + up(nr * 32);
     const size_t o_c = o_m + up(nl * sizeof(vxh_stereo_match));
     const size_t total = o_c + 256;
-    std::vector<uint32_t> words((nl + nr) * 8);
+    std::vector<uint32_t> words;
+    try {
+        words.resize((nl + nr) * 8);
+    } catch (const std::bad_alloc &) {
+        return vxh::report_error(VXH_ERR_BUFFER_ALLOC, "buffer allocation failed (descriptors)");
+    }
     for (size_t i = 0; i < nl; ++i) std::memcpy(&words[i * 8], ldesc[i].desc, 32);
     for (size_t i = 0; i < nr; ++i) std::memcpy(&words[(nl + i) * 8], rdesc[i].desc, 32);
     vxh::CaptureRelaxed relaxed;

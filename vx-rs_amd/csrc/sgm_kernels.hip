@@ -140,10 +140,14 @@ cost_kernel(const uint2 *__restrict__ lc, const uint2 *__restrict__ rc, int w, i
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // Chunk i is chunk c of pixel px: i / nch by a multiply, exact for
+        // nch <= 8 and i < 64 * nch (the error is below 1/128, the gap to the
+        // next multiple at least 1/8); offsets from dst fit 32 bits.
         const int nch = segw / 16;
+        const uint32_t magic = (65536u + (uint32_t)nch - 1u) / (uint32_t)nch;
         for (int i = lane; i < npx * nch; i += 64) {
-            const int px = i / nch, c = i - px * nch;
-            *(uint4 *)(dst + (size_t)px * Dp + seg + c * 16) =
+            const int px = (int)(((uint32_t)i * magic) >> 16), c = i - px * nch;
+            *(uint4 *)(dst + (uint32_t)(px * Dp + seg + c * 16)) =
                 *(const uint4 *)(out + px * kCcOutRow + c * 16);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

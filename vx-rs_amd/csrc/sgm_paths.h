@@ -421,7 +421,8 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
             for (int r = 0; r < L - 1; ++r) {
                 const uint32_t cand = pk_min(A[0], from_lower_lane<L>(q[K - 1]));
                 const uint32_t e = (cand & take) | (e_used & ~take);
-                if (lanes_ne(e, e_used) == 0) break;
+                // Usually no lane's carry changes: the exit falls through.
+                if (__builtin_expect(lanes_ne(e, e_used) == 0, 1)) break;
                 if (e != e_used) {
                     e_used = e;
                     q[0] = pk_add(B[0], e);

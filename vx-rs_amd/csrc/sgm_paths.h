@@ -418,16 +418,23 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
                                       ? ((act0 ? 0x0000FFFFu : 0u) | (act1 ? 0xFFFF0000u : 0u))
                                       : 0u;
             uint32_t e_used = A[0];
-            for (int r = 0; r < L - 1; ++r) {
+            auto carry = [&]() {
                 const uint32_t cand = pk_min(A[0], from_lower_lane<L>(q[K - 1]));
-                const uint32_t e = (cand & take) | (e_used & ~take);
-                // Usually no lane's carry changes: the exit falls through.
-                if (__builtin_expect(lanes_ne(e, e_used) == 0, 1)) break;
-                if (e != e_used) {
-                    e_used = e;
-                    q[0] = pk_add(B[0], e);
+                return (cand & take) | (e_used & ~take);
+            };
+            uint32_t e = carry();
+            // Usually no lane's carry changes: that path falls straight through,
+            // and the rounds (at most L - 1 recomputations) sit out of line.
+            if (__builtin_expect(lanes_ne(e, e_used) != 0, 0)) {
+                for (int r = 0; r < L - 1; ++r) {
+                    if (e != e_used) {
+                        e_used = e;
+                        q[0] = pk_add(B[0], e);
 #pragma unroll
-                    for (int i = 1; i < K; ++i) q[i] = pk_add(B[i], pk_min(A[i], q[i - 1]));
+                        for (int i = 1; i < K; ++i) q[i] = pk_add(B[i], pk_min(A[i], q[i - 1]));
+                    }
+                    e = carry();
+                    if (lanes_ne(e, e_used) == 0) break;
                 }
             }
 #pragma unroll

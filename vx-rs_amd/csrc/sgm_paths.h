@@ -448,7 +448,7 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
         pm = group_min_pk<L>(m);
         // The previous step's high-byte slices, if any (see flush_high),
         // before prev is overwritten.
-        if (mode == kOutU8) flush_high(t - 1);
+        if (mode == kOutU8 && __builtin_expect(high_lanes != 0, 0)) flush_high(t - 1);
 #pragma unroll
         for (int i = 0; i < K; ++i) {
             prev[i] = agg[i];

@@ -34,6 +34,9 @@ extern "C" {
 #endif
 
 /* ---- Pipeline (vx-vision/src/pipeline.rs:12-80) ---- */
+/* One pipeline is used from one thread at a time (like the reference's,
+ * which is neither Send nor Sync); separate pipelines may be encoded from
+ * separate threads at once. */
 typedef struct vxh_pipeline vxh_pipeline;
 
 /* Pipeline::begin (pipeline.rs:21-31): a new, empty pipeline on ctx's
@@ -122,7 +125,8 @@ typedef struct vxh_texture_pool vxh_texture_pool;
 /* TexturePool::new (max_per_bucket 8) / with_capacity (pool.rs:26-43). */
 vxh_status vxh_texture_pool_create(size_t max_per_bucket, vxh_texture_pool **out);
 /* Destroys the pool and every texture cached in it (acquired textures are
- * the caller's). */
+ * the caller's).  As for any texture, the cached ones must be destroyed
+ * before the context they were made on. */
 void vxh_texture_pool_destroy(vxh_texture_pool *pool);
 
 /* TexturePool::acquire (pool.rs:61-81): a cached texture of that size and

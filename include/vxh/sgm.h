@@ -123,6 +123,7 @@ uint32_t vxh_texture_width(const vxh_texture *tex);
 uint32_t vxh_texture_height(const vxh_texture *tex);
 /* Device address of the texture's pixels (row-major, w*h elements). */
 void *vxh_texture_device_ptr(const vxh_texture *tex);
+/* Frees the texture; it must be destroyed before its context. */
 void vxh_texture_destroy(vxh_texture *tex);
 
 /* ---- SGM kernel struct (vx-vision/src/kernels/sgm.rs:61-105) ---- */

@@ -1,6 +1,6 @@
 """SGM stereo benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms plain,batched,served]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms plain,batched,stream,served]
 
 Headline (`value`): the plain per-pair path.  A step is one pass of the SGM
 pipeline (census of both images, matching costs, path aggregation over every
@@ -27,7 +27,10 @@ confirms that the timed GPU map is bit-identical to it.
 `batched`: the 64-pair mixture of sizes {320x240, 640x480, 1280x720,
 1920x1080} x D {64, 128, 256}, 8 paths, as per-pair pipelines over several
 streams replayed from one hipGraph, against a loop of plain per-pair calls on
-one stream; all maps compared bit for bit.  `served`: Poisson arrivals at 200/500/1000 req/s of the same mixture
+one stream; all maps compared bit for bit.  `stream`: the headline workload as a stream of
+identical 720p pairs overlapped over 2 or 4 streams (one graph replay of 8 pairs), per-pair time
+and Gpx*disp/s, every map compared with a plain call; `value` stays the plain per-pair figure.
+`served`: Poisson arrivals at 200/500/1000 req/s of the same mixture
 against the serving loop and against a FIFO queue into the plain path;
 p50/p99 latency and Gpx*disp/s for each.
 """
@@ -53,7 +56,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle threads for the CPU baseline (0 = host cores, max 16)")
-    ap.add_argument("--forms", default="plain,batched,served")
+    ap.add_argument("--forms", default="plain,batched,stream,served")
+    ap.add_argument("--stream-pairs", type=int, default=8,
+                    help="720p pairs per replay of the stream form")
     ap.add_argument("--batch-pairs", type=int, default=64)
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--rates", default="200,500,1000")
@@ -161,6 +166,48 @@ def batched_leg(vx, hip, np, args, sgm, tuner):
         "bit_exact_vs_plain": bool(exact),
         "form": "per-pair pipelines over several streams, replayed from one hipGraph",
         "streams": opts[0] if opts else "default",
+    }
+
+
+def stream_leg(vx, hip, np, args, sgm):
+    """The headline workload as a stream of pairs: n copies of the 720p pair
+    (one input pair, n outputs) replayed from one graph over S streams, so a
+    pair's aggregation overlaps other pairs' census, costs and sums.  Every
+    map is compared bit for bit with one plain call."""
+    left, right, _, cfg = vx.workload_pair("720p_d128_8path")
+    h, w = left.shape
+    d = cfg.num_disparities
+    dl, dr = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
+    outs = [hip.DeviceBuffer(w * h * 4) for _ in range(args.stream_pairs)]
+    descs = [(dl.ptr, dr.ptr, o.ptr, w, h, cfg, None) for o in outs]
+    want = sgm.compute_host(left, right, w, h, cfg)
+    stream = hip.Stream()
+    e0, e1 = hip.Event(), hip.Event()
+    per_streams = {}
+    exact = True
+    for nstreams in (2, 4):
+        batch = vx.SGMBatch(descs, options=(nstreams,))
+        batch.launch(stream.handle)
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(args.batch_reps):
+            batch.launch(stream.handle)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / args.batch_reps
+        batch.close()
+        exact = exact and all(np.array_equal(o.download(np.float32).view(np.uint32),
+                                             want.view(np.uint32)) for o in outs)
+        per_streams[nstreams] = round(ms / len(outs), 4)
+        log(f"stream: {nstreams} streams, {ms / len(outs):.4f} ms per pair")
+    best = min(per_streams, key=per_streams.get)
+    ms_pair = per_streams[best]
+    return {
+        "workload": "720p_d128_8path", "pairs_per_replay": len(outs),
+        "ms_per_pair": ms_pair, "ms_per_pair_by_streams": per_streams,
+        "value": round(w * h * d / (ms_pair * 1e-3) / 1e9, 3), "unit": "Gpx*disp/s",
+        "bit_exact_vs_plain": bool(exact),
+        "form": "identical 720p pairs overlapped over several streams, one hipGraph replay",
     }
 
 
@@ -340,6 +387,9 @@ def main():
     if "batched" in forms:
         log(f"batched leg ({args.batch_pairs} pairs)")
         result["batched"] = batched_leg(vx, hip, np, args, sgm, tuner)
+    if "stream" in forms:
+        log(f"stream leg ({args.stream_pairs} pairs of 720p)")
+        result["stream"] = stream_leg(vx, hip, np, args, sgm)
     if "served" in forms:
         log(f"served leg (rates {args.rates})")
         result["served"] = served_leg(vx, np, args)

@@ -226,3 +226,29 @@ def test_textures_made_while_encoding(ctx, sgm):
         pool.close()
     finally:
         pipe.close()
+
+
+@pytest.mark.parametrize("fmt,split,lr", [("u16", 0, True), ("u8", 1, True), ("u16", 1, False)])
+def test_encode_volume_formats_split_and_lr(ctx, fmt, split, lr):
+    """The encode path with u16 or forced-u8 per-direction volumes, split
+    scanlines and the LR check (D = 96 on a tall pair, so the vertical
+    directions qualify for splitting), bit for bit against the oracle."""
+    w, h, D = 120, 400, 96
+    left, right, tl, tr = pair_textures(ctx, w, h, 81 + len(fmt) + 2 * split)
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        s.set_volume_format(D, fmt)
+        s.set_split(D, split)
+        cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=8, lr_check=lr, p1=7.0,
+                                    p2=200.0)
+        out = ctx.texture_output_r32float(w, h)
+        pipe = vxsgm.Pipeline.begin(ctx)
+        try:
+            s.encode(pipe, tl, tr, out, cfg)
+            pipe.commit_and_wait()
+            want = O.sgm(left, right, D, p1=7.0, p2=200.0, num_paths=8, lr_check=lr).ravel()
+            assert (bits(out.read_r32float()) == bits(want)).all()
+        finally:
+            pipe.close()
+    finally:
+        s.close()

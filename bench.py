@@ -55,13 +55,19 @@ def parse():
     ap.add_argument("--workload", default="720p_d128_8path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0,
-                    help="oracle threads for the CPU baseline (0 = host cores, max 16)")
+                    help="oracle threads of the all-cores CPU baseline (0 = every CPU this "
+                         "process may run on)")
     ap.add_argument("--forms", default="plain,batched,stream,served")
     ap.add_argument("--stream-pairs", type=int, default=8,
                     help="720p pairs per replay of the stream form")
     ap.add_argument("--batch-pairs", type=int, default=64)
     ap.add_argument("--batch-reps", type=int, default=5)
-    ap.add_argument("--rates", default="200,500,1000")
+    ap.add_argument("--rates", default="200,500,1000,1300,1600,2000",
+                    help="Poisson arrival rates of the served leg (req/s)")
+    ap.add_argument("--slo-ms", type=float, default=25.0,
+                    help="p99 latency target of the served leg's capacity figure")
+    ap.add_argument("--serve-streams", type=int, default=4,
+                    help="HIP streams of the serving loop")
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -97,6 +103,16 @@ def committed_traffic(schedule, workload):
         if line.startswith("schedule:"):
             sched = line.split(":", 1)[1].strip()
     return got if sched == schedule else None
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def log(msg):
@@ -153,6 +169,9 @@ def batched_leg(vx, hip, np, args, sgm, tuner):
     log(f"batched: plain loop {ms_plain:.3f} ms")
     exact = all(np.array_equal(ob.download(np.uint32), op.download(np.uint32))
                 for _, _, ob, op in bufs)
+    want = golden_hashes("batch_64") if args.batch_pairs == 64 else None
+    vs_oracle = None if want is None else all(
+        map_sha256(ob.download(np.float32)) == want[k] for k, (_, _, ob, _) in enumerate(bufs))
     pxd = sum(w * h * d for w, h, d, _ in host)
     alg = sum(w * h * (4 * d + 6) for w, h, d, _ in host)
     batch.close()
@@ -164,6 +183,7 @@ def batched_leg(vx, hip, np, args, sgm, tuner):
         "unit": "Gpx*disp/s",
         "roofline_frac_batch": round(alg / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "bit_exact_vs_plain": bool(exact),
+        "bit_exact_vs_oracle": vs_oracle,
         "form": "per-pair pipelines over several streams, replayed from one hipGraph",
         "streams": opts[0] if opts else "default",
     }
@@ -211,25 +231,53 @@ def stream_leg(vx, hip, np, args, sgm):
     }
 
 
+def golden_hashes(name):
+    """SHA-256 of the oracle's map for every pair of a mixture
+    (tests/golden/mixture_hashes.json), or None when the file lacks it."""
+    path = os.path.join(ROOT, "tests", "golden", "mixture_hashes.json")
+    try:
+        with open(path) as f:
+            return [r["sha256"] for r in json.load(f)[name]["pairs"]]
+    except (OSError, KeyError):
+        return None
+
+
+def map_sha256(a):
+    import hashlib
+    return hashlib.sha256(a.view("uint32").tobytes()).hexdigest()
+
+
 def served_leg(vx, np, args):
-    """Poisson arrivals of the mixture against the serving loop and against a
-    FIFO queue into the plain per-pair path (one request at a time)."""
+    """Poisson arrivals of the mixture's 32-pair request pool against the
+    serving loop and against a FIFO queue into the plain per-pair path (one
+    request at a time), over a sweep of rates; both keep their device slots
+    reserved before the timed runs.  Every map is checked against the oracle's
+    hash for its pair.  `capacity_at_slo`: for each form, the highest swept
+    rate whose p99 stays within the SLO, and the Gpx*disp/s it sustained."""
     pool = []
     for w, h, d, seed in vx.mixture(32, seed=0x5347_0004):
         left, right, cfg = vx.mixture_pair(w, h, d, seed)
         pool.append((left, right, w, h, cfg))
-    ref = [None] * len(pool)
-    out = {"requests_per_rate": args.requests, "budget_gb": 64, "bit_exact": True}
-    for name, kw in (("served", dict(max_inflight=32, num_streams=8)),
-                     ("plain_fifo", dict(max_inflight=1, num_streams=1))):
+    want = golden_hashes("serve_pool_32")
+    out = {"requests_per_rate": args.requests, "budget_gb": 64, "slo_p99_ms": args.slo_ms,
+           "bit_exact_vs_oracle": want is not None}
+    shapes = sorted({(w, h, cfg.num_disparities) for _, _, w, h, cfg in pool},
+                    key=lambda s: s[0] * s[1] * s[2])
+    forms = (("served", dict(max_inflight=32, num_streams=args.serve_streams), 2),
+             ("plain_fifo", dict(max_inflight=1, num_streams=1), 0))
+    for name, kw, per_shape in forms:
         srv = vx.SGMServer(hbm_budget_bytes=64 << 30, **kw)
+        # Resident slots: `per_shape` per shape of the pool plus the largest.
+        w, h, d = shapes[-1]
+        srv.reserve(w, h, vx.SGMStereoConfig(num_disparities=d, num_paths=8), 4)
+        for w, h, d in shapes[:-1]:
+            if per_shape:
+                srv.reserve(w, h, vx.SGMStereoConfig(num_disparities=d, num_paths=8), per_shape)
         for k, (l, r, w, h, cfg) in enumerate(pool):
             o = np.empty(w * h, np.float32)
             srv.wait(srv.submit(l, r, w, h, cfg, o))
-            if ref[k] is None:
-                ref[k] = o
-            elif not np.array_equal(o.view(np.uint32), ref[k].view(np.uint32)):
-                out["bit_exact"] = False
+            if want is not None and map_sha256(o) != want[k]:
+                out["bit_exact_vs_oracle"] = False
         rates = {}
         for rate in [float(x) for x in args.rates.split(",")]:
             rng = np.random.default_rng(int(rate))
@@ -247,18 +295,24 @@ def served_leg(vx, np, args):
             srv.drain()
             st = srv.stats()
             span = st["last_complete_s"] - st["first_submit_s"]
-            exact = all(np.array_equal(o.view(np.uint32), ref[i % len(pool)].view(np.uint32))
-                        for i, o in enumerate(outs))
-            out["bit_exact"] = out["bit_exact"] and exact
+            if want is not None:
+                exact = all(map_sha256(o) == want[i % len(pool)] for i, o in enumerate(outs))
+                out["bit_exact_vs_oracle"] = out["bit_exact_vs_oracle"] and exact
             rates[str(int(rate))] = {
                 "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
                 "gpxd_per_s": round(st["pxd_total"] / span / 1e9, 3),
                 "offered_gpxd_per_s": round(rate * st["pxd_total"] / st["completed"] / 1e9, 3),
                 "peak_inflight": st["peak_inflight"],
-                "peak_admitted_gb": round(st["peak_admitted_bytes"] / 2**30, 3)}
+                "peak_admitted_gb": round(st["peak_admitted_bytes"] / 2**30, 3),
+                "slot_misses": st["slot_misses"]}
             log(f"served[{name}] {rate:.0f} req/s: {rates[str(int(rate))]}")
         srv.close()
         out[name] = rates
+        ok = [(float(r), v) for r, v in rates.items() if v["p99_ms"] <= args.slo_ms]
+        best = max(ok, key=lambda kv: kv[1]["gpxd_per_s"]) if ok else None
+        out.setdefault("capacity_at_slo", {})[name] = (
+            {"rate": int(best[0]), "gpxd_per_s": best[1]["gpxd_per_s"],
+             "p99_ms": best[1]["p99_ms"]} if best else None)
     return out
 
 
@@ -332,11 +386,12 @@ def main():
 
     pxd = w * h * D
     value = pxd * args.steps / wall / 1e9
+    fmt_ran = {1: "u8", 2: "u16"}[sgm.volume_format_in_effect(D, cfg)]
     alg_bytes = w * h * (4 * D + 6)
     agg_bytes = w * h * (2 * D + 2)
     achieved = agg_bytes / (stage[1] * 1e-3) / 1e9
     result = {
-        "metric": "Gpx*disp/s @1280x720 D128 8-path (bit-exact)",
+        "metric": f"Gpx*disp/s @{w}x{h} D{D} {paths}-path",
         "value": round(value, 3),
         "unit": "Gpx*disp/s",
         "n_gpus": 1,
@@ -346,14 +401,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u16",
+        "dtype": "u16",   # the recurrence's arithmetic (mod 2^16); volumes stored per config
         "data": "synthetic",
         "config": {"workload": args.workload, "width": w, "height": h,
                    "num_disparities": D, "num_paths": paths, "p1": cfg.p1,
                    "p2": cfg.p2, "census": "9x7", "form": "plain per-pair",
                    "schedule": {"horizontal": list(plain_sched[0]),
                                 "other": list(plain_sched[1]), "split": plain_split},
-                   "volume_format": args.volume_format},
+                   "volume_format": args.volume_format, "volume_format_ran": fmt_ran},
         "roofline": {"bound": "hbm", "kernel": "paths_kernel (aggregation, all directions)",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -370,19 +425,34 @@ def main():
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as O
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        threads = args.cpu_threads or len(os.sched_getaffinity(0))
         t0 = time.perf_counter()
         want = O.sgm(left, right, D, p1=cfg.p1, p2=cfg.p2, num_paths=paths,
                      nthreads=threads)
         cpu_s = time.perf_counter() - t0
+        # One core on a bounded sample: the first rows of the same pair (every
+        # path still runs over whole scanlines of the cropped pair).
+        rows = max(16, min(h, int(h * min(1.0, 4.0 / max(cpu_s * threads, 1e-3)))))
+        t0 = time.perf_counter()
+        O.sgm(left[:rows], right[:rows], D, p1=cfg.p1, p2=cfg.p2, num_paths=paths,
+              nthreads=1)
+        cpu1_s = time.perf_counter() - t0
+        model = cpu_model()
         result["cpu_baseline"] = {
             "value": round(pxd / cpu_s / 1e9, 4), "unit": "Gpx*disp/s",
-            "cores": threads, "kind": "port",
+            "cores": threads, "kind": "port", "cpu_model": model,
             "sample": f"one full {w}x{h} D={D} {paths}-path pair, scalar C++ oracle, "
-                      f"{threads} threads over independent scanlines, {cpu_s:.2f} s"}
+                      f"{threads} threads over independent scanlines, {cpu_s:.2f} s",
+            "single_core": {"value": round(w * rows * D / cpu1_s / 1e9, 4),
+                            "unit": "Gpx*disp/s", "cores": 1,
+                            "sample": f"the first {rows} rows of the same pair "
+                                      f"({w}x{rows}, D={D}, {paths} paths), {cpu1_s:.2f} s"}}
         result["bit_exact"] = bool((gpu_disp.view(np.uint32) ==
                                     want.astype(np.float32).view(np.uint32)).all())
-        log(f"cpu baseline {cpu_s:.2f} s, bit_exact {result['bit_exact']}")
+        log(f"cpu baseline {cpu_s:.2f} s on {threads} threads, {cpu1_s:.2f} s on one core "
+            f"for {rows} rows ({model}); bit_exact {result['bit_exact']}")
+    if result.get("bit_exact"):
+        result["metric"] += " (bit-exact)"
     forms = set(args.forms.split(","))
     if "batched" in forms:
         log(f"batched leg ({args.batch_pairs} pairs)")

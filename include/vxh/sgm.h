@@ -114,9 +114,16 @@ uint32_t vxh_texture_format(const vxh_texture *tex);
  * len != w*h or a non-R32Float texture -> INVALID_CONFIG. */
 vxh_status vxh_texture_read_r32float(const vxh_texture *tex, float *dst,
                                      size_t len);
-/* Replaces the bytes of an existing gray8 texture (same size). */
+/* Replaces the bytes of an existing gray8 texture (same size).  Refused with
+ * VXH_ERR_INVALID_CONFIG while the texture is in flight: a committed or
+ * relaunched pipeline that encoded work on it has not finished (the
+ * reference's UnifiedBuffer refuses mutable access while in flight,
+ * vx-core/src/buffer.rs:59-72).  Reads of a texture wait for such work. */
 vxh_status vxh_texture_write_gray8(vxh_texture *tex, const uint8_t *pixels,
                                    size_t len);
+/* UnifiedBuffer::is_in_flight (vx-core/src/buffer.rs:89-92): 1 while a
+ * launched pipeline that used the texture has not finished, else 0. */
+int vxh_texture_in_flight(const vxh_texture *tex);
 /* HIP device ordinal the texture lives on (MTLTexture.device); -1 for NULL. */
 int vxh_texture_device(const vxh_texture *tex);
 uint32_t vxh_texture_width(const vxh_texture *tex);
@@ -195,6 +202,18 @@ vxh_status vxh_sgm_enqueue_marked(vxh_sgm *sgm, const uint8_t *d_left,
                                   void *d_workspace, size_t workspace_bytes,
                                   void *stream, void *const *events3);
 
+/* Status of the enqueued call that used `d_workspace`, once the caller has
+ * synchronised the stream it was enqueued on: VXH_ERR_GPU when a split
+ * scanline's bounded wait expired in that call (its map is then incomplete),
+ * else VXH_OK.  Reads the call's error word with a blocking copy.  Every
+ * enqueue clears the word first, so it describes the last call on the
+ * workspace.  No reference counterpart (the reference's compute_disparity
+ * waits with waitUntilCompleted and checks nothing, sgm.rs:260-261). */
+vxh_status vxh_sgm_workspace_status(vxh_sgm *sgm, const void *d_workspace);
+/* Byte offset of that error word (a uint32, non-zero on failure) in the
+ * workspace, for callers that copy it back in stream order themselves. */
+size_t vxh_sgm_workspace_error_offset(void);
+
 /* Stage-level entry points on device buffers, for parity tests of the
  * intermediate results.  d_census is 2*w*h uint32 (lo, hi per pixel);
  * d_volume is w*h*vxh_sgm_volume_stride(D) uint16, pixel-major with d
@@ -242,6 +261,13 @@ vxh_status vxh_sgm_set_schedule(vxh_sgm *sgm, uint32_t num_disparities,
  * either way while the longest chain of sequential steps halves.  No
  * reference counterpart. */
 vxh_status vxh_sgm_set_split(vxh_sgm *sgm, uint32_t num_disparities, uint32_t split);
+/* Polls a split scanline's first half makes of its second half's progress
+ * flag before it gives up, raises the call's error word (VXH_ERR_GPU from
+ * compute_disparity, vxh_sgm_workspace_status, pipeline waits) and goes on
+ * without the flag; 0xFFFFFFFF restores the default (seconds of polling).
+ * With 0 every split call reports the error: the way a caller exercises its
+ * error path.  No reference counterpart. */
+vxh_status vxh_sgm_set_split_wait(vxh_sgm *sgm, uint32_t polls);
 /* The schedule in effect for `num_disparities`: out5 = lanes_h, per_lane_h,
  * lanes, per_lane, split. */
 vxh_status vxh_sgm_get_schedule(vxh_sgm *sgm, uint32_t num_disparities,
@@ -255,6 +281,12 @@ vxh_status vxh_sgm_get_schedule(vxh_sgm *sgm, uint32_t num_disparities,
  * u16 volume, SGMStereo.metal:164-165). */
 vxh_status vxh_sgm_set_volume_format(vxh_sgm *sgm, uint32_t num_disparities,
                                      uint32_t format);
+
+/* The volume storage calls with `cfg` use on this handle for pairs with
+ * `num_disparities` disparities (cfg->num_disparities must equal it): 1 = u8,
+ * 2 = u16; 0 for invalid arguments. */
+uint32_t vxh_sgm_volume_format_in_effect(vxh_sgm *sgm, uint32_t num_disparities,
+                                         const vxh_sgm_config *cfg);
 
 /* Plain device-to-device copy kernel (16-byte vectors, grid-stride), used to
  * calibrate HBM traffic counters against a kernel of known traffic.  bytes

@@ -51,15 +51,25 @@ uint64_t vxh_server_request_bytes(uint32_t w, uint32_t h, const vxh_sgm_config *
 
 vxh_status vxh_server_create(int device, const vxh_server_config *cfg,
                              vxh_server **out);
-/* Queues a request.  left/right (w*h bytes) are copied before the call
- * returns; out (w*h floats) must stay valid until the request completes.
+/* Pre-allocates `count` idle device slots (and pinned staging buffers) sized
+ * for w x h pairs with `cfg`, so that the first requests of that shape (or any
+ * smaller one) find their buffers resident instead of allocating them.
+ * Slots are reused by capacity: any request of at most w*h pixels and at most
+ * that workspace takes one. */
+vxh_status vxh_server_reserve(vxh_server *srv, uint32_t w, uint32_t h,
+                              const vxh_sgm_config *cfg, uint32_t count);
+/* Queues a request.  left/right (w*h bytes) are copied into pinned staging
+ * memory before the call returns; out (w*h floats) must stay valid until the
+ * request completes.
  * A request whose volume alone exceeds the budget is rejected with
  * VXH_ERR_BUFFER_ALLOC; an invalid config with VXH_ERR_INVALID_CONFIG. */
 vxh_status vxh_server_submit(vxh_server *srv, const uint8_t *left,
                              const uint8_t *right, uint32_t w, uint32_t h,
                              const vxh_sgm_config *cfg, float *out,
                              uint64_t *request_id);
-/* Blocks until request `id` has completed; returns its latency in ms. */
+/* Blocks until request `id` has completed; returns its latency in ms.
+ * VXH_ERR_GPU when its device work failed, including an expired bounded wait
+ * of a split scanline (its map is then not written to out). */
 vxh_status vxh_server_wait(vxh_server *srv, uint64_t id, double *latency_ms);
 /* Blocks until every submitted request has completed. */
 vxh_status vxh_server_drain(vxh_server *srv);

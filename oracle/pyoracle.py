@@ -19,13 +19,6 @@ _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
 
 
-class HazardStats(ctypes.Structure):
-    _fields_ = [("steps", ctypes.c_uint64), ("updates", ctypes.c_uint64),
-                ("o1_wins", ctypes.c_uint64), ("wrap_updates", ctypes.c_uint64),
-                ("wrap_steps", ctypes.c_uint64), ("max_chain", ctypes.c_uint64),
-                ("chain_hist", ctypes.c_uint64 * 9)]
-
-
 def _sig(name, res, *args):
     f = getattr(lib, name)
     f.restype = res
@@ -46,8 +39,6 @@ _sig("oracle_sgm", ctypes.c_int, _vp, _vp, _u32, _u32, _u32, ctypes.c_float,
 _sig("oracle_sgm_volume", ctypes.c_int, _vp, _vp, _u32, _u32, _u32, ctypes.c_float,
      ctypes.c_float, _u32, _u32, _u32, _u32, _vp)
 _sig("oracle_scanline", None, _vp, _u32, _u32, _u32, _u32, _vp)
-_sig("oracle_hazard_dir", None, _vp, _vp, _u32, _u32, _u32, _u32, _u32,
-     ctypes.c_int, ctypes.c_int, ctypes.POINTER(HazardStats))
 
 
 _sig("oracle_depth_bilateral", None, _vp, _u32, _u32, ctypes.c_int, ctypes.c_float,
@@ -249,12 +240,3 @@ def scanline(costs, p1=10, p2=120):
     out = np.empty((steps, nd), dtype=np.uint16)
     lib.oracle_scanline(costs.ctypes.data, steps, nd, p1, p2, out.ctypes.data)
     return out
-
-
-def hazard_dir(lc, rc, nd, p1, p2, dx, dy):
-    h, w = lc.shape[:2]
-    st = HazardStats()
-    lib.oracle_hazard_dir(np.ascontiguousarray(lc).ctypes.data,
-                          np.ascontiguousarray(rc).ctypes.data, w, h, nd, p1, p2,
-                          dx, dy, ctypes.byref(st))
-    return st

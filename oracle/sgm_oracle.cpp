@@ -325,63 +325,4 @@ void oracle_scanline(const uint8_t *costs, uint32_t steps, uint32_t nd,
     }
 }
 
-void oracle_hazard_dir(const uint32_t *lc, const uint32_t *rc, uint32_t w,
-                       uint32_t h, uint32_t nd, uint32_t P1, uint32_t P2,
-                       int dx, int dy, oracle_hazard_stats *st) {
-    std::memset(st, 0, sizeof(*st));
-    std::vector<std::pair<int, int>> starts;
-    start_pixels(w, h, dx, dy, starts);
-    for (auto &sp : starts) {
-        uint16_t prev[256] = {0};
-        uint16_t pm = 0;
-        for (int step = 0;; ++step) {
-            int x = sp.first + step * dx, y = sp.second + step * dy;
-            if (x < 0 || x >= (int)w || y < 0 || y >= (int)h) break;
-            uint16_t cur = 0xFFFF;
-            bool wrapped = false;
-            uint64_t run = 0;
-            for (uint32_t d = 0; d < nd; ++d) {
-                uint16_t c = (uint16_t)oracle_cost(lc, rc, w, x, y, d), agg;
-                if (step == 0) {
-                    agg = c;
-                } else {
-                    uint32_t s1 = d > 0 ? prev[d - 1] + P1 : 0xFFFF;
-                    uint32_t s2 = d + 1 < nd ? prev[d + 1] + P1 : 0xFFFF;
-                    uint32_t s3 = pm + P2;
-                    uint16_t o0 = prev[d], o1 = (uint16_t)s1, o2 = (uint16_t)s2,
-                             o3 = (uint16_t)s3;
-                    uint16_t best = std::min(std::min(o0, o1), std::min(o2, o3));
-                    int32_t t = (int32_t)c + best - pm;
-                    agg = (uint16_t)t;
-                    bool wr = (d > 0 && s1 > 0xFFFF) || (d + 1 < nd && s2 > 0xFFFF) ||
-                              s3 > 0xFFFF || t < 0 || t > 0xFFFF;
-                    st->updates++;
-                    if (wr) { st->wrap_updates++; wrapped = true; }
-                    bool o1win = d > 0 && o1 < o0 && o1 < o2 && o1 < o3;
-                    if (o1win) {
-                        st->o1_wins++;
-                        run++;
-                    }
-                    if (!o1win || d + 1 == nd) {
-                        if (run) {
-                            st->max_chain = std::max(st->max_chain, run);
-                            int b = 0;
-                            while (b < 8 && (1ull << b) < run) ++b;
-                            st->chain_hist[b]++;
-                        }
-                        run = 0;
-                    }
-                }
-                prev[d] = agg;
-                cur = std::min(cur, agg);
-            }
-            if (step > 0) {
-                st->steps++;
-                if (wrapped) st->wrap_steps++;
-            }
-            pm = cur;
-        }
-    }
-}
-
 }  // extern "C"

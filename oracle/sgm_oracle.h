@@ -76,22 +76,6 @@ int oracle_sgm_volume(const uint8_t *left, const uint8_t *right, uint32_t w,
 void oracle_scanline(const uint8_t *costs, uint32_t steps, uint32_t nd,
                      uint32_t p1, uint32_t p2, uint16_t *out);
 
-/* Hazard statistics over one direction: counts of pixel-steps and (px,d)
- * updates in which the in-place o1 term won, or an arithmetic wrap occurred. */
-typedef struct {
-    uint64_t steps;          /* pixel-steps with step > 0                     */
-    uint64_t updates;        /* (pixel-step, d) updates with step > 0         */
-    uint64_t o1_wins;        /* updates where o1 was strictly the smallest    */
-    uint64_t wrap_updates;   /* updates with a wrap in o1/o2/o3 or in agg     */
-    uint64_t wrap_steps;     /* pixel-steps containing at least one wrap      */
-    uint64_t max_chain;      /* longest run of consecutive o1 wins over d     */
-    uint64_t chain_hist[9];  /* o1-win runs of length 1,2,3-4,...,>=129       */
-} oracle_hazard_stats;
-
-void oracle_hazard_dir(const uint32_t *lc, const uint32_t *rc, uint32_t w,
-                       uint32_t h, uint32_t nd, uint32_t p1, uint32_t p2,
-                       int dx, int dy, oracle_hazard_stats *st);
-
 /* Depth filters and disparity-to-depth (depth_oracle.cpp). */
 void oracle_depth_bilateral(const float *in, uint32_t w, uint32_t h, int radius, float ss,
                             float sd, float *out);

@@ -418,7 +418,12 @@ def test_720p_d128_8path_bit_exact(ctx, sgm):
         s.set_volume_format(128, "auto")
         s.set_schedule(128, ((32, 4), (64, 2)))
         assert_same_disp(s.compute_host(left, right, 1280, 720, cfg), want)
-        s.set_split(128, False)
+        # Split scanlines at full size (the tuner tries them on this pair).
+        s.set_split(128, True)
+        assert s.split(128)
+        assert_same_disp(s.compute_host(left, right, 1280, 720, cfg), want)
+        s.set_schedule(128, ((16, 8), (16, 8)))
+        assert s.split(128)
         assert_same_disp(s.compute_host(left, right, 1280, 720, cfg), want)
     finally:
         s.close()
@@ -436,6 +441,47 @@ def test_1080p_d256_8path_matches_golden_hash(ctx, sgm):
     try:
         s.set_volume_format(256, "u16")
         got = s.compute_host(left, right, 1920, 1080, cfg)
+        assert hashlib.sha256(bits(got).tobytes()).hexdigest() == golden["1080p_d256_8path"]
+        s.set_volume_format(256, "auto")
+        s.set_split(256, True)
+        got = s.compute_host(left, right, 1920, 1080, cfg)
+        assert hashlib.sha256(bits(got).tobytes()).hexdigest() == golden["1080p_d256_8path"]
     finally:
         s.close()
-    assert hashlib.sha256(bits(got).tobytes()).hexdigest() == golden["1080p_d256_8path"]
+
+
+def test_enqueue_reports_expired_split_wait(ctx):
+    """A device-resident caller learns of an expired split-scanline wait: with
+    the bounded wait forced to zero polls from the host (every head gives up at
+    once, no kernel waits at all), vxh_sgm_workspace_status reports Gpu after
+    the enqueue; with the default wait the same call is VXH_OK and exact, and
+    the next call clears the word."""
+    w, h, D = 600, 48, 64
+    left, right, _ = vxsgm.synth_pair(w, h, D, 4, 1, 9100)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=8)
+    want = O.sgm(left, right, D, num_paths=8)
+    ws = vxsgm.workspace_bytes(w, h, cfg)
+    assert vxsgm.lib.vxh_sgm_workspace_error_offset() == 4
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        s.set_split(D, True)
+        d_l = hip.DeviceBuffer.from_array(left)
+        d_r = hip.DeviceBuffer.from_array(right)
+        d_ws = hip.DeviceBuffer(ws)
+        out = hip.DeviceBuffer(w * h * 4)
+        st = hip.Stream()
+        s.set_split_wait(0)
+        s.enqueue(d_l.ptr, d_r.ptr, w, h, cfg, out.ptr, d_ws.ptr, ws, st.handle)
+        st.synchronize()
+        with pytest.raises(vxsgm.Gpu):
+            s.workspace_status(d_ws.ptr)
+        with pytest.raises(vxsgm.Gpu):
+            s.compute_host(left, right, w, h, cfg)
+        s.set_split_wait(None)
+        s.enqueue(d_l.ptr, d_r.ptr, w, h, cfg, out.ptr, d_ws.ptr, ws, st.handle)
+        st.synchronize()
+        s.workspace_status(d_ws.ptr)
+        assert_same_disp(out.download(np.float32), want)
+        st.close()
+    finally:
+        s.close()

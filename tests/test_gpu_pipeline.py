@@ -140,6 +140,49 @@ def test_relaunch_reads_current_contents(ctx, sgm):
         pipe.close()
 
 
+def test_write_refused_while_relaunch_in_flight(ctx, sgm):
+    """A host write into an input texture of a relaunched pipeline that has
+    not finished is refused, as UnifiedBuffer refuses mutable access while in
+    flight (vx-core/src/buffer.rs:59-72); reads wait for the pipeline; after
+    wait() the write goes through and the next relaunch reads it.  Five
+    relaunches of a 1920x1080 D=256 pair keep the stream busy for several
+    milliseconds, far longer than the host calls that follow them."""
+    w, h, D = 1920, 1080, 256
+    left, right, _ = vxsgm.synth_pair(w, h, D, 12, 2, 7700)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=8)
+    tl, tr = ctx.texture_gray8(left, w, h), ctx.texture_gray8(right, w, h)
+    out = ctx.texture_output_r32float(w, h)
+    pipe = vxsgm.Pipeline.begin(ctx)
+    try:
+        sgm.encode(pipe, tl, tr, out, cfg)
+        assert not tl.is_in_flight()          # encoded, not launched
+        pipe.commit_and_wait()
+        assert not tl.is_in_flight()
+        first = out.read_r32float()
+        for _ in range(5):
+            pipe.relaunch()
+        assert tl.is_in_flight() and out.is_in_flight()
+        with pytest.raises(vxsgm.InvalidConfig):
+            tl.write_gray8(right)
+        again = out.read_r32float()            # waits for the relaunches
+        assert not out.is_in_flight()
+        assert (bits(again) == bits(first)).all()
+        pipe.wait()
+        l2, r2, _ = vxsgm.synth_pair(w, h, D, 6, 0, 7701)
+        tl.write_gray8(l2)
+        tr.write_gray8(r2)
+        pipe.relaunch()
+        pipe.wait()
+        # The plain path on the new pair (pinned to the oracle by the parity
+        # tests at this size) is the expected map.
+        want = sgm.compute_host(l2, r2, w, h, cfg)
+        assert (bits(out.read_r32float()) == bits(want)).all()
+    finally:
+        pipe.close()
+        for t in (tl, tr, out):
+            t.close()
+
+
 def test_several_pairs_and_split_in_one_pipeline(ctx):
     """Three pairs of different sizes, one with split scanlines, each with its
     own scratch; then the same pipeline again reuses the pooled scratch."""

@@ -1,7 +1,11 @@
 """GPU checks of the serving loop: every request's map equals the plain path
 and the oracle whether it runs alone or in a mix with others in flight, and
 admission never holds more cost-volume bytes than the HBM budget."""
+import hashlib
+import json
+import os
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -96,3 +100,68 @@ def test_concurrent_clients():
     for k, (l, r, cfg) in enumerate(pairs):
         want = O.sgm(l, r, cfg.num_disparities, num_paths=8)
         assert (bits(results[k]) == bits(want.ravel())).all()
+
+
+def test_poisson_arrivals_full_mixture_match_oracle_hashes():
+    """The serving workload: Poisson arrivals at 1000 req/s of the 32-pair
+    request pool (the batched mixture's sizes and disparity ranges, 8 paths),
+    96 requests through a server with a 64 GB budget.  Requests overlap on the
+    device, and every map's SHA-256 equals the oracle's for its pair
+    (tests/golden/mixture_hashes.json), so each result is the same alone and in
+    the mix."""
+    with open(os.path.join(os.path.dirname(__file__), "golden", "mixture_hashes.json")) as f:
+        g = json.load(f)["serve_pool_32"]
+    rows = [(r["w"], r["h"], r["d"], r["seed"], r["sha256"]) for r in g["pairs"]]
+    assert [r[:4] for r in rows] == [tuple(m) for m in vxsgm.mixture(len(rows), seed=g["seed"])]
+    pool = [vxsgm.mixture_pair(w, h, d, s) for w, h, d, s, _ in rows]
+    srv = vxsgm.SGMServer(hbm_budget_bytes=64 << 30, max_inflight=32, num_streams=4)
+    try:
+        rng = np.random.default_rng(1000)
+        gaps = rng.exponential(1.0 / 1000.0, 96)
+        outs, ids = [], []
+        t = time.perf_counter()
+        for i in range(96):
+            t += gaps[i]
+            while time.perf_counter() < t:
+                pass
+            l, r, cfg = pool[i % len(pool)]
+            h, w = l.shape
+            out = np.full(w * h, np.nan, np.float32)
+            ids.append(srv.submit(l, r, w, h, cfg, out))
+            outs.append(out)
+        for rid in ids:
+            srv.wait(rid)
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["completed"] == 96 and st["peak_inflight"] > 1
+    assert st["peak_admitted_bytes"] <= 64 << 30
+    bad = [i for i, o in enumerate(outs) if hashlib.sha256(bits(o).tobytes()).hexdigest()
+           != rows[i % len(rows)][4]]
+    assert not bad, f"requests {bad} differ from the oracle"
+
+
+def test_reserved_slots_serve_smaller_requests():
+    """Slots reserved for the largest shape serve every smaller request
+    without allocating (all hits), and the results stay exact."""
+    pairs = small_mix(6, seed=31)
+    srv = vxsgm.SGMServer(hbm_budget_bytes=1 << 30, max_inflight=4, num_streams=2)
+    try:
+        cfg = vxsgm.SGMStereoConfig(num_disparities=max(p[2].num_disparities for p in pairs),
+                                    num_paths=8)
+        w = max(p[0].shape[1] for p in pairs)
+        h = max(p[0].shape[0] for p in pairs)
+        srv.reserve(w, h, cfg, 4)
+        outs = []
+        for l, r, c in pairs:
+            out = np.empty(l.size, np.float32)
+            srv.submit(l, r, l.shape[1], l.shape[0], c, out)
+            outs.append(out)
+        srv.drain()
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["slot_misses"] == 0 and st["slot_hits"] == len(pairs)
+    for (l, r, c), out in zip(pairs, outs):
+        want = O.sgm(l, r, c.num_disparities, num_paths=8)
+        assert (bits(out) == bits(want.ravel())).all()

@@ -2,6 +2,10 @@
 per-pair pipelines over several streams) equals the plain per-pair path and
 the oracle bit for bit, for mixed sizes, disparity ranges, path counts,
 schedules, split settings, stream counts and repeated graph replays."""
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -11,6 +15,22 @@ from sgm_cases import random_pair
 from vxsgm import hip
 
 pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def golden_mixture(name):
+    """(w, h, D, seed, sha256) rows of a mixture in tests/golden/mixture_hashes.json,
+    checked to be the draw vxsgm.mixture makes with the recorded seed."""
+    with open(os.path.join(GOLDEN, "mixture_hashes.json")) as f:
+        g = json.load(f)[name]
+    rows = [(r["w"], r["h"], r["d"], r["seed"], r["sha256"]) for r in g["pairs"]]
+    assert [r[:4] for r in rows] == [tuple(m) for m in vxsgm.mixture(len(rows), seed=g["seed"])]
+    return rows
+
+
+def map_sha256(a):
+    return hashlib.sha256(np.ascontiguousarray(a, np.float32).view(np.uint32).tobytes()).hexdigest()
 
 
 def run_batch(pairs, scheds=None, replays=1, formats=None, splits=None, options=None):
@@ -149,3 +169,17 @@ def test_batch_schedules_and_splits_per_pair():
             want = O.sgm(l, r, cfg.num_disparities, num_paths=cfg.num_paths,
                          lr_check=cfg.lr_check).ravel()
             assert (bits(got) == bits(want)).all()
+
+
+def test_batch_full_mixture_matches_oracle_hashes():
+    """The batched workload at full size: all 64 pairs of the mixture
+    ({320x240, 640x480, 1280x720, 1920x1080} x D {64, 128, 256}, 8 paths) in
+    one batch, replayed twice; every map's SHA-256 equals the oracle's
+    (tests/golden/mixture_hashes.json)."""
+    rows = golden_mixture("batch_64")
+    pairs = [vxsgm.mixture_pair(w, h, d, seed) for w, h, d, seed, _ in rows]
+    outs = run_batch(pairs, replays=2)
+    bad = [f"replay {k} pair {i} ({rows[i][0]}x{rows[i][1]} D={rows[i][2]})"
+           for k, rep in enumerate(outs) for i, got in enumerate(rep)
+           if map_sha256(got) != rows[i][4]]
+    assert not bad, "maps differ from the oracle: " + ", ".join(bad)

@@ -8,6 +8,7 @@ shared library that exports the C ABI declared in include/vxh/*.h.
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -16,8 +17,9 @@ OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "vxsgm", "libvxsgm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["sgm_kernels.hip", "sgm_runtime.hip", "sgm_batch.hip", "sgm_server.cpp",
-               "depth.hip", "pipeline.hip", "rectify.hip", "datasets.cpp", "stereo_match.hip"]
+HIP_SOURCES = ["sgm_paths_c0.hip", "sgm_paths_c1.hip", "sgm_paths_c2.hip", "sgm_kernels.hip",
+               "sgm_runtime.hip", "sgm_batch.hip", "sgm_server.cpp", "depth.hip", "pipeline.hip",
+               "rectify.hip", "datasets.cpp", "stereo_match.hip"]
 CXX_SOURCES = ["synth.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I", os.path.join(ROOT, "include"),
@@ -33,18 +35,24 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose=True):
+def build(verbose=True, jobs=None):
     os.makedirs(OBJ, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in HIP_SOURCES:
         o = os.path.join(OBJ, src + ".o")
-        _run([HIPCC, "-x", "hip", *HIP_FLAGS, *COMMON, "-c",
-              os.path.join(CSRC, src), "-o", o])
+        cmds.append([HIPCC, "-x", "hip", *HIP_FLAGS, *COMMON, "-c",
+                     os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     for src in CXX_SOURCES:
         o = os.path.join(OBJ, src + ".o")
-        _run(["g++", *COMMON, "-c", os.path.join(CSRC, src), "-o", o])
+        cmds.append(["g++", *COMMON, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
+    # Independent translation units compile side by side (at most 8 at once:
+    # each hipcc instance holds a few GB while optimising the kernels).
+    jobs = jobs or min(8, os.cpu_count() or 1, len(cmds))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = OUT + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"])
     os.replace(tmp, OUT)

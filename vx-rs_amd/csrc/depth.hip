@@ -311,6 +311,8 @@ vxh_status vxh_depth_filter_encode_bilateral(vxh_depth_filter *f, vxh_pipeline *
     if (st != VXH_OK) return st;
     hipStream_t s = nullptr;
     if ((st = vxh::pipeline_stream(p, &s)) != VXH_OK) return st;
+    vxh::pipeline_use(p, input);
+    vxh::pipeline_use(p, output);
     return vxh_depth_bilateral_enqueue((const float *)vxh_texture_device_ptr(input),
                                        (float *)vxh_texture_device_ptr(output),
                                        vxh_texture_width(input), vxh_texture_height(input), cfg,
@@ -325,6 +327,8 @@ vxh_status vxh_depth_filter_encode_median(vxh_depth_filter *f, vxh_pipeline *p,
     if (st != VXH_OK) return st;
     hipStream_t s = nullptr;
     if ((st = vxh::pipeline_stream(p, &s)) != VXH_OK) return st;
+    vxh::pipeline_use(p, input);
+    vxh::pipeline_use(p, output);
     return vxh_depth_median_enqueue((const float *)vxh_texture_device_ptr(input),
                                     (float *)vxh_texture_device_ptr(output),
                                     vxh_texture_width(input), vxh_texture_height(input), cfg, s);

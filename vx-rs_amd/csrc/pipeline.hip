@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -52,6 +53,9 @@ struct vxh_pipeline {
     uint32_t *d_error = nullptr;       // the start of scratch[0]
     std::vector<void *> scratch;       // context pool blocks, held until destroy
     std::vector<vxh_texture *> retained;
+    // Recorded on the stream after every launch of the graph (the textures the
+    // encodes used are in flight until it completes); null before the first.
+    std::atomic<hipEvent_t> done{nullptr};
 };
 
 namespace vxh {
@@ -79,6 +83,8 @@ vxh_status pipeline_watch_word(vxh_pipeline *p, const uint32_t *d_word) {
 
 vxh_context *pipeline_context(vxh_pipeline *p) { return p ? p->ctx : nullptr; }
 
+hipEvent_t pipeline_done_event(vxh_pipeline *p) { return p ? p->done.load() : nullptr; }
+
 }  // namespace vxh
 
 extern "C" {
@@ -98,6 +104,8 @@ void vxh_pipeline_destroy(vxh_pipeline *p) {
     }
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
+    vxh::pipeline_forget(p);
+    if (hipEvent_t ev = p->done.exchange(nullptr)) (void)hipEventDestroy(ev);
     for (void *b : p->scratch) vxh::ctx_give_block(p->ctx, b);
     if (p->stream) vxh::ctx_give_stream(p->ctx, p->stream);
     for (vxh_texture *t : p->retained) vxh_texture_destroy(t);
@@ -176,9 +184,15 @@ vxh_status vxh_pipeline_commit(vxh_pipeline *p) {
         p->exec = nullptr;
         return gpu_error("pipeline graph", e);
     }
+    hipEvent_t ev = nullptr;
+    if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
+        return gpu_error("pipeline event", e);
+    p->done.store(ev);
     if ((e = hipGraphLaunch(p->exec, p->stream)) != hipSuccess)
         return gpu_error("pipeline launch", e);
     p->launched = true;
+    if ((e = hipEventRecord(ev, p->stream)) != hipSuccess)
+        return gpu_error("pipeline event", e);
     return VXH_OK;
 }
 
@@ -209,7 +223,8 @@ vxh_status vxh_pipeline_relaunch(vxh_pipeline *p) {
     if (!p->launched) return invalid("pipeline not committed");
     if (hipSetDevice(p->device) != hipSuccess)
         return vxh::report_error(VXH_ERR_DEVICE_NOT_FOUND, "no HIP device found");
-    const hipError_t e = hipGraphLaunch(p->exec, p->stream);
+    hipError_t e = hipGraphLaunch(p->exec, p->stream);
+    if (e == hipSuccess) e = hipEventRecord(p->done.load(), p->stream);
     return e == hipSuccess ? VXH_OK : gpu_error("pipeline launch", e);
 }
 

@@ -287,6 +287,8 @@ vxh_status vxh_undistorter_encode(vxh_undistorter *u, vxh_pipeline *p, const vxh
     if (v != VXH_OK) return v;
     hipStream_t s = nullptr;
     if ((v = vxh::pipeline_stream(p, &s)) != VXH_OK) return v;
+    for (const vxh_texture *t : {input, map_x, map_y, (const vxh_texture *)output})
+        vxh::pipeline_use(p, t);
     return remap_textures(input, map_x, map_y, output, s);
 }
 

@@ -194,9 +194,8 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
             what = vxh_last_error();
             break;
         }
-        const WsLayout L = workspace_layout(q.width, q.height, q.cfg.num_disparities,
-                                            (int)q.cfg.num_paths, p.mode, q.cfg.lr_check != 0);
-        e = launch_or_word(b->d_error, (const uint32_t *)(ws + L.sync) + 1, b->sub[p.stream]);
+        e = launch_or_word(b->d_error, (const uint32_t *)(ws + kErrorWordOffset),
+                           b->sub[p.stream]);
         b->launches += 6 + (q.cfg.lr_check ? 1 : 0);
     }
     for (uint32_t s = 0; s < streams && e == hipSuccess && st == VXH_OK; ++s) {

@@ -43,6 +43,7 @@ struct PassParams {
     uint32_t xqx2;         // P1 - the cost volume's bias, in both halves
     int dir, dx, dy, nscan;
     int mode;              // kOutU8 / kOutU16 (read only where the kernel takes it at run time)
+    uint32_t wait_polls;   // split heads: polls of a tail's flag before the error word is set
 };
 
 // Lane layouts of the aggregation: layout class c covers D <= 64 << c with
@@ -60,10 +61,14 @@ inline int layout_class(int D) { return D <= 64 ? 0 : (D <= 128 ? 1 : 2); }
 
 // Lane layout of one direction class of a pair: horizontal passes (the
 // longest scanlines) and the other directions are laid out separately.
+// Polls of a head's bounded wait for its tail's flag before it gives up and
+// raises the error word (seconds of polling).
+constexpr uint32_t kSplitWaitPolls = 1u << 22;
 struct Schedule {
     int horiz;   // index into kClassLayouts[layout_class(D)]
     int other;
     int split;   // 1: long scanlines run as head and tail halves at once
+    uint32_t wait_polls = kSplitWaitPolls;   // the heads' bounded wait
 };
 
 // Words of the split launch's synchronisation area for a w x h pair: an
@@ -78,27 +83,29 @@ inline size_t sync_words(uint32_t w, uint32_t h) {
 constexpr size_t kWsAlign = 256;
 inline size_t ws_align(size_t v) { return (v + kWsAlign - 1) & ~(kWsAlign - 1); }
 
-// Scratch of one pair, byte offsets from its base: census of both images, the
+// Scratch of one pair, byte offsets from its base: the synchronisation words
+// first (word 1 is the pair's error word, so its offset never depends on the
+// format: kErrorWordOffset), then the census of both images, the
 // matching-cost volume (u8, pix*Dp + d), the exception flags (u8 volumes
-// only, 8 bytes per pixel), the split launch's synchronisation words, one
-// volume per path direction (u8 or u16 entries, vol_bytes apart), the
-// high-byte volumes (u8 only, also vol_bytes apart), and for the LR check the
-// summed u16 volume and the winner index.
+// only, 8 bytes per pixel), one volume per path direction (u8 or u16
+// entries, vol_bytes apart), the high-byte volumes (u8 only, also vol_bytes
+// apart), and for the LR check the summed u16 volume and the winner index.
 struct WsLayout {
     size_t lc, rc, cost, flags, sync, vol, vol_bytes, hi, sum, best, total;
 };
+constexpr size_t kErrorWordOffset = 4;
 inline WsLayout workspace_layout(uint32_t w, uint32_t h, uint32_t D, int paths, int mode,
                                  bool lr) {
     WsLayout l;
     const size_t n = (size_t)w * h;
     const size_t Dp = (size_t)volume_stride((int)D);
     const bool u8 = mode == kOutU8;
-    l.lc = 0;
+    l.sync = 0;
+    l.lc = l.sync + ws_align(sync_words(w, h) * 4);
     l.rc = l.lc + ws_align(n * 8);
     l.cost = l.rc + ws_align(n * 8);
     l.flags = l.cost + ws_align(n * Dp);
-    l.sync = l.flags + (u8 ? ws_align(n * 8) : 0);
-    l.vol = l.sync + ws_align(sync_words(w, h) * 4);
+    l.vol = l.flags + (u8 ? ws_align(n * 8) : 0);
     l.vol_bytes = ws_align(n * Dp * (u8 ? 1 : 2));
     l.hi = l.vol + (size_t)paths * l.vol_bytes;
     l.sum = l.hi + (u8 ? (size_t)paths * l.vol_bytes : 0);
@@ -163,6 +170,12 @@ hipError_t launch_zero(uint32_t *p, size_t n, hipStream_t s);
 // Default schedule for D disparities.
 Schedule default_schedule(int D);
 bool schedule_valid(Schedule sc, int D);
+// The aggregation launchers of layout classes 0, 1 and 2 (one translation
+// unit each): paths_split_kernel when `split`, else paths_kernel, in storage
+// `mode`, over `blocks` blocks of 256 threads.
+hipError_t launch_paths_c0(int mode, bool split, const MultiPass &m, int blocks, hipStream_t s);
+hipError_t launch_paths_c1(int mode, bool split, const MultiPass &m, int blocks, hipStream_t s);
+hipError_t launch_paths_c2(int mode, bool split, const MultiPass &m, int blocks, hipStream_t s);
 // Runs the passes dirs[0..n) at once, pass j into vols[j] (and his[j] for
 // kOutU8).  base supplies the shared fields; `sync` (sync_words words,
 // cleared beforehand) enables the split launch.

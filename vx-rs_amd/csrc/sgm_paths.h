@@ -138,10 +138,12 @@ __device__ __forceinline__ int split_point(int len) {
 // Roles of a wave in a split launch.
 constexpr int kRoleWhole = 0, kRoleHead = 1, kRoleTail = 2;
 
-// Bounded wait for one of a tail's flags (seconds of polling): a head that
-// times out raises *error and treats the flag as set.
-__device__ __forceinline__ void wait_flag(const uint32_t *flag, uint32_t *error) {
-    for (uint32_t n = 0; n < (1u << 22); ++n) {
+// Bounded wait for one of a tail's flags (`polls` polls, seconds of polling
+// by default): a head that times out raises *error and treats the flag as
+// set.
+__device__ __forceinline__ void wait_flag(const uint32_t *flag, uint32_t *error,
+                                          uint32_t polls) {
+    for (uint32_t n = 0; n < polls; ++n) {
         const uint32_t v = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if ((uint32_t)__builtin_amdgcn_readfirstlane((int)v) != 0) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -312,8 +314,8 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
     // tail has finished; other lanes and steps read the sink).
     auto issue_tail = [&](TailBuf<K> &tb, int t) {
         if (t < vfirst) return;   // wave-uniform: only the checking steps load
-        if (t == vfirst) wait_flag(tail_early, error);
-        if (t == vfirst + kRange) wait_flag(tail_done, error);
+        if (t == vfirst) wait_flag(tail_early, error, p.wait_polls);
+        if (t == vfirst + kRange) wait_flag(tail_done, error, p.wait_polls);
 #pragma unroll
         for (int hh = 0; hh < 2; ++hh) {
             const bool v = t >= vfrom[hh] && t < len[hh] && lane_store;

@@ -56,6 +56,18 @@ vxh_status pipeline_scratch(vxh_pipeline *p, size_t bytes, void **ptr);
 vxh_status pipeline_watch_word(vxh_pipeline *p, const uint32_t *d_word);
 vxh_context *pipeline_context(vxh_pipeline *p);
 
+// Textures in flight (the reference's GpuGuard, vx-core/src/buffer.rs:59-66,
+// 102-122): every encode records the textures it reads or writes
+// (pipeline_use).  While a pipeline that used a texture has launched work that
+// has not finished (its done event, recorded after each commit and relaunch,
+// is pending), a host write into the texture is refused (INVALID_CONFIG) and
+// a host read waits for that work.  pipeline_forget drops a destroyed
+// pipeline's records.
+void pipeline_use(vxh_pipeline *p, const vxh_texture *t);
+void pipeline_forget(vxh_pipeline *p);
+// The pipeline's done event (nullptr before its first launch).
+hipEvent_t pipeline_done_event(vxh_pipeline *p);
+
 // Count of pipelines between begin and commit in the process (texture copies
 // avoid the legacy null stream while it is non-zero).
 void capture_opened();

@@ -127,12 +127,70 @@ struct vxh_sgm {
     vxh::Schedule sched[257] = {};   // per D, used when sched_set[D]
     bool sched_set[257] = {};
     uint8_t format[257] = {};        // per D: 0 auto, 1 u8, 2 u16
+    std::atomic<uint32_t> wait_polls{vxh::kSplitWaitPolls};   // split heads' bounded wait
 };
+
+namespace {
+
+// (texture, pipeline) pairs recorded by encodes, process-wide.
+std::mutex g_use_mu;
+std::vector<std::pair<const vxh_texture *, vxh_pipeline *>> g_uses;
+
+// In-flight state of `t` for a host copy: VXH_OK when no pipeline that used it
+// has launched work still running; otherwise, with `wait`, waits for that work
+// first, else refuses with INVALID_CONFIG.
+std::vector<hipEvent_t> tex_pending(const vxh_texture *t) {
+    std::vector<hipEvent_t> pending;
+    std::lock_guard<std::mutex> g(g_use_mu);
+    for (const auto &u : g_uses) {
+        if (u.first != t) continue;
+        hipEvent_t ev = vxh::pipeline_done_event(u.second);
+        if (ev && hipEventQuery(ev) == hipErrorNotReady) pending.push_back(ev);
+    }
+    return pending;
+}
+
+vxh_status tex_settle(const vxh_texture *t, bool wait) {
+    const std::vector<hipEvent_t> pending = tex_pending(t);
+    if (pending.empty()) return VXH_OK;
+    if (!wait)
+        return fail(VXH_ERR_INVALID_CONFIG,
+                    "invalid config: texture is in flight on the GPU (a launched pipeline that "
+                    "uses it has not finished; wait for it first)");
+    for (hipEvent_t ev : pending) {
+        const hipError_t e = hipEventSynchronize(ev);
+        if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+    }
+    return VXH_OK;
+}
+
+void tex_forget(const vxh_texture *t) {
+    std::lock_guard<std::mutex> g(g_use_mu);
+    g_uses.erase(std::remove_if(g_uses.begin(), g_uses.end(),
+                                [&](const auto &u) { return u.first == t; }),
+                 g_uses.end());
+}
+
+}  // namespace
 
 namespace vxh {
 
 void capture_opened() { g_open_captures.fetch_add(1); }
 void capture_closed() { g_open_captures.fetch_sub(1); }
+
+void pipeline_use(vxh_pipeline *p, const vxh_texture *t) {
+    std::lock_guard<std::mutex> g(g_use_mu);
+    for (const auto &u : g_uses)
+        if (u.first == t && u.second == p) return;
+    g_uses.emplace_back(t, p);
+}
+
+void pipeline_forget(vxh_pipeline *p) {
+    std::lock_guard<std::mutex> g(g_use_mu);
+    g_uses.erase(std::remove_if(g_uses.begin(), g_uses.end(),
+                                [&](const auto &u) { return u.second == p; }),
+                 g_uses.end());
+}
 
 }  // namespace vxh
 
@@ -157,7 +215,9 @@ static vxh_status copy_blocking(vxh_context *ctx, void *dst, const void *src, si
 
 static vxh::Schedule schedule_for(vxh_sgm *s, int D) {
     std::lock_guard<std::mutex> g(s->mu);
-    return s->sched_set[D] ? s->sched[D] : vxh::default_schedule(D);
+    vxh::Schedule sc = s->sched_set[D] ? s->sched[D] : vxh::default_schedule(D);
+    sc.wait_polls = s->wait_polls.load();
+    return sc;
 }
 
 static int mode_for(vxh_sgm *s, const vxh_sgm_config *c) {
@@ -389,17 +449,25 @@ vxh_status vxh_texture_r32float(vxh_context *ctx, const float *data, size_t len,
 
 uint32_t vxh_texture_format(const vxh_texture *t) { return t ? (uint32_t)t->format : 0u; }
 
+// UnifiedBuffer::write refuses while in flight (vx-core/src/buffer.rs:59-72):
+// so does a write into a texture that a launched, unfinished pipeline uses.
 vxh_status vxh_texture_write_gray8(vxh_texture *t, const uint8_t *pixels, size_t len) {
-    if (!t || t->format != kGray8 || len != (size_t)t->w * t->h)
+    if (!t || !pixels || t->format != kGray8 || len != (size_t)t->w * t->h)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: gray8 write size");
+    vxh_status v = tex_settle(t, false);
+    if (v != VXH_OK) return v;
     return copy_blocking(t->ctx, t->dptr, pixels, len, hipMemcpyHostToDevice);
 }
 
+// Reads wait for the launched pipelines that use the texture, so they return
+// what those pipelines wrote.
 vxh_status vxh_texture_read_r32float(const vxh_texture *t, float *dst, size_t len) {
     if (!t || t->format != kR32F)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: not an R32Float texture");
     if (len != (size_t)t->w * t->h)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: read length");
+    vxh_status v = tex_settle(t, true);
+    if (v != VXH_OK) return v;
     return copy_blocking(t->ctx, dst, t->dptr, len * 4, hipMemcpyDeviceToHost);
 }
 
@@ -408,8 +476,12 @@ static vxh_status tex_read_bytes(const vxh_texture *t, int fmt, uint8_t *dst, si
     if (!t || t->format != fmt || !dst)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: texture format");
     if (len != t->bytes) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: read length");
+    vxh_status v = tex_settle(t, true);
+    if (v != VXH_OK) return v;
     return copy_blocking(t->ctx, dst, t->dptr, len, hipMemcpyDeviceToHost);
 }
+
+int vxh_texture_in_flight(const vxh_texture *t) { return t && !tex_pending(t).empty() ? 1 : 0; }
 
 vxh_status vxh_texture_read_gray8(const vxh_texture *t, uint8_t *dst, size_t len) {
     return tex_read_bytes(t, kGray8, dst, len);
@@ -426,6 +498,7 @@ void *vxh_texture_device_ptr(const vxh_texture *t) { return t ? t->dptr : nullpt
 
 void vxh_texture_destroy(vxh_texture *t) {
     if (!t) return;
+    tex_forget(t);
     vxh::CaptureRelaxed relaxed;
     (void)hipSetDevice(t->ctx->device);
     (void)hipFree(t->dptr);
@@ -485,6 +558,7 @@ static vxh_status paths_and_wta(const uint8_t *cost, uint32_t w, uint32_t h,
     base.p2x2 = p2 | (p2 << 16);
     const uint32_t xq = p1 - vxh::cost_bias(p1);
     base.xqx2 = xq | (xq << 16);
+    base.wait_polls = sc.wait_polls;
     int dirs[vxh::kMaxPasses];
     uint8_t *vols[vxh::kMaxPasses], *his[vxh::kMaxPasses];
     for (int i = 0; i < P; ++i) {
@@ -649,6 +723,18 @@ vxh_status vxh_sgm_enqueue_marked(vxh_sgm *s, const uint8_t *d_left, const uint8
                         (hipStream_t)stream, (hipEvent_t const *)events3);
 }
 
+size_t vxh_sgm_workspace_error_offset(void) { return vxh::kErrorWordOffset; }
+
+vxh_status vxh_sgm_workspace_status(vxh_sgm *s, const void *d_ws) {
+    if (!s || !d_ws) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null argument");
+    uint32_t err = 0;
+    vxh_status v = copy_blocking(s->ctx, &err, (const char *)d_ws + vxh::kErrorWordOffset, 4,
+                                 hipMemcpyDeviceToHost);
+    if (v != VXH_OK) return v;
+    if (err) return fail(VXH_ERR_GPU, "GPU error: a split scanline's wait timed out");
+    return VXH_OK;
+}
+
 static vxh_status acquire_slot(vxh_sgm *s, size_t bytes, Slot **out) {
     Slot *sl = nullptr;
     {
@@ -725,7 +811,8 @@ vxh_status vxh_sgm_encode(vxh_sgm *s, vxh_pipeline *p, const vxh_texture *left,
                           schedule_for(s, (int)c->num_disparities), st);
     if (v != VXH_OK) return v;
     // The split launch's error word (a head whose wait for its tail timed out).
-    return vxh::pipeline_watch_word(p, (const uint32_t *)((char *)ws + L.sync) + 1);
+    for (const vxh_texture *t : {left, right, (const vxh_texture *)out}) vxh::pipeline_use(p, t);
+    return vxh::pipeline_watch_word(p, (const uint32_t *)((char *)ws + vxh::kErrorWordOffset));
 }
 
 vxh_status vxh_sgm_compute_disparity(vxh_sgm *s, vxh_context *ctx,
@@ -748,12 +835,9 @@ vxh_status vxh_sgm_compute_disparity(vxh_sgm *s, vxh_context *ctx,
     hipError_t e = hipStreamSynchronize(sl->stream);
     // The split launch's error word: a head whose wait for its tail timed out.
     uint32_t err = 0;
-    if (v == VXH_OK && e == hipSuccess) {
-        const WsLayout L = workspace_layout(w, h, c->num_disparities, (int)c->num_paths,
-                                            mode_for(s, c), c->lr_check != 0);
-        e = hipMemcpyWithStream(&err, (char *)sl->ws + L.sync + 4, 4, hipMemcpyDeviceToHost,
-                                sl->stream);
-    }
+    if (v == VXH_OK && e == hipSuccess)
+        e = hipMemcpyWithStream(&err, (char *)sl->ws + vxh::kErrorWordOffset, 4,
+                                hipMemcpyDeviceToHost, sl->stream);
     release_slot(s, sl);
     if (v != VXH_OK) return v;
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
@@ -856,6 +940,17 @@ vxh_status vxh_sgm_get_schedule(vxh_sgm *s, uint32_t nd, uint32_t *out4) {  // o
     out4[2] = (uint32_t)vxh::kClassLayouts[c][sc.other][0];
     out4[3] = (uint32_t)vxh::kClassLayouts[c][sc.other][1];
     out4[4] = (uint32_t)sc.split;
+    return VXH_OK;
+}
+
+uint32_t vxh_sgm_volume_format_in_effect(vxh_sgm *s, uint32_t nd, const vxh_sgm_config *c) {
+    if (!s || !c || nd < 1 || nd > 256 || c->num_disparities != nd) return 0;
+    return mode_for(s, c) == vxh::kOutU8 ? 1u : 2u;
+}
+
+vxh_status vxh_sgm_set_split_wait(vxh_sgm *s, uint32_t polls) {
+    if (!s) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null handle");
+    s->wait_polls.store(polls == 0xFFFFFFFFu ? vxh::kSplitWaitPolls : polls);
     return VXH_OK;
 }
 

@@ -98,6 +98,11 @@ _sig("vxh_sgm_enqueue", ctypes.c_int, _vp, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp
      _sz, _vp)
 _sig("vxh_sgm_enqueue_marked", ctypes.c_int, _vp, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp,
      _sz, _vp, ctypes.POINTER(_vp))
+_sig("vxh_sgm_workspace_status", ctypes.c_int, _vp, _vp)
+_sig("vxh_sgm_set_split_wait", ctypes.c_int, _vp, _u32)
+_sig("vxh_sgm_volume_format_in_effect", _u32, _vp, _u32, _cfgp)
+_sig("vxh_sgm_workspace_error_offset", _sz)
+_sig("vxh_texture_in_flight", ctypes.c_int, _vp)
 _sig("vxh_sgm_census", ctypes.c_int, _vp, _u32, _u32, _u32, _u32, _vp, _vp)
 _sig("vxh_sgm_aggregate", ctypes.c_int, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp)
 _sig("vxh_sgm_wta", ctypes.c_int, _vp, _u32, _u32, _cfgp, _vp, _vp)
@@ -242,8 +247,14 @@ class Texture:
         return h
 
     def write_gray8(self, pixels):
+        """Refused (InvalidConfig) while a launched pipeline that uses the
+        texture has not finished (UnifiedBuffer::write, vx-core/src/buffer.rs:68-72)."""
         p = _as_u8(pixels).ravel()
         check(lib.vxh_texture_write_gray8(self._h, p.ctypes.data, p.size))
+
+    def is_in_flight(self):
+        """UnifiedBuffer::is_in_flight (vx-core/src/buffer.rs:89-92)."""
+        return bool(lib.vxh_texture_in_flight(self._h))
 
     def close(self):
         if self._h:
@@ -424,6 +435,21 @@ class SGMStereo:
                                   ctypes.byref(c), d_out, d_workspace,
                                   workspace_bytes, stream))
 
+    def volume_format_in_effect(self, num_disparities, config):
+        """1 (u8) or 2 (u16): the per-direction volume storage a call with this
+        config uses on this handle."""
+        c = config.to_c()
+        return lib.vxh_sgm_volume_format_in_effect(self._h, num_disparities, ctypes.byref(c))
+
+    def set_split_wait(self, polls=None):
+        """Polls of a split head's bounded wait (None: the default)."""
+        check(lib.vxh_sgm_set_split_wait(self._h, 0xFFFFFFFF if polls is None else polls))
+
+    def workspace_status(self, d_workspace):
+        """After the stream of an enqueue on d_workspace has drained: raises Gpu
+        when a split scanline's bounded wait expired in that call."""
+        check(lib.vxh_sgm_workspace_status(self._h, d_workspace))
+
     def enqueue_marked(self, d_left, d_right, width, height, config, d_out, d_workspace,
                        workspace_bytes, stream, events):
         """enqueue recording events[0..2] after census, aggregation and winner."""
@@ -550,6 +576,7 @@ class vxh_server_stats(ctypes.Structure):
 _sig("vxh_server_request_bytes", ctypes.c_uint64, _u32, _u32, _cfgp)
 _sig("vxh_server_create", ctypes.c_int, ctypes.c_int, ctypes.POINTER(vxh_server_config),
      ctypes.POINTER(_vp))
+_sig("vxh_server_reserve", ctypes.c_int, _vp, _u32, _u32, _cfgp, _u32)
 _sig("vxh_server_submit", ctypes.c_int, _vp, _vp, _vp, _u32, _u32, _cfgp, _vp,
      ctypes.POINTER(ctypes.c_uint64))
 _sig("vxh_server_wait", ctypes.c_int, _vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double))
@@ -580,6 +607,11 @@ class SGMServer:
         _check_server(lib.vxh_server_create(device, ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
         self._keep = {}
+
+    def reserve(self, width, height, config, count=1):
+        """Pre-allocates `count` device slots for pairs of this shape (or smaller)."""
+        c = config.to_c()
+        _check_server(lib.vxh_server_reserve(self._h, width, height, ctypes.byref(c), count))
 
     def submit(self, left, right, width, height, config, out):
         """Queues a request; `out` (float32, w*h) is filled on completion."""

@@ -202,6 +202,25 @@ vxh_status vxh_sgm_enqueue_marked(vxh_sgm *sgm, const uint8_t *d_left,
                                   void *d_workspace, size_t workspace_bytes,
                                   void *stream, void *const *events3);
 
+/* vxh_sgm_enqueue for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * pipeline, with the library's static device scratch of
+ * vxh_sgm_static_scratch_bytes() (96 MiB, held by every GPU the library runs
+ * on until the process ends) as its workspace instead of a caller-supplied
+ * one.  It allocates nothing: no hipMalloc, hipFree or allocating library
+ * call.  Pairs whose workspace (vxh_sgm_workspace_bytes) exceeds the static
+ * scratch are rejected with VXH_ERR_INVALID_CONFIG (320x240 with D=64 and 8
+ * paths fits).  Every call on one GPU shares that GPU's scratch, so calls
+ * that overlap on one GPU (other streams, other threads) race; the scratch
+ * keeps the last call's contents, and each call clears what must start at
+ * zero itself.  Split scanlines stay off.  No counterpart in the reference. */
+vxh_status vxh_sgm_enqueue_static(vxh_sgm *sgm, const uint8_t *d_left,
+                                  const uint8_t *d_right, uint32_t w, uint32_t h,
+                                  const vxh_sgm_config *cfg, float *d_out,
+                                  void *stream);
+/* Bytes of the static scratch vxh_sgm_enqueue_static works in. */
+size_t vxh_sgm_static_scratch_bytes(void);
+
 /* Status of the enqueued call that used `d_workspace`, once the caller has
  * synchronised the stream it was enqueued on: VXH_ERR_GPU when a split
  * scanline's bounded wait expired in that call (its map is then incomplete),

@@ -173,6 +173,41 @@ void tex_forget(const vxh_texture *t) {
 
 }  // namespace
 
+namespace {
+
+// Scratch of vxh_sgm_enqueue_static, the entry point for callers that cannot
+// allocate: a file-scope device array (every GPU the library runs on holds
+// its own copy, at its own address, until the process ends).  Host code never
+// passes `scratch` itself: the name is a host-side stand-in, and
+// resolve_static_scratch() looks up each device's address once
+// (hipGetSymbolAddress with that device current) into g_scratch_ptr.
+constexpr size_t kStaticScratchBytes = (size_t)96 << 20;
+__attribute__((used)) static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;   // per device ordinal; null until resolved
+
+// Looks up `device`'s copy of the array (the caller has made `device`
+// current); once per device.
+vxh_status resolve_static_scratch(int device) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (g_scratch_ptr[(size_t)device]) return VXH_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(VXH_ERR_SHADER_MISSING,
+                    std::string("shader missing: static scratch symbol: ") + hipGetErrorString(e));
+    g_scratch_ptr[(size_t)device] = p;
+    return VXH_OK;
+}
+
+void *static_scratch(int device) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    return (size_t)device < g_scratch_ptr.size() ? g_scratch_ptr[(size_t)device] : nullptr;
+}
+
+}  // namespace
+
 namespace vxh {
 
 void capture_opened() { g_open_captures.fetch_add(1); }
@@ -509,6 +544,8 @@ vxh_status vxh_sgm_create(vxh_context *ctx, vxh_sgm **out) {
     if (!ctx || !out) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null argument");
     *out = nullptr;
     VXH_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    vxh_status v = resolve_static_scratch(ctx->device);
+    if (v != VXH_OK) return v;
     vxh_sgm *s = new vxh_sgm();
     s->ctx = ctx;
     *out = s;
@@ -709,6 +746,40 @@ vxh_status vxh_sgm_enqueue(vxh_sgm *s, const uint8_t *d_left, const uint8_t *d_r
     VXH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
     return enqueue_impl(s, d_left, d_right, w, h, c, d_out, d_ws, ws_bytes,
                         (hipStream_t)stream);
+}
+
+size_t vxh_sgm_static_scratch_bytes(void) { return kStaticScratchBytes; }
+
+// vxh_sgm_enqueue with the device's static scratch as the workspace: no
+// allocation of any kind.  Split scanlines stay off (their bounded waits
+// report through the workspace, which no caller holds here); results are the
+// same either way.
+vxh_status vxh_sgm_enqueue_static(vxh_sgm *s, const uint8_t *d_left, const uint8_t *d_right,
+                                  uint32_t w, uint32_t h, const vxh_sgm_config *c, float *d_out,
+                                  void *stream) {
+    if (!s) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null handle");
+    vxh_status v = vxh_sgm_config_validate(c, w, h);
+    if (v != VXH_OK) return v;
+    if (!d_left || !d_right || !d_out)
+        return fail(VXH_ERR_INVALID_CONFIG, "invalid config: null device buffer");
+    const int mode = mode_for(s, c);
+    const WsLayout L =
+        workspace_layout(w, h, c->num_disparities, (int)c->num_paths, mode, c->lr_check != 0);
+    if (L.total > kStaticScratchBytes) {
+        char buf[200];
+        snprintf(buf, sizeof buf,
+                 "invalid config: %ux%u with %u disparities and %u paths needs %zu bytes of "
+                 "scratch; the static scratch holds %zu",
+                 w, h, c->num_disparities, c->num_paths, L.total, kStaticScratchBytes);
+        return fail(VXH_ERR_INVALID_CONFIG, buf);
+    }
+    VXH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
+    void *ws = static_scratch(s->ctx->device);
+    if (!ws) return fail(VXH_ERR_SHADER_MISSING, "shader missing: static scratch not resolved");
+    vxh::Schedule sc = schedule_for(s, (int)c->num_disparities);
+    sc.split = 0;
+    return vxh::run_pipeline(d_left, d_right, w, h, c, d_out, (char *)ws, mode, sc,
+                             (hipStream_t)stream);
 }
 
 vxh_status vxh_sgm_enqueue_marked(vxh_sgm *s, const uint8_t *d_left, const uint8_t *d_right,

@@ -99,6 +99,8 @@ _sig("vxh_sgm_enqueue", ctypes.c_int, _vp, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp
 _sig("vxh_sgm_enqueue_marked", ctypes.c_int, _vp, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp,
      _sz, _vp, ctypes.POINTER(_vp))
 _sig("vxh_sgm_workspace_status", ctypes.c_int, _vp, _vp)
+_sig("vxh_sgm_enqueue_static", ctypes.c_int, _vp, _vp, _vp, _u32, _u32, _cfgp, _vp, _vp)
+_sig("vxh_sgm_static_scratch_bytes", _sz)
 _sig("vxh_sgm_set_split_wait", ctypes.c_int, _vp, _u32)
 _sig("vxh_sgm_volume_format_in_effect", _u32, _vp, _u32, _cfgp)
 _sig("vxh_sgm_workspace_error_offset", _sz)

@@ -80,8 +80,9 @@ def parse():
     ap.add_argument("--volume-format", default="auto", choices=["auto", "u8", "u16"],
                     help="per-direction volume storage of the plain path")
     ap.add_argument("--batch-options", default="",
-                    help="streams of the batched form (default: tuned, or the built-in "
-                         "default with --no-tune)")
+                    help='options of the batched form as JSON, e.g. \'{"form": "streams", '
+                         '"streams": 4}\' (default: tuned, or the built-in default with '
+                         '--no-tune)')
     return ap.parse_args()
 
 
@@ -132,13 +133,18 @@ def batched_leg(vx, hip, np, args, sgm, tuner):
         ob, op = hip.DeviceBuffer(w * h * 4), hip.DeviceBuffer(w * h * 4)
         bufs.append((dl, dr, ob, op))
         descs.append((dl.ptr, dr.ptr, ob.ptr, w, h, cfg, None))
+    from vxsgm.tuner import apply_layouts
+    lay = "default"
     if args.batch_options:
-        opts = tuple(int(v) for v in args.batch_options.split(","))
+        opts = json.loads(args.batch_options)
     else:
-        opts = None if tuner is None else tuner.tune_batch(descs)
-    log(f"batched: {len(host)} pairs, options {opts}")
+        opts, lay = (None, "default") if tuner is None else tuner.tune_batch(descs)
+    log(f"batched: {len(host)} pairs, options {opts}, layouts {lay}")
+    if tuner is not None:
+        log(f"batched candidates (ms): {tuner.table.get('batch', {}).get('ms')}")
     stream = hip.Stream()
-    batch = vx.SGMBatch(descs, options=opts)
+    batch = vx.SGMBatch(apply_layouts(descs, lay), options=opts)
+    chunks = batch.chunks()
     batch.launch(stream.handle)
     stream.synchronize()
     e0, e1 = hip.Event(), hip.Event()
@@ -184,8 +190,12 @@ def batched_leg(vx, hip, np, args, sgm, tuner):
         "roofline_frac_batch": round(alg / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "bit_exact_vs_plain": bool(exact),
         "bit_exact_vs_oracle": vs_oracle,
-        "form": "per-pair pipelines over several streams, replayed from one hipGraph",
-        "streams": opts[0] if opts else "default",
+        "form": ("persistent: multi-pair census, cost and winner launches around one "
+                 "persistent aggregation launch per layout class, replayed from one hipGraph"
+                 if not opts or opts.get("form", "persistent") == "persistent" else
+                 "streams: per-pair pipelines over several streams, replayed from one hipGraph"),
+        "options": opts, "layouts": lay, "chunks": chunks,
+        "candidates_ms": None if tuner is None else tuner.table.get("batch", {}).get("ms"),
     }
 
 
@@ -205,8 +215,9 @@ def stream_leg(vx, hip, np, args, sgm):
     e0, e1 = hip.Event(), hip.Event()
     per_streams = {}
     exact = True
-    for nstreams in (2, 4):
-        batch = vx.SGMBatch(descs, options=(nstreams,))
+    for nstreams in (2, 4, "persistent"):
+        opts = {"form": "persistent"} if nstreams == "persistent" else (nstreams,)
+        batch = vx.SGMBatch(descs, options=opts)
         batch.launch(stream.handle)
         stream.synchronize()
         e0.record(stream)
@@ -219,7 +230,8 @@ def stream_leg(vx, hip, np, args, sgm):
         exact = exact and all(np.array_equal(o.download(np.float32).view(np.uint32),
                                              want.view(np.uint32)) for o in outs)
         per_streams[nstreams] = round(ms / len(outs), 4)
-        log(f"stream: {nstreams} streams, {ms / len(outs):.4f} ms per pair")
+        log(f"stream: {nstreams}{' streams' if nstreams != 'persistent' else ''}, "
+            f"{ms / len(outs):.4f} ms per pair")
     best = min(per_streams, key=per_streams.get)
     ms_pair = per_streams[best]
     return {
@@ -227,7 +239,8 @@ def stream_leg(vx, hip, np, args, sgm):
         "ms_per_pair": ms_pair, "ms_per_pair_by_streams": per_streams,
         "value": round(w * h * d / (ms_pair * 1e-3) / 1e9, 3), "unit": "Gpx*disp/s",
         "bit_exact_vs_plain": bool(exact),
-        "form": "identical 720p pairs overlapped over several streams, one hipGraph replay",
+        "form": "identical 720p pairs in one hipGraph replay: per-pair pipelines over 2 or 4 "
+                "streams, or the persistent multi-pair form",
     }
 
 

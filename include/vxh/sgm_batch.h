@@ -1,7 +1,8 @@
 /*
  * vxh batched SGM: many stereo pairs of uneven resolution and disparity range
- * replayed from one hipGraph that runs the per-pair pipeline of every pair
- * over several streams at once.
+ * replayed from one hipGraph, by default as multi-pair launches around one
+ * persistent aggregation launch per lane-layout class (see
+ * vxh_sgm_batch_options for the forms).
  *
  * No single reference entry point corresponds: the reference computes one
  * pair per SGMStereo::compute_disparity call
@@ -45,28 +46,47 @@ typedef struct {
 
 typedef struct vxh_sgm_batch vxh_sgm_batch;
 
-/* Launch options (the autotuner's knob; 0 = default): `streams`, 1..16
- * (default 4), is the number of streams the pairs are dealt to, largest pairs
- * first; pairs on one stream run in turn and share one workspace. */
+/* Launch options (the autotuner's knobs; 0 = default):
+ *   form           0 = persistent (default): pairs in chunks of at most
+ *                  budget_bytes of workspace, each chunk one census launch,
+ *                  one cost launch, one persistent aggregation launch per
+ *                  lane-layout class and volume format present (waves take
+ *                  scanline groups of every pair by ticket), one sum +
+ *                  winner launch; split scanlines are not used.
+ *                  1 = streams: the per-pair pipelines dealt to `streams`
+ *                  streams, largest pairs first, pairs on one stream in turn
+ *                  sharing one workspace.
+ *   streams        streams form: 1..16 (default 4).
+ *   blocks_per_cu  persistent form: blocks of 256 threads of the aggregation
+ *                  per CU (default: as many as fit).
+ *   budget_bytes   persistent form: workspace bytes of one chunk (default
+ *                  64 GiB; a pair larger than the budget forms its own
+ *                  chunk). */
 typedef struct {
     uint32_t streams;
+    uint32_t form;
+    uint32_t blocks_per_cu;
+    uint32_t reserved;
+    uint64_t budget_bytes;
 } vxh_sgm_batch_options;
 
 /* Per-pair plan summary. */
 typedef struct {
-    uint32_t stream;          /* stream the pair runs on */
+    uint32_t stream;          /* streams form: stream the pair runs on */
     uint32_t lanes;           /* layout of the non-horizontal directions */
     uint32_t per_lane;
     uint32_t lanes_h;         /* layout of the horizontal directions */
     uint32_t per_lane_h;
     uint32_t volume_format;   /* 1 = u8, 2 = u16 */
     uint64_t workspace_bytes; /* scratch the pair needs */
+    uint32_t chunk;           /* persistent form: chunk the pair runs in */
+    uint32_t form;            /* 0 persistent, 1 streams */
 } vxh_sgm_batch_stats;
 
-/* Validates every pair, allocates one workspace per stream (the largest
- * vxh_sgm_workspace_bytes of the pairs dealt to it) and captures the
- * pipelines of all pairs in a hipGraph.  Buffers named in `pairs` are bound
- * at creation. */
+/* Validates every pair, allocates the workspaces (persistent form: one shared
+ * by the chunks, the largest chunk's; streams form: one per stream, the
+ * largest of the pairs dealt to it) and captures every launch in a hipGraph.
+ * Buffers named in `pairs` are bound at creation. */
 vxh_status vxh_sgm_batch_create(int device, const vxh_sgm_pair_desc *pairs,
                                 uint32_t n, vxh_sgm_batch **out);
 /* vxh_sgm_batch_create with explicit options (NULL = defaults). */
@@ -84,6 +104,8 @@ vxh_status vxh_sgm_batch_stats_get(const vxh_sgm_batch *batch, uint32_t pair,
                                    vxh_sgm_batch_stats *out);
 /* Kernel launches in the batch's graph. */
 uint32_t vxh_sgm_batch_items(const vxh_sgm_batch *batch);
+/* Chunks of the persistent form (1 for the streams form). */
+uint32_t vxh_sgm_batch_chunks(const vxh_sgm_batch *batch);
 void vxh_sgm_batch_destroy(vxh_sgm_batch *batch);
 /* Message of the last batch failure on this thread. */
 const char *vxh_sgm_batch_last_error(void);

@@ -60,9 +60,14 @@ def bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
-def test_batch_small_mix_matches_oracle():
+@pytest.mark.parametrize("options", [None, {"form": "streams"},
+                                     {"form": "persistent", "blocks_per_cu": 1,
+                                      "budget_bytes": 24 << 20}])
+def test_batch_small_mix_matches_oracle(options):
     """Mixed sizes, D, path counts, LR checks, volume formats (u8 with a P2
-    that overflows bytes, u16) and penalties, over repeated replays."""
+    that overflows bytes, u16) and penalties, over repeated replays, in the
+    persistent form (one chunk; several small chunks with one aggregation
+    block per CU) and the streams form."""
     pairs, formats = [], []
     for i, (w, h, d, paths) in enumerate([(64, 40, 16, 2), (100, 31, 48, 4), (80, 80, 64, 8),
                                           (129, 17, 100, 8), (300, 20, 256, 8), (33, 3, 8, 8),
@@ -73,7 +78,7 @@ def test_batch_small_mix_matches_oracle():
                                     lr_check=(i % 2 == 1))
         pairs.append((l, r, cfg))
         formats.append(["auto", "u16", "auto", "u16", "auto", "u8", "u8", "u16"][i])
-    outs = run_batch(pairs, replays=3, formats=formats)
+    outs = run_batch(pairs, replays=3, formats=formats, options=options)
     bad = []
     for k, rep in enumerate(outs):
         for i, ((l, r, cfg), got) in enumerate(zip(pairs, rep)):
@@ -89,10 +94,12 @@ def test_batch_small_mix_matches_oracle():
     assert not bad, "\n".join(bad)
 
 
-@pytest.mark.parametrize("opts", [None, (1,), (3,), (16,)])
+@pytest.mark.parametrize("opts", [None, (1,), (3,), (16,), {"blocks_per_cu": 2}])
 def test_batch_launch_schedules_equal_plain(ctx, opts):
-    """Every stream count gives the plain path's maps, and the plain path
-    gives the same maps under each of its lane layouts."""
+    """Both forms (the persistent one by default or with two aggregation
+    blocks per CU, the streams one with every stream count) give the plain
+    path's maps, and the plain path gives the same maps under each of its lane
+    layouts."""
     pairs = [vxsgm.mixture_pair(256, 96, 128, s)[:2] for s in range(3)]
     cfg = vxsgm.SGMStereoConfig(num_disparities=128, num_paths=8)
     bufs, descs = [], []
@@ -164,6 +171,7 @@ def test_batch_schedules_and_splits_per_pair():
         formats.append(["auto", "u16", "u8"][i % 3])
     outs = run_batch(pairs, scheds=scheds, replays=2, formats=formats, splits=splits,
                      options=(3,))
+    outs += run_batch(pairs, scheds=scheds, replays=1, formats=formats, splits=splits)
     for rep in outs:
         for (l, r, cfg), got in zip(pairs, rep):
             want = O.sgm(l, r, cfg.num_disparities, num_paths=cfg.num_paths,
@@ -171,14 +179,17 @@ def test_batch_schedules_and_splits_per_pair():
             assert (bits(got) == bits(want)).all()
 
 
-def test_batch_full_mixture_matches_oracle_hashes():
+@pytest.mark.parametrize("options", [None, {"form": "persistent", "budget_bytes": 16 << 30},
+                                     {"form": "streams"}])
+def test_batch_full_mixture_matches_oracle_hashes(options):
     """The batched workload at full size: all 64 pairs of the mixture
     ({320x240, 640x480, 1280x720, 1920x1080} x D {64, 128, 256}, 8 paths) in
     one batch, replayed twice; every map's SHA-256 equals the oracle's
-    (tests/golden/mixture_hashes.json)."""
+    (tests/golden/mixture_hashes.json).  Persistent form with the default
+    budget and with a 16 GiB budget (several chunks), and the streams form."""
     rows = golden_mixture("batch_64")
     pairs = [vxsgm.mixture_pair(w, h, d, seed) for w, h, d, seed, _ in rows]
-    outs = run_batch(pairs, replays=2)
+    outs = run_batch(pairs, replays=2, options=options)
     bad = [f"replay {k} pair {i} ({rows[i][0]}x{rows[i][1]} D={rows[i][2]})"
            for k, rep in enumerate(outs) for i, got in enumerate(rep)
            if map_sha256(got) != rows[i][4]]

@@ -35,16 +35,20 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose=True, jobs=None):
-    os.makedirs(OBJ, exist_ok=True)
+def build(verbose=True, jobs=None, out=OUT, defines=(), objdir=OBJ):
+    """Compiles every source and links `out`; `defines` (NAME=VALUE strings)
+    select compile-time variants of the kernels, built in their own
+    `objdir`."""
+    os.makedirs(objdir, exist_ok=True)
     cmds, objs = [], []
+    dflags = ["-D" + d for d in defines]
     for src in HIP_SOURCES:
-        o = os.path.join(OBJ, src + ".o")
-        cmds.append([HIPCC, "-x", "hip", *HIP_FLAGS, *COMMON, "-c",
+        o = os.path.join(objdir, src + ".o")
+        cmds.append([HIPCC, "-x", "hip", *HIP_FLAGS, *COMMON, *dflags, "-c",
                      os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     for src in CXX_SOURCES:
-        o = os.path.join(OBJ, src + ".o")
+        o = os.path.join(objdir, src + ".o")
         cmds.append(["g++", *COMMON, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     # Independent translation units compile side by side (at most 8 at once:
@@ -53,10 +57,10 @@ def build(verbose=True, jobs=None):
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         for f in [ex.submit(_run, c) for c in cmds]:
             f.result()
-    tmp = OUT + ".tmp"
+    tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"])
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

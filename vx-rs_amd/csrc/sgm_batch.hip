@@ -3,20 +3,36 @@
 //
 // The reference runs one pair per command buffer (vx-vision/src/kernels/
 // sgm.rs:135-261) and its Pipeline batches encodes of different kernels into
-// one command buffer (vx-vision/src/pipeline.rs:12-80).  Here a batch is the
-// per-pair pipeline (census, costs, aggregation, sum + winner, LR check) of
-// every pair, captured once into a graph over `streams` streams: pairs are
-// ordered by work (px * disparities, largest first) and dealt round-robin to
-// the streams, which run side by side, so that one pair's HBM-bound passes
-// (census, costs, sums) overlap another's instruction-bound aggregation.
-// Pairs on one stream run one after the other and share that stream's
-// workspace, sized for the largest of them.  Every pair runs exactly the
-// plain path's kernels with its own schedule and volume format, so its map
-// is bit-identical to the plain path's.
+// one command buffer (vx-vision/src/pipeline.rs:12-80).  Two forms:
+//
+// Persistent (default).  Pairs, largest first, are grouped into chunks whose
+// workspaces fit a byte budget; chunks run one after the other in one shared
+// workspace.  A chunk is four steps, each one launch over all of its pairs:
+// the census of every image, the cost volume of every pair, the path
+// aggregation of every direction of every pair, and the sum + winner-take-all
+// of every pair (plus the LR check of the pairs that ask for it).  The
+// aggregation is one persistent launch per lane-layout class and volume
+// format present (launched side by side): its waves take (pass, scanline
+// group) items by ticket from one list per XCD, ordered pair by pair, so a
+// single chunk fills the chip with waves from many pairs instead of the 1.7
+// waves per SIMD one 720p pair provides.  Split scanlines are not used (they
+// shorten one pair's critical path, which a full chip does not need).
+//
+// Streams.  The per-pair pipeline of every pair (the plain path's exact
+// launches, including split scanlines) is captured over several streams:
+// pairs are ordered by work and dealt round-robin to the streams, which run
+// side by side; pairs on one stream run in turn and share that stream's
+// workspace.
+//
+// Both forms run the plain path's kernels or their multi-pair wrappers around
+// the same per-tile, per-scanline and per-pixel device code, with each pair's
+// own schedule and volume format, so every map is bit-identical to the plain
+// path's.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -36,6 +52,8 @@ vxh_status bfail(vxh_status st, const std::string &m) {
 
 constexpr uint32_t kDefaultStreams = 4;
 constexpr uint32_t kMaxStreams = 16;
+constexpr uint64_t kDefaultBudget = (uint64_t)64 << 30;
+constexpr uint32_t kFormPersistent = 0, kFormStreams = 1;
 
 // Index of (L, K) among the layouts of class c, -1 if none.
 int layout_index(int c, uint32_t L, uint32_t K) {
@@ -50,24 +68,356 @@ struct PairPlan {
     vxh::Schedule sched;
     int mode;
     size_t ws_bytes;
-    uint32_t stream;
+    uint32_t stream;   // streams form: stream; persistent form: chunk
+    size_t ws_off;     // persistent form: offset in the shared workspace
 };
+
+// One chunk of the persistent form: its pairs and the device tables of its
+// launches (pointers into the batch's table allocation).
+struct Chunk {
+    std::vector<uint32_t> pairs;
+    size_t ws_bytes = 0;
+    const vxh::CensusJob *census = nullptr;
+    int ncensus = 0;
+    uint32_t census_blocks = 0;
+    const vxh::CostJob *cost = nullptr;
+    int ncost = 0;
+    uint32_t cost_blocks = 0;
+    const vxh::WtaJob *wta = nullptr;
+    int nwta = 0;
+    uint32_t wta_blocks = 0;
+    struct Group {
+        int cls, mode;
+        vxh::BatchPaths args;
+        int blocks;
+    };
+    std::vector<Group> groups;
+};
+
+int class_occupancy(int cls, int mode) {
+    return cls == 0 ? vxh::batch_paths_occupancy_c0(mode)
+                    : cls == 1 ? vxh::batch_paths_occupancy_c1(mode)
+                               : vxh::batch_paths_occupancy_c2(mode);
+}
+
+hipError_t launch_class(int cls, int mode, const vxh::BatchPaths &b, int blocks, hipStream_t s) {
+    return cls == 0 ? vxh::launch_batch_paths_c0(mode, b, blocks, s)
+                    : cls == 1 ? vxh::launch_batch_paths_c1(mode, b, blocks, s)
+                               : vxh::launch_batch_paths_c2(mode, b, blocks, s);
+}
+
+// Appends `bytes` (aligned to 256) to a host staging image of the device
+// tables; returns its offset.
+size_t stage(std::vector<uint8_t> &img, const void *src, size_t bytes) {
+    const size_t off = vxh::ws_align(img.size());
+    img.resize(off + vxh::ws_align(bytes));
+    std::memcpy(img.data() + off, src, bytes);
+    return off;
+}
 
 }  // namespace
 
 struct vxh_sgm_batch {
     int device = 0;
+    uint32_t form = kFormPersistent;
     std::vector<vxh_sgm_pair_desc> pairs;
     std::vector<PairPlan> plan;
-    std::vector<void *> ws;              // one workspace per stream
+    std::vector<void *> ws;              // streams: one per stream; persistent: one shared
+    void *tables = nullptr;              // persistent: job tables, passes, items, counters
+    uint32_t *counters = nullptr;        // persistent: ticket counters (in `tables`)
+    size_t ncounters = 0;
+    std::vector<Chunk> chunks;
     uint32_t *d_error = nullptr;         // OR of every pair's error word
     hipStream_t stream = nullptr;        // the batch's own stream (capture origin)
-    std::vector<hipStream_t> sub;        // the streams the pairs run on
-    std::vector<hipEvent_t> events;      // fork + one join per stream
+    std::vector<hipStream_t> sub;        // streams the pairs (or aggregation groups) run on
+    std::vector<hipEvent_t> events;      // fork + one join per sub stream
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint32_t launches = 0;
 };
+
+namespace {
+
+// Builds the chunks of the persistent form, allocates the shared workspace
+// (b->ws[0]) and the device tables of every chunk's launches (b->tables: job
+// tables, passes, item lists, ticket counters), and copies the tables.
+hipError_t plan_persistent(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
+                           uint64_t budget, uint32_t blocks_per_cu) {
+    using namespace vxh;
+    // Chunks: pairs in order, a new chunk when the budget would be exceeded.
+    Chunk cur;
+    for (uint32_t i : order) {
+        const size_t wbytes = ws_align(b->plan[i].ws_bytes);
+        if (!cur.pairs.empty() && cur.ws_bytes + wbytes > budget) {
+            b->chunks.push_back(cur);
+            cur = Chunk();
+        }
+        b->plan[i].ws_off = cur.ws_bytes;
+        b->plan[i].stream = (uint32_t)b->chunks.size();
+        cur.pairs.push_back(i);
+        cur.ws_bytes += wbytes;
+    }
+    b->chunks.push_back(cur);
+    size_t ws_total = 256;
+    for (const Chunk &c : b->chunks) ws_total = std::max(ws_total, c.ws_bytes);
+    void *ws = nullptr;
+    hipError_t e = hipMalloc(&ws, ws_total);
+    if (e != hipSuccess) return e;
+    b->ws.push_back(ws);
+
+    int dev = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, dev)) != hipSuccess) return e;
+    const int cus = prop.multiProcessorCount;
+
+    // Host image of the tables; offsets into it become device pointers once
+    // it is allocated.
+    std::vector<uint8_t> img;
+    struct Off {
+        size_t census, cost, wta;
+        std::vector<size_t> passes, items;
+    };
+    std::vector<Off> off(b->chunks.size());
+    size_t ncounters = 0;
+    for (size_t ci = 0; ci < b->chunks.size(); ++ci) {
+        Chunk &c = b->chunks[ci];
+        std::vector<CensusJob> cj;
+        std::vector<CostJob> kj;
+        std::vector<WtaJob> wj;
+        uint32_t cb = 0, kb = 0, wb = 0;
+        struct G {
+            int cls, mode;
+            std::vector<BatchPass> passes;
+            std::vector<BatchItem> items[8];
+        };
+        std::vector<G> groups;
+        for (uint32_t i : c.pairs) {
+            const vxh_sgm_pair_desc &q = b->pairs[i];
+            const PairPlan &p = b->plan[i];
+            const int w = (int)q.width, h = (int)q.height, D = (int)q.cfg.num_disparities;
+            const int Dp = volume_stride(D);
+            const WsLayout L = workspace_layout(q.width, q.height, (uint32_t)D,
+                                                (int)q.cfg.num_paths, p.mode,
+                                                q.cfg.lr_check != 0);
+            char *base = (char *)ws + p.ws_off;
+            const bool u8 = p.mode == kOutU8;
+            // Census of both images; the left one also clears the flags and
+            // the synchronisation words, as run_pipeline does.
+            CensusJob l{q.d_left, (uint2 *)(base + L.lc),
+                        u8 ? (uint2 *)(base + L.flags) : nullptr, (uint32_t *)(base + L.sync),
+                        sync_words(q.width, q.height), w, h, (int)q.cfg.census_radius_x,
+                        (int)q.cfg.census_radius_y, (w + 63) / 64, cb};
+            cb += (uint32_t)(l.nbx * ((h + 3) / 4));
+            CensusJob r = l;
+            r.img = q.d_right;
+            r.out = (uint2 *)(base + L.rc);
+            r.zero_flags = nullptr;
+            r.zero_words = nullptr;
+            r.nwords = 0;
+            r.first_block = cb;
+            cb += (uint32_t)(r.nbx * ((h + 3) / 4));
+            cj.push_back(l);
+            cj.push_back(r);
+            const uint32_t p1 = (uint32_t)q.cfg.p1 & 0xFFFFu, p2 = (uint32_t)q.cfg.p2 & 0xFFFFu;
+            CostJob k{(const uint2 *)(base + L.lc), (const uint2 *)(base + L.rc),
+                      (uint8_t *)(base + L.cost), w, h, D, Dp, cost_bias(p1),
+                      (w + 255) / 256, kb};
+            kb += (uint32_t)(k.nbx * h);
+            kj.push_back(k);
+            const bool lr = q.cfg.lr_check != 0;
+            WtaJob t{(const uint8_t *)(base + L.vol), L.vol_bytes,
+                     u8 ? (const uint8_t *)(base + L.hi) : nullptr,
+                     u8 ? (const uint2 *)(base + L.flags) : nullptr, q.d_out,
+                     lr ? (int16_t *)(base + L.best) : nullptr,
+                     lr ? (uint16_t *)(base + L.sum) : nullptr, (int)q.cfg.num_paths, p.mode,
+                     w * h, D, Dp, wb};
+            wb += (uint32_t)((w * h + 15) / 16);
+            wj.push_back(t);
+            // Direction passes, longest scanlines first, into the group of
+            // the pair's layout class and volume format.
+            const int cls = layout_class(D);
+            size_t gi = groups.size();
+            for (size_t x = 0; x < groups.size(); ++x)
+                if (groups[x].cls == cls && groups[x].mode == p.mode) gi = x;
+            if (gi == groups.size()) groups.push_back(G{cls, p.mode, {}, {}});
+            G &g = groups[gi];
+            PassParams pp{};
+            pp.cost = (const uint8_t *)(base + L.cost);
+            pp.flags = u8 ? (uint8_t *)(base + L.flags) : nullptr;
+            pp.w = w;
+            pp.h = h;
+            pp.D = D;
+            pp.Dp = Dp;
+            pp.p1x2 = p1 | (p1 << 16);
+            pp.p2x2 = p2 | (p2 << 16);
+            const uint32_t xq = p1 - cost_bias(p1);
+            pp.xqx2 = xq | (xq << 16);
+            pp.mode = p.mode;
+            pp.wait_polls = kSplitWaitPolls;
+            int dirs[kMaxPasses];
+            const int P = (int)q.cfg.num_paths;
+            for (int j = 0; j < P; ++j) dirs[j] = j;
+            std::stable_sort(dirs, dirs + P, [&](int a, int z) {
+                return scan_length(a, w, h) > scan_length(z, w, h);
+            });
+            for (int j = 0; j < P; ++j) {
+                const int dir = dirs[j];
+                BatchPass bp{pp, 0};
+                bp.p.vol = (uint8_t *)(base + L.vol + (size_t)dir * L.vol_bytes);
+                bp.p.hi = u8 ? (uint8_t *)(base + L.hi + (size_t)dir * L.vol_bytes) : nullptr;
+                bp.p.dir = dir;
+                bp.p.dx = kDirX[dir];
+                bp.p.dy = kDirY[dir];
+                bp.p.nscan = num_scanlines(dir, w, h);
+                bp.layout = bp.p.dy == 0 ? p.sched.horiz : p.sched.other;
+                const int per_wave = 128 / kClassLayouts[cls][bp.layout][0];
+                const uint32_t nw = (uint32_t)((bp.p.nscan + per_wave - 1) / per_wave);
+                const uint32_t pi = (uint32_t)g.passes.size();
+                g.passes.push_back(bp);
+                for (uint32_t xk = 0; xk < 8; ++xk)
+                    for (uint32_t v = nw * xk / 8; v < nw * (xk + 1) / 8; ++v)
+                        g.items[xk].push_back(BatchItem{pi, v});
+            }
+        }
+        c.ncensus = (int)cj.size();
+        c.census_blocks = cb;
+        c.ncost = (int)kj.size();
+        c.cost_blocks = kb;
+        c.nwta = (int)wj.size();
+        c.wta_blocks = wb;
+        off[ci].census = stage(img, cj.data(), cj.size() * sizeof(CensusJob));
+        off[ci].cost = stage(img, kj.data(), kj.size() * sizeof(CostJob));
+        off[ci].wta = stage(img, wj.data(), wj.size() * sizeof(WtaJob));
+        for (G &g : groups) {
+            Chunk::Group cg{g.cls, g.mode, {}, 0};
+            std::vector<BatchItem> all;
+            uint32_t most = 0;
+            for (int xk = 0; xk < 8; ++xk) {
+                cg.args.item_off[xk] = (uint32_t)all.size();
+                cg.args.item_cnt[xk] = (uint32_t)g.items[xk].size();
+                most = std::max(most, cg.args.item_cnt[xk]);
+                all.insert(all.end(), g.items[xk].begin(), g.items[xk].end());
+            }
+            off[ci].passes.push_back(
+                stage(img, g.passes.data(), g.passes.size() * sizeof(BatchPass)));
+            off[ci].items.push_back(stage(img, all.data(), all.size() * sizeof(BatchItem)));
+            // Persistent grid: as many blocks as the chip holds at once (or
+            // blocks_per_cu per CU), no more than the items need, and a
+            // multiple of 8 so every XCD's list has waves.
+            int per_cu = std::max(1, class_occupancy(g.cls, g.mode));
+            if (blocks_per_cu) per_cu = std::min(per_cu, (int)blocks_per_cu);
+            const int need = 8 * (int)((most + 3) / 4);
+            cg.blocks = std::max(8, std::min(need, (per_cu * cus + 7) / 8 * 8));
+            cg.args.counters = nullptr;
+            c.groups.push_back(cg);
+            ncounters += 8;
+        }
+    }
+    const size_t counters_off = ws_align(img.size());
+    img.resize(counters_off + ws_align(std::max<size_t>(ncounters, 1) * 4), 0);
+    if ((e = hipMalloc(&b->tables, img.size())) != hipSuccess) return e;
+    char *tb = (char *)b->tables;
+    b->counters = (uint32_t *)(tb + counters_off);
+    b->ncounters = ncounters;
+    size_t next_counter = 0;
+    for (size_t ci = 0; ci < b->chunks.size(); ++ci) {
+        Chunk &c = b->chunks[ci];
+        c.census = (const CensusJob *)(tb + off[ci].census);
+        c.cost = (const CostJob *)(tb + off[ci].cost);
+        c.wta = (const WtaJob *)(tb + off[ci].wta);
+        for (size_t g = 0; g < c.groups.size(); ++g) {
+            BatchPaths &a = c.groups[g].args;
+            a.passes = (const BatchPass *)(tb + off[ci].passes[g]);
+            a.items = (const BatchItem *)(tb + off[ci].items[g]);
+            a.counters = b->counters + next_counter;
+            next_counter += 8;
+        }
+    }
+    return hipMemcpy(b->tables, img.data(), img.size(), hipMemcpyHostToDevice);
+}
+
+// Records the persistent form's launches on b->stream (being captured): the
+// counters and the error word cleared, then per chunk census, costs, the
+// aggregation groups side by side on the sub streams, sums + winners and the
+// LR checks.
+hipError_t capture_persistent(vxh_sgm_batch *b) {
+    using namespace vxh;
+    hipError_t e = launch_zero(b->d_error, 1, b->stream);
+    if (e == hipSuccess && b->ncounters) e = launch_zero(b->counters, b->ncounters, b->stream);
+    b->launches = 2;
+    for (const Chunk &c : b->chunks) {
+        if (e == hipSuccess) e = launch_census_multi(c.census, c.ncensus, c.census_blocks, b->stream);
+        if (e == hipSuccess) e = launch_cost_multi(c.cost, c.ncost, c.cost_blocks, b->stream);
+        b->launches += 2;
+        if (e == hipSuccess && c.groups.size() == 1) {
+            const Chunk::Group &g = c.groups[0];
+            e = launch_class(g.cls, g.mode, g.args, g.blocks, b->stream);
+            b->launches += 1;
+        } else if (e == hipSuccess) {
+            // Groups side by side: fork onto the sub streams, join back.
+            e = hipEventRecord(b->events[0], b->stream);
+            for (size_t g = 0; g < c.groups.size() && e == hipSuccess; ++g) {
+                hipStream_t s = b->sub[g];
+                e = hipStreamWaitEvent(s, b->events[0], 0);
+                if (e == hipSuccess)
+                    e = launch_class(c.groups[g].cls, c.groups[g].mode, c.groups[g].args,
+                                     c.groups[g].blocks, s);
+                if (e == hipSuccess) e = hipEventRecord(b->events[1 + g], s);
+                if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->events[1 + g], 0);
+                b->launches += 1;
+            }
+        }
+        if (e == hipSuccess) e = launch_wta_multi(c.wta, c.nwta, c.wta_blocks, b->stream);
+        b->launches += 1;
+        for (uint32_t i : c.pairs) {
+            const vxh_sgm_pair_desc &q = b->pairs[i];
+            if (!q.cfg.lr_check || e != hipSuccess) continue;
+            const PairPlan &p = b->plan[i];
+            const WsLayout L = workspace_layout(q.width, q.height, q.cfg.num_disparities,
+                                                (int)q.cfg.num_paths, p.mode, true);
+            char *base = (char *)b->ws[0] + p.ws_off;
+            const int D = (int)q.cfg.num_disparities;
+            e = launch_lr_check((const uint16_t *)(base + L.sum), (int)q.width, (int)q.height, D,
+                                volume_stride(D), (int)q.cfg.lr_tolerance,
+                                (const int16_t *)(base + L.best), q.d_out, b->stream);
+            b->launches += 1;
+        }
+    }
+    return e;
+}
+
+// Records the streams form's launches on b->stream (being captured).
+vxh_status capture_streams(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
+                           hipError_t *err) {
+    using namespace vxh;
+    const uint32_t streams = (uint32_t)b->sub.size();
+    hipError_t e = launch_zero(b->d_error, 1, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(b->events[0], b->stream);
+    for (uint32_t s = 0; s < streams && e == hipSuccess; ++s)
+        e = hipStreamWaitEvent(b->sub[s], b->events[0], 0);
+    vxh_status st = VXH_OK;
+    for (size_t k = 0; k < order.size() && e == hipSuccess && st == VXH_OK; ++k) {
+        const uint32_t i = order[k];
+        const vxh_sgm_pair_desc &q = b->pairs[i];
+        const PairPlan &p = b->plan[i];
+        char *ws = (char *)b->ws[p.stream];
+        st = run_pipeline(q.d_left, q.d_right, q.width, q.height, &q.cfg, q.d_out, ws, p.mode,
+                          p.sched, b->sub[p.stream]);
+        if (st != VXH_OK) break;
+        e = launch_or_word(b->d_error, (const uint32_t *)(ws + kErrorWordOffset),
+                           b->sub[p.stream]);
+        b->launches += 6 + (q.cfg.lr_check ? 1 : 0);
+    }
+    for (uint32_t s = 0; s < streams && e == hipSuccess && st == VXH_OK; ++s) {
+        e = hipEventRecord(b->events[1 + s], b->sub[s]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->events[1 + s], 0);
+    }
+    *err = e;
+    return st;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -83,11 +433,15 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
     using namespace vxh;
     if (!out || (!pairs && n)) return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: null argument");
     *out = nullptr;
-    uint32_t streams = kDefaultStreams;
+    uint32_t streams = kDefaultStreams, form = kFormPersistent, blocks_per_cu = 0;
+    uint64_t budget = kDefaultBudget;
     if (opt) {
-        if (opt->streams > kMaxStreams)
+        if (opt->streams > kMaxStreams || opt->form > kFormStreams)
             return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: batch options");
         if (opt->streams) streams = opt->streams;
+        form = opt->form;
+        blocks_per_cu = opt->blocks_per_cu;
+        if (opt->budget_bytes) budget = opt->budget_bytes;
     }
     if (n == 0 || n > 65535) return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: batch size");
     if (hipSetDevice(device) != hipSuccess)
@@ -113,7 +467,7 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
             return bfail(VXH_ERR_INVALID_CONFIG,
                          "invalid config: unsupported lane layout (pair " + std::to_string(i) +
                              ", D=" + std::to_string(D) + ")");
-        sc.split = (int)q.split;
+        sc.split = form == kFormStreams ? (int)q.split : 0;
         plan[i].sched = sc;
         plan[i].mode = q.volume_format == 1 ? kOutU8
                        : q.volume_format == 2 ? kOutU16
@@ -122,24 +476,20 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
                                             (int)q.cfg.num_paths, plan[i].mode,
                                             q.cfg.lr_check != 0)
                                .total;
+        plan[i].ws_off = 0;
     }
-    // Largest pairs first, dealt round-robin to the streams.
+    // Largest pairs first.
     std::vector<uint32_t> order(n);
     for (uint32_t i = 0; i < n; ++i) order[i] = i;
     auto work = [&](uint32_t i) {
         return (size_t)pairs[i].width * pairs[i].height * pairs[i].cfg.num_disparities;
     };
     std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return work(a) > work(b); });
-    std::vector<size_t> ws_bytes(streams, 0);
-    for (uint32_t k = 0; k < n; ++k) {
-        PairPlan &p = plan[order[k]];
-        p.stream = k % streams;
-        ws_bytes[p.stream] = std::max(ws_bytes[p.stream], p.ws_bytes);
-    }
+                     [&](uint32_t a, uint32_t c) { return work(a) > work(c); });
 
     vxh_sgm_batch *b = new vxh_sgm_batch();
     b->device = device;
+    b->form = form;
     b->pairs.assign(pairs, pairs + n);
     b->plan = plan;
     auto gpu_fail = [&](const char *what, hipError_t e) {
@@ -149,58 +499,54 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
         return bfail(VXH_ERR_GPU, std::string("GPU error: ") + what + ": " + hipGetErrorString(e));
     };
     hipError_t e = hipSuccess;
-    for (uint32_t s = 0; s < streams && e == hipSuccess; ++s) {
-        void *p = nullptr;
-        e = hipMalloc(&p, ws_bytes[s]);
-        if (e == hipSuccess) b->ws.push_back(p);
+    // Sub streams: the streams form's pair streams, the persistent form's
+    // side-by-side aggregation groups (at most the three layout classes times
+    // two volume formats).
+    const uint32_t nsub = form == kFormStreams ? streams : 6;
+    if (form == kFormStreams) {
+        // One workspace per stream, sized for the largest pair dealt to it.
+        std::vector<size_t> ws_bytes(streams, 0);
+        for (uint32_t k = 0; k < n; ++k) {
+            PairPlan &p = b->plan[order[k]];
+            p.stream = k % streams;
+            ws_bytes[p.stream] = std::max(ws_bytes[p.stream], p.ws_bytes);
+        }
+        for (uint32_t s = 0; s < streams && e == hipSuccess; ++s) {
+            void *p = nullptr;
+            e = hipMalloc(&p, ws_bytes[s]);
+            if (e == hipSuccess) b->ws.push_back(p);
+        }
+        if (e != hipSuccess) return gpu_fail("batch workspaces", e);
+    } else {
+        if ((e = plan_persistent(b, order, budget, blocks_per_cu)) != hipSuccess)
+            return gpu_fail("batch plan", e);
     }
-    if (e != hipSuccess) return gpu_fail("batch workspaces", e);
     if ((e = hipMalloc((void **)&b->d_error, 256)) != hipSuccess)
         return gpu_fail("batch error word", e);
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess)
         return gpu_fail("stream", e);
-    for (uint32_t s = 0; s < streams; ++s) {
+    for (uint32_t s = 0; s < nsub; ++s) {
         hipStream_t st = nullptr;
         if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess)
             return gpu_fail("stream", e);
         b->sub.push_back(st);
     }
-    for (uint32_t s = 0; s <= streams; ++s) {
+    for (uint32_t s = 0; s <= nsub; ++s) {
         hipEvent_t ev = nullptr;
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
             return gpu_fail("event", e);
         b->events.push_back(ev);
     }
 
-    // Capture: clear the error word, fork to every stream, each pair's
-    // pipeline on its stream followed by folding its error word into the
-    // batch's, join.
     e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return gpu_fail("graph capture", e);
-    std::string what;
     vxh_status st = VXH_OK;
-    e = launch_zero(b->d_error, 1, b->stream);
-    if (e == hipSuccess) e = hipEventRecord(b->events[0], b->stream);
-    for (uint32_t s = 0; s < streams && e == hipSuccess; ++s)
-        e = hipStreamWaitEvent(b->sub[s], b->events[0], 0);
-    for (uint32_t k = 0; k < n && e == hipSuccess && st == VXH_OK; ++k) {
-        const uint32_t i = order[k];
-        const vxh_sgm_pair_desc &q = pairs[i];
-        const PairPlan &p = plan[i];
-        char *ws = (char *)b->ws[p.stream];
-        st = run_pipeline(q.d_left, q.d_right, q.width, q.height, &q.cfg, q.d_out, ws, p.mode,
-                          p.sched, b->sub[p.stream]);
-        if (st != VXH_OK) {
-            what = vxh_last_error();
-            break;
-        }
-        e = launch_or_word(b->d_error, (const uint32_t *)(ws + kErrorWordOffset),
-                           b->sub[p.stream]);
-        b->launches += 6 + (q.cfg.lr_check ? 1 : 0);
-    }
-    for (uint32_t s = 0; s < streams && e == hipSuccess && st == VXH_OK; ++s) {
-        e = hipEventRecord(b->events[1 + s], b->sub[s]);
-        if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->events[1 + s], 0);
+    std::string what;
+    if (form == kFormStreams) {
+        st = capture_streams(b, order, &e);
+        if (st != VXH_OK) what = vxh_last_error();
+    } else {
+        e = capture_persistent(b);
     }
     const hipError_t e2 = hipStreamEndCapture(b->stream, &b->graph);
     if (st != VXH_OK) {
@@ -250,17 +596,23 @@ vxh_status vxh_sgm_batch_stats_get(const vxh_sgm_batch *b, uint32_t pair,
     const vxh_sgm_pair_desc &q = b->pairs[pair];
     const PairPlan &p = b->plan[pair];
     const int c = vxh::layout_class((int)q.cfg.num_disparities);
-    out->stream = p.stream;
+    out->stream = b->form == kFormStreams ? p.stream : 0;
     out->lanes = (uint32_t)vxh::kClassLayouts[c][p.sched.other][0];
     out->per_lane = (uint32_t)vxh::kClassLayouts[c][p.sched.other][1];
     out->lanes_h = (uint32_t)vxh::kClassLayouts[c][p.sched.horiz][0];
     out->per_lane_h = (uint32_t)vxh::kClassLayouts[c][p.sched.horiz][1];
     out->volume_format = p.mode == vxh::kOutU8 ? 1u : 2u;
     out->workspace_bytes = p.ws_bytes;
+    out->chunk = b->form == kFormStreams ? 0 : p.stream;
+    out->form = b->form;
     return VXH_OK;
 }
 
 uint32_t vxh_sgm_batch_items(const vxh_sgm_batch *b) { return b ? b->launches : 0; }
+
+uint32_t vxh_sgm_batch_chunks(const vxh_sgm_batch *b) {
+    return b ? (b->form == kFormStreams ? 1u : (uint32_t)b->chunks.size()) : 0u;
+}
 
 void vxh_sgm_batch_destroy(vxh_sgm_batch *b) {
     if (!b) return;
@@ -273,6 +625,7 @@ void vxh_sgm_batch_destroy(vxh_sgm_batch *b) {
     for (hipStream_t s : b->sub) (void)hipStreamDestroy(s);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     for (void *p : b->ws) (void)hipFree(p);
+    if (b->tables) (void)hipFree(b->tables);
     if (b->d_error) (void)hipFree(b->d_error);
     delete b;
 }

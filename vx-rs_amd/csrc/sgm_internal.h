@@ -22,6 +22,14 @@ inline int num_scanlines(int dir, int w, int h) {
     return w + h - 1;
 }
 
+// Longest scanline of direction `dir`: the passes with the longest scanlines
+// are placed first in a launch so that their waves start first.
+inline int scan_length(int dir, int w, int h) {
+    if (kDirY[dir] == 0) return w;
+    if (kDirX[dir] == 0) return h;
+    return w < h ? w : h;
+}
+
 // Per-pixel stride, in u16 elements, of the summed cost volume: D rounded up
 // to 16 so every pixel's disparity slice starts 32-byte aligned and each lane's
 // packed segment of it never straddles a pixel.
@@ -147,6 +155,72 @@ struct MultiPass {
     PassDesc pass[kMaxPasses];
 };
 
+
+// ---- Multi-pair launches of the batched form ----
+// Each launch covers the blocks of all its jobs; jobs are sorted by
+// first_block, the first at 0, and block b belongs to the job whose
+// [first_block, next job's first_block) holds it.
+struct CensusJob {   // one image: tiles of 64 x 4 pixels, nbx per tile row
+    const uint8_t *img;
+    uint2 *out;
+    uint2 *zero_flags;       // as launch_census
+    uint32_t *zero_words;
+    size_t nwords;
+    int w, h, rx, ry, nbx;
+    uint32_t first_block;
+};
+struct CostJob {     // one pair: tiles of 256 pixels of one row, nbx per row
+    const uint2 *lc, *rc;
+    uint8_t *cost;
+    int w, h, D, Dp;
+    uint32_t bias;
+    int nbx;
+    uint32_t first_block;
+};
+struct WtaJob {      // one pair, as launch_wta_sum: 16 pixels per block
+    const uint8_t *vol0;
+    size_t vstride;
+    const uint8_t *hi0;
+    const uint2 *flags;
+    float *out;
+    int16_t *best;
+    uint16_t *sum_out;
+    int nv, mode, npx, D, Dp;
+    uint32_t first_block;
+};
+hipError_t launch_census_multi(const CensusJob *d_jobs, int njobs, uint32_t blocks,
+                               hipStream_t s);
+hipError_t launch_cost_multi(const CostJob *d_jobs, int njobs, uint32_t blocks, hipStream_t s);
+hipError_t launch_wta_multi(const WtaJob *d_jobs, int njobs, uint32_t blocks, hipStream_t s);
+
+// Persistent aggregation over the direction passes of many pairs.  A pass is
+// one direction of one pair (p carries its volume, direction and scanline
+// count; layout indexes its class's lane layouts).  Item (pass, wave) is the
+// scanline group `wave` of a pass.  Waves of block b take tickets from
+// counters[b % 8] (the XCD the hardware places the block on) and run item
+// items[item_off[k] + ticket] of that XCD's list until it is exhausted; the
+// host orders each list pair by pair, so an XCD works through contiguous
+// scanlines of one pair's passes at a time.  counters must be zero at launch.
+struct BatchPass {
+    PassParams p;
+    int layout;
+};
+struct BatchItem {
+    uint32_t pass, wave;
+};
+struct BatchPaths {
+    const BatchPass *passes;
+    const BatchItem *items;
+    uint32_t *counters;
+    uint32_t item_off[8], item_cnt[8];
+};
+hipError_t launch_batch_paths_c0(int mode, const BatchPaths &b, int blocks, hipStream_t s);
+hipError_t launch_batch_paths_c1(int mode, const BatchPaths &b, int blocks, hipStream_t s);
+hipError_t launch_batch_paths_c2(int mode, const BatchPaths &b, int blocks, hipStream_t s);
+// Blocks of 256 threads of the class's persistent kernel one CU holds at once.
+int batch_paths_occupancy_c0(int mode);
+int batch_paths_occupancy_c1(int mode);
+int batch_paths_occupancy_c2(int mode);
 
 // Census of one image; with `zero_flags` also clears the 8 flag bytes of
 // every pixel (the kOutU8 exception flags of the pass that follows), with

@@ -18,16 +18,18 @@ using namespace dev;
 // image directly.
 constexpr int kCtW = 64 + 8, kCtH = 4 + 6;
 
-__global__ void __launch_bounds__(256)
-census_kernel(const uint8_t *__restrict__ img, int w, int h, int rx, int ry,
-              uint2 *__restrict__ out, uint2 *__restrict__ zero_flags,
-              uint32_t *__restrict__ zero_words, size_t nwords) {
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y = blockIdx.y * 4 + threadIdx.y;
+// Tile (bx, by) of the census of one image (block-uniform arguments).
+__device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, int w, int h, int rx,
+                                            int ry, uint2 *__restrict__ out,
+                                            uint2 *__restrict__ zero_flags,
+                                            uint32_t *__restrict__ zero_words, size_t nwords,
+                                            int bx, int by) {
+    const int x = bx * 64 + threadIdx.x;
+    const int y = by * 4 + threadIdx.y;
     uint2 word;
     if (rx == 4 && ry == 3) {   // block-uniform
         __shared__ uint8_t tile[kCtH * kCtW];
-        const int x0 = blockIdx.x * 64 - 4, y0 = blockIdx.y * 4 - 3;
+        const int x0 = bx * 64 - 4, y0 = by * 4 - 3;
         for (int t = threadIdx.y * 64 + threadIdx.x; t < kCtH * kCtW; t += 256) {
             const int r = t / kCtW, c = t - r * kCtW;
             tile[t] = img[(size_t)min(max(y0 + r, 0), h - 1) * w + min(max(x0 + c, 0), w - 1)];
@@ -55,6 +57,38 @@ census_kernel(const uint8_t *__restrict__ img, int w, int h, int rx, int ry,
     for (size_t k = i; k < nwords; k += (size_t)w * h) zero_words[k] = 0;
 }
 
+__global__ void __launch_bounds__(256)
+census_kernel(const uint8_t *__restrict__ img, int w, int h, int rx, int ry,
+              uint2 *__restrict__ out, uint2 *__restrict__ zero_flags,
+              uint32_t *__restrict__ zero_words, size_t nwords) {
+    census_tile(img, w, h, rx, ry, out, zero_flags, zero_words, nwords, (int)blockIdx.x,
+                (int)blockIdx.y);
+}
+
+// Index of the table entry whose [first_block, next first_block) holds block
+// b (entries sorted by first_block, the first at 0; b wave-uniform).
+template <typename E>
+__device__ __forceinline__ int find_entry(const E *__restrict__ t, int n, uint32_t b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t[mid].first_block <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Census of many images in one launch (blockDim (64, 4)): block b belongs to
+// the entry whose block range holds it.
+__global__ void __launch_bounds__(256)
+census_multi_kernel(const CensusJob *__restrict__ jobs, int njobs) {
+    const int j = find_entry(jobs, njobs, blockIdx.x);
+    const CensusJob &c = jobs[j];
+    const int local = (int)(blockIdx.x - c.first_block);
+    census_tile(c.img, c.w, c.h, c.rx, c.ry, c.out, c.zero_flags, c.zero_words, c.nwords,
+                local % c.nbx, local / c.nbx);
+}
+
 // Matching-cost volume of a pair from its census (SGMStereo.metal:133-142),
 // computed once and read by every direction pass.  A block (4 waves) owns
 // 256 pixels of one row: it stages the left census words of the tile and the
@@ -70,16 +104,17 @@ constexpr int kCcOutRow = kCcSeg + 16;            // LDS row of one pixel (bank 
 constexpr int kCcSpanR = kCcTile + 256;           // right census words (D - 1 <= 255 before)
 constexpr size_t kCcLds = (size_t)(kCcSpanR + kCcTile) * 8 + 4 * 64 * kCcOutRow;
 
-__global__ void __launch_bounds__(256)
-cost_kernel(const uint2 *__restrict__ lc, const uint2 *__restrict__ rc, int w, int h, int D,
-            int Dp, uint32_t bias, uint8_t *__restrict__ cost) {
-    extern __shared__ __align__(16) uint8_t lds[];
+// Tile bx of row y of a pair's cost volume (block-uniform arguments; `lds`
+// holds kCcLds bytes).
+__device__ __forceinline__ void cost_tile(const uint2 *__restrict__ lc,
+                                          const uint2 *__restrict__ rc, int w, int h, int D,
+                                          int Dp, uint32_t bias, uint8_t *__restrict__ cost,
+                                          int bx, int y, uint8_t *lds) {
     uint2 *cen_r = (uint2 *)lds;                            // kCcSpanR words
     uint2 *cen_l = cen_r + kCcSpanR;                        // kCcTile words
     uint8_t *out_all = (uint8_t *)(cen_l + kCcTile);        // 4 x 64 x kCcOutRow, 16-aligned
     const int t = threadIdx.x;
-    const int y = blockIdx.y;
-    const int x0 = blockIdx.x * kCcTile;
+    const int x0 = bx * kCcTile;
     const int xs0 = max(0, x0 - (D - 1));                   // first right-census pixel
     const int xs1 = min(w, x0 + kCcTile);
     const int nr = xs1 - xs0, nl = xs1 - x0;
@@ -156,6 +191,24 @@ cost_kernel(const uint2 *__restrict__ lc, const uint2 *__restrict__ rc, int w, i
     }
 }
 
+__global__ void __launch_bounds__(256)
+cost_kernel(const uint2 *__restrict__ lc, const uint2 *__restrict__ rc, int w, int h, int D,
+            int Dp, uint32_t bias, uint8_t *__restrict__ cost) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    cost_tile(lc, rc, w, h, D, Dp, bias, cost, (int)blockIdx.x, (int)blockIdx.y, lds);
+}
+
+// Cost volumes of many pairs in one launch: block b belongs to the entry
+// whose block range holds it (tiles of kCcTile pixels, row-major).
+__global__ void __launch_bounds__(256)
+cost_multi_kernel(const CostJob *__restrict__ jobs, int njobs) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const int j = find_entry(jobs, njobs, blockIdx.x);
+    const CostJob &c = jobs[j];
+    const int local = (int)(blockIdx.x - c.first_block);
+    cost_tile(c.lc, c.rc, c.w, c.h, c.D, c.Dp, c.bias, c.cost, local % c.nbx, local / c.nbx, lds);
+}
+
 __global__ void or_word_kernel(uint32_t *dst, const uint32_t *src) {
     if (threadIdx.x == 0) *dst |= *src;
 }
@@ -178,6 +231,16 @@ wta_sum_kernel(const uint8_t *__restrict__ vol0, size_t vstride,
                   sum_out);
 }
 
+// Sum + winner-take-all of many pairs in one launch (16 pixels per block).
+__global__ void __launch_bounds__(256)
+wta_multi_kernel(const WtaJob *__restrict__ jobs, int njobs) {
+    const int j = find_entry(jobs, njobs, blockIdx.x);
+    const WtaJob &c = jobs[j];
+    const int pix = (int)((blockIdx.x - c.first_block) * 16 + (threadIdx.x >> 4));
+    wta_sum_pixel(c.vol0, c.vstride, c.hi0, c.flags, c.nv, c.Dp > 128 ? 2 : 1, c.mode, pix,
+                  c.npx, c.D, c.Dp, c.out, c.best, c.sum_out);
+}
+
 __global__ void __launch_bounds__(256)
 lr_kernel(const uint16_t *__restrict__ vol, int w, int h, int D, int Dp, int tol,
           const int16_t *__restrict__ best, float *__restrict__ out) {
@@ -191,14 +254,6 @@ copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
-}
-
-// Scanline length of a direction: the passes with the longest scanlines are
-// placed first in a launch so that their waves start first.
-int scan_length(int dir, int w, int h) {
-    if (kDirY[dir] == 0) return w;
-    if (kDirX[dir] == 0) return h;
-    return std::min(w, h);
 }
 
 }  // namespace
@@ -228,6 +283,25 @@ hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
     dim3 grid((w + 63) / 64, (h + 3) / 4);
     hipLaunchKernelGGL(census_kernel, grid, block, 0, s, img, w, h, rx, ry, out, zero_flags,
                        zero_words, zero_words ? nwords : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_census_multi(const CensusJob *d_jobs, int njobs, uint32_t blocks,
+                               hipStream_t s) {
+    if (njobs < 1 || blocks < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(census_multi_kernel, dim3(blocks), dim3(64, 4), 0, s, d_jobs, njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_cost_multi(const CostJob *d_jobs, int njobs, uint32_t blocks, hipStream_t s) {
+    if (njobs < 1 || blocks < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(cost_multi_kernel, dim3(blocks), dim3(256), kCcLds, s, d_jobs, njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_wta_multi(const WtaJob *d_jobs, int njobs, uint32_t blocks, hipStream_t s) {
+    if (njobs < 1 || blocks < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(wta_multi_kernel, dim3(blocks), dim3(256), 0, s, d_jobs, njobs);
     return hipGetLastError();
 }
 

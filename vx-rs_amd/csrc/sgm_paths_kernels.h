@@ -132,6 +132,30 @@ paths_kernel(MultiPass m) {
     run_layout<C, MODE, false>(p, layout, wave, kRoleWhole, nullptr, nullptr, nullptr);
 }
 
+// The persistent aggregation of the batched form (see BatchPaths): every
+// wave takes items from its XCD's list until the list is exhausted.
+template <int C, int MODE>
+__global__ void __launch_bounds__(256)
+batch_paths_kernel(BatchPaths b) {
+    const int k = (int)(blockIdx.x & 7);
+    const uint32_t n = b.item_cnt[k];
+    const BatchItem *__restrict__ items = b.items + b.item_off[k];
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        uint32_t tk = 0;
+        if (lane == 0)
+            tk = __hip_atomic_fetch_add(b.counters + k, 1u, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t ticket = (uint32_t)__builtin_amdgcn_readfirstlane((int)tk);
+        if (ticket >= n) break;   // wave-uniform
+        const BatchItem it = items[ticket];
+        const BatchPass &d = b.passes[it.pass];
+        const PassParams p = d.p;
+        run_layout<C, MODE, false>(p, d.layout, (int)it.wave, kRoleWhole, nullptr, nullptr,
+                                   nullptr);
+    }
+}
+
 }  // namespace
 
 }  // namespace vxh
@@ -159,5 +183,27 @@ paths_kernel(MultiPass m) {
             return hipErrorInvalidValue;                                                   \
         }                                                                                  \
         return hipGetLastError();                                                          \
+    }                                                                                      \
+    hipError_t launch_batch_paths_c##CC(int mode, const BatchPaths &b, int blocks,         \
+                                        hipStream_t s) {                                   \
+        if (mode == kOutU8)                                                                \
+            hipLaunchKernelGGL((batch_paths_kernel<CC, kOutU8>), dim3(blocks), dim3(256),  \
+                               0, s, b);                                                   \
+        else if (mode == kOutU16)                                                          \
+            hipLaunchKernelGGL((batch_paths_kernel<CC, kOutU16>), dim3(blocks), dim3(256), \
+                               0, s, b);                                                   \
+        else                                                                               \
+            return hipErrorInvalidValue;                                                   \
+        return hipGetLastError();                                                          \
+    }                                                                                      \
+    int batch_paths_occupancy_c##CC(int mode) {                                            \
+        int n = 0;                                                                         \
+        const hipError_t e =                                                               \
+            mode == kOutU8                                                                 \
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(                            \
+                      &n, batch_paths_kernel<CC, kOutU8>, 256, 0)                          \
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(                            \
+                      &n, batch_paths_kernel<CC, kOutU16>, 256, 0);                        \
+        return e == hipSuccess ? n : 0;                                                    \
     }                                                                                      \
     }

@@ -475,12 +475,17 @@ class vxh_sgm_pair_desc(ctypes.Structure):
 
 
 class vxh_sgm_batch_options(ctypes.Structure):
-    _fields_ = [("streams", _u32)]
+    _fields_ = [("streams", _u32), ("form", _u32), ("blocks_per_cu", _u32),
+                ("reserved", _u32), ("budget_bytes", ctypes.c_uint64)]
 
 
 class vxh_sgm_batch_stats(ctypes.Structure):
     _fields_ = [("stream", _u32), ("lanes", _u32), ("per_lane", _u32), ("lanes_h", _u32),
-                ("per_lane_h", _u32), ("volume_format", _u32), ("workspace_bytes", ctypes.c_uint64)]
+                ("per_lane_h", _u32), ("volume_format", _u32), ("workspace_bytes", ctypes.c_uint64),
+                ("chunk", _u32), ("form", _u32)]
+
+
+BATCH_FORMS = {"persistent": 0, "streams": 1}
 
 
 _sig("vxh_sgm_batch_create", ctypes.c_int, ctypes.c_int, ctypes.POINTER(vxh_sgm_pair_desc),
@@ -492,6 +497,7 @@ _sig("vxh_sgm_batch_launch", ctypes.c_int, _vp, _vp)
 _sig("vxh_sgm_batch_run", ctypes.c_int, _vp)
 _sig("vxh_sgm_batch_stats_get", ctypes.c_int, _vp, _u32, ctypes.POINTER(vxh_sgm_batch_stats))
 _sig("vxh_sgm_batch_items", _u32, _vp)
+_sig("vxh_sgm_batch_chunks", _u32, _vp)
 _sig("vxh_sgm_batch_destroy", None, _vp)
 _sig("vxh_sgm_batch_last_error", ctypes.c_char_p)
 
@@ -509,7 +515,9 @@ class SGMBatch:
     schedule[, volume_format[, split]]) with device addresses; schedule is
     (lanes, per_lane) for every direction, ((lanes_h, per_lane_h),
     (lanes, per_lane)), or None; volume_format "auto", "u8" or "u16"; split
-    a bool.  `options` is (streams,) or None (default: 4 streams)."""
+    a bool.  `options` is None (defaults) or a dict with any of "form"
+    ("persistent" or "streams"), "streams", "blocks_per_cu", "budget_bytes";
+    a tuple (streams,) selects the streams form with that many streams."""
 
     def __init__(self, pairs, device=0, options=None):
         arr = (vxh_sgm_pair_desc * len(pairs))()
@@ -526,7 +534,15 @@ class SGMBatch:
             arr[i] = vxh_sgm_pair_desc(l, r, o, w, h, cfg.to_c(), os[0], os[1], hs[0],
                                        hs[1], fmt, split)
         h = _vp()
-        opt = ctypes.byref(vxh_sgm_batch_options(*options)) if options else None
+        opt = None
+        if isinstance(options, dict):
+            o = dict(options)
+            o["form"] = BATCH_FORMS[o.get("form", "persistent")]
+            opt = ctypes.byref(vxh_sgm_batch_options(
+                o.get("streams", 0), o["form"], o.get("blocks_per_cu", 0), 0,
+                o.get("budget_bytes", 0)))
+        elif options:
+            opt = ctypes.byref(vxh_sgm_batch_options(options[0], 1, 0, 0, 0))
         _check_batch(lib.vxh_sgm_batch_create_ex(device, arr, len(pairs), opt, ctypes.byref(h)))
         self._h = h
         self.n = len(pairs)
@@ -542,7 +558,11 @@ class SGMBatch:
         _check_batch(lib.vxh_sgm_batch_stats_get(self._h, i, ctypes.byref(s)))
         return {"stream": s.stream, "schedule": ((s.lanes_h, s.per_lane_h), (s.lanes, s.per_lane)),
                 "volume_format": {1: "u8", 2: "u16"}[s.volume_format],
-                "workspace_bytes": s.workspace_bytes}
+                "workspace_bytes": s.workspace_bytes, "chunk": s.chunk,
+                "form": {0: "persistent", 1: "streams"}[s.form]}
+
+    def chunks(self):
+        return lib.vxh_sgm_batch_chunks(self._h)
 
     @property
     def items(self):

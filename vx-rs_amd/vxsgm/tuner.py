@@ -11,9 +11,13 @@ speed does, and differently for the two forms:
       directions (the longest scanlines; wide groups cut the critical path)
       and one to the others, and every pair of layouts is timed through the
       per-pair pipeline.
-  tune_batch(pairs)  a batch runs every pair's pipeline with its own
-      schedule over several streams at once; what is tuned is the number of
-      streams.  Timed on the batch itself, replayed from its hipGraph.
+  tune_batch(pairs)  a batch either runs multi-pair launches around one
+      persistent aggregation launch per layout class (throughput-bound: the
+      tuner tries the per-pair layouts "default" and "widest", the one with
+      the most disparities per lane and so the fewest instructions per entry,
+      and the aggregation blocks per CU) or runs the per-pair pipelines over
+      several streams (the tuner tries the stream count).  Timed on the batch
+      itself, replayed from its hipGraph.
 
 Timings use HIP events on the launch stream; the median of `reps` replays
 after one warm-up wins.  Winners are cached in a JSON file keyed by form and
@@ -28,8 +32,10 @@ from vxsgm import hip
 
 CLASS_LAYOUTS = [[(32, 2), (16, 4), (8, 8)], [(64, 2), (32, 4), (16, 8)],
                  [(64, 4), (32, 8)]]
-# Batch launch options tried: (streams,).
-BATCH_OPTIONS = [(1,), (2,), (3,), (4,), (6,)]
+# Batch candidates tried: (options, per-pair layout policy).
+BATCH_CANDIDATES = ([({"form": "streams", "streams": s}, "default") for s in (2, 4, 6)] +
+                    [({"form": "persistent", "blocks_per_cu": b}, lay)
+                     for b in (0, 2) for lay in ("default", "widest")])
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 
@@ -40,6 +46,26 @@ def layout_class(D):
 def candidates(D):
     """The lane layouts of D's class."""
     return list(CLASS_LAYOUTS[layout_class(D)])
+
+
+def widest(D):
+    """The layout of D's class with the most disparities per lane."""
+    return max(candidates(D), key=lambda lk: lk[1])
+
+
+def apply_layouts(descs, policy):
+    """Pair descriptors (as for SGMBatch) with the per-pair schedule of
+    `policy`: "default" leaves them as given, "widest" sets every direction of
+    every pair to widest(D)."""
+    if policy == "default":
+        return list(descs)
+    out = []
+    for d in descs:
+        d = list(d)
+        lay = widest(d[5].num_disparities)
+        d[6] = (lay, lay)
+        out.append(tuple(d))
+    return out
 
 
 def schedules(D):
@@ -92,21 +118,22 @@ class Tuner:
         return statistics.median(ms)
 
     def tune_batch(self, descs, key="batch", reps=3, force=False):
-        """Best launch options (streams,) for the batch of pair descriptors
-        `descs` (as for SGMBatch)."""
+        """Best (options, layout policy) for the batch of pair descriptors
+        `descs` (as for SGMBatch); apply the policy with apply_layouts."""
         if not force and key in self.table:
-            return tuple(self.table[key]["options"])
+            e = self.table[key]
+            return e["options"], e["layouts"]
         stream = hip.Stream()
         times = {}
-        for opt in BATCH_OPTIONS:
-            b = vxsgm.SGMBatch(descs, options=opt)
-            times["/".join(str(v) for v in opt)] = self._time(lambda: b.launch(stream.handle),
-                                                             stream, reps)
+        for opt, lay in BATCH_CANDIDATES:
+            b = vxsgm.SGMBatch(apply_layouts(descs, lay), options=opt)
+            name = ",".join(f"{k}={v}" for k, v in sorted(opt.items())) + f",layouts={lay}"
+            times[name] = self._time(lambda: b.launch(stream.handle), stream, reps)
             b.close()
         best = min(times, key=times.get)
-        opt = tuple(int(v) for v in best.split("/"))
-        self.table[key] = {"options": list(opt), "ms": times}
-        return opt
+        opt, lay = BATCH_CANDIDATES[list(times).index(best)]
+        self.table[key] = {"options": opt, "layouts": lay, "ms": times}
+        return opt, lay
 
     def tune_plain(self, sgm, w, h, D, paths, reps=5, force=False):
         """Sweeps every schedule of the class for one pair through the

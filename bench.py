@@ -75,8 +75,8 @@ def parse():
                          "known-traffic reference for HBM byte counters) and exit")
     ap.add_argument("--schedule", default="",
                     help="lane layouts of the plain path, horizontal/other (e.g. "
-                         "64x2/16x8) or one LxK for all directions; default: tuned or "
-                         "the built-in default")
+                         "64x2/16x8) or one LxK for all directions, optionally followed by "
+                         "/split or /splitall; default: tuned or the built-in default")
     ap.add_argument("--volume-format", default="auto", choices=["auto", "u8", "u16"],
                     help="per-direction volume storage of the plain path")
     ap.add_argument("--batch-options", default="",
@@ -88,7 +88,7 @@ def parse():
 
 def sched_label(sched, split):
     (a, b), (c, d) = sched
-    return f"{a}x{b}/{c}x{d}" + ("/split" if split else "")
+    return f"{a}x{b}/{c}x{d}" + ("/split" if split == 1 else "/splitall" if split else "")
 
 
 def committed_traffic(schedule, workload):
@@ -366,8 +366,10 @@ def main():
     tuner = None
     sgm.set_volume_format(D, args.volume_format)
     if args.schedule:
-        parts = [tuple(int(v) for v in p.split("x")) for p in args.schedule.lower().split("/")]
+        toks = args.schedule.lower().split("/")
+        parts = [tuple(int(v) for v in p.split("x")) for p in toks if "x" in p]
         sgm.set_schedule(D, (parts[0], parts[-1]))
+        sgm.set_split(D, 2 if "splitall" in toks else (1 if "split" in toks else 0))
     elif not args.no_tune:
         from vxsgm.tuner import Tuner
         tuner = Tuner(path=None)

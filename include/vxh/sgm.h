@@ -270,10 +270,11 @@ vxh_status vxh_sgm_wta(const uint16_t *d_volume, uint32_t w, uint32_t h,
 vxh_status vxh_sgm_set_schedule(vxh_sgm *sgm, uint32_t num_disparities,
                                 uint32_t lanes_h, uint32_t per_lane_h,
                                 uint32_t lanes, uint32_t per_lane);
-/* Whether this handle splits scanlines into two halves aggregated at once
- * (1) or not (0, the default), for pairs with exactly `num_disparities`
- * disparities.  Split are the directions whose scanlines are at least 96
- * pixels and longer than 2/3 of the longest direction's.  The second half
+/* Whether this handle splits scanlines into two halves aggregated at once,
+ * for pairs with exactly `num_disparities` disparities: 0 never (the
+ * default), 1 in the directions whose scanlines are longer than 2/3 of the
+ * longest direction's, 2 in every direction; in either case only scanlines
+ * of at least 256 pixels.  The second half
  * starts from the state every path starts from; the first half's wave then
  * continues into it until its state equals the second half's bit for bit
  * and overwrites the values before that point, so the result is identical

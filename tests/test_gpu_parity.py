@@ -342,13 +342,14 @@ def test_schedule_setting_validated(ctx):
 
 
 @pytest.mark.parametrize("w,h,D", [(600, 80, 64), (110, 420, 100), (512, 96, 128), (300, 40, 256),
-                                   (257, 30, 32), (96, 96, 16)])
+                                   (257, 30, 32), (96, 96, 16), (300, 290, 48), (270, 700, 96)])
 @pytest.mark.parametrize("fmt", ["u8", "u16"])
 def test_split_scanlines_match_oracle(ctx, w, h, D, fmt):
-    """Scanlines of 256 pixels or more (in the directions whose scanlines are
-    longest) run as a head and a warmed-up tail at once; the head's check and
-    any overwrite of the tail's first steps must leave the oracle's map, with
-    and without the LR check, and identical to the unsplit run (96 x 96: no
+    """Scanlines of 256 pixels or more (level 1: in the directions whose
+    scanlines are longest; level 2: in every direction, diagonals included)
+    run as a head and a warmed-up tail at once; the head's check and any
+    overwrite of the tail's first steps must leave the oracle's map, with and
+    without the LR check, and identical to the unsplit run (96 x 96: no
     scanline is split)."""
     left, right, _ = vxsgm.synth_pair(w, h, D, 4, 1, 5000 + w + h + D)
     s = vxsgm.SGMStereo(ctx)
@@ -357,17 +358,19 @@ def test_split_scanlines_match_oracle(ctx, w, h, D, fmt):
         for lr in (False, True):
             cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=8, lr_check=lr)
             want = O.sgm(left, right, D, num_paths=8, lr_check=lr)
-            s.set_split(D, True)
-            assert s.split(D)
-            assert_same_disp(s.compute_host(left, right, w, h, cfg), want)
+            for level in (1, 2):
+                s.set_split(D, level)
+                assert s.split(D) == level
+                assert_same_disp(s.compute_host(left, right, w, h, cfg), want)
             s.set_split(D, False)
             assert_same_disp(s.compute_host(left, right, w, h, cfg), want)
     finally:
         s.close()
 
 
+@pytest.mark.parametrize("level", [1, 2])
 @pytest.mark.parametrize("p2", [300, 2000])
-def test_split_scanlines_with_wraps(ctx, p2):
+def test_split_scanlines_with_wraps(ctx, p2, level):
     """u8 volumes forced with a large P2: the tail's and head's exception flags
     and high bytes in the re-run steps must resolve to the oracle's sums (a
     flagged step never counts as met, so heads run on past the early range and
@@ -378,15 +381,16 @@ def test_split_scanlines_with_wraps(ctx, p2):
     s = vxsgm.SGMStereo(ctx)
     try:
         s.set_volume_format(D, "u8")
-        s.set_split(D, True)
+        s.set_split(D, level)
         got = s.compute_host(left, right, w, h, cfg)
     finally:
         s.close()
     assert_same_disp(got, O.sgm(left, right, D, p2=p2, num_paths=8))
 
 
+@pytest.mark.parametrize("level", [1, 2])
 @pytest.mark.parametrize("fmt", ["u8", "u16", "auto"])
-def test_split_slow_convergence(ctx, fmt):
+def test_split_slow_convergence(ctx, fmt, level):
     """Penalties far above the costs (P1 = 1000, P2 = 2000) keep a fresh
     start's state apart from the true one for longer than the tail's warm-up:
     heads then continue through the tail's early range, wait for the tail to
@@ -397,7 +401,7 @@ def test_split_slow_convergence(ctx, fmt):
     s = vxsgm.SGMStereo(ctx)
     try:
         s.set_volume_format(D, fmt)
-        s.set_split(D, True)
+        s.set_split(D, level)
         got = s.compute_host(left, right, w, h, cfg)
     finally:
         s.close()
@@ -425,6 +429,9 @@ def test_720p_d128_8path_bit_exact(ctx, sgm):
         s.set_schedule(128, ((16, 8), (16, 8)))
         assert s.split(128)
         assert_same_disp(s.compute_host(left, right, 1280, 720, cfg), want)
+        s.set_split(128, 2)
+        s.set_schedule(128, ((32, 4), (16, 8)))
+        assert_same_disp(s.compute_host(left, right, 1280, 720, cfg), want)
     finally:
         s.close()
 
@@ -443,9 +450,10 @@ def test_1080p_d256_8path_matches_golden_hash(ctx, sgm):
         got = s.compute_host(left, right, 1920, 1080, cfg)
         assert hashlib.sha256(bits(got).tobytes()).hexdigest() == golden["1080p_d256_8path"]
         s.set_volume_format(256, "auto")
-        s.set_split(256, True)
-        got = s.compute_host(left, right, 1920, 1080, cfg)
-        assert hashlib.sha256(bits(got).tobytes()).hexdigest() == golden["1080p_d256_8path"]
+        for level in (1, 2):
+            s.set_split(256, level)
+            got = s.compute_host(left, right, 1920, 1080, cfg)
+            assert hashlib.sha256(bits(got).tobytes()).hexdigest() == golden["1080p_d256_8path"]
     finally:
         s.close()
 

@@ -456,7 +456,7 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
                          std::string("pair ") + std::to_string(i) + ": " + vxh_last_error());
         if (!q.d_left || !q.d_right || !q.d_out)
             return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: null pair buffer");
-        if (q.volume_format > 2 || q.split > 1)
+        if (q.volume_format > 2 || q.split > 2)
             return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: pair options");
         const int D = (int)q.cfg.num_disparities;
         const int cls = layout_class(D);

@@ -75,7 +75,8 @@ constexpr uint32_t kSplitWaitPolls = 1u << 22;
 struct Schedule {
     int horiz;   // index into kClassLayouts[layout_class(D)]
     int other;
-    int split;   // 1: long scanlines run as head and tail halves at once
+    int split;   // long scanlines run as head and tail halves at once: 1 those of
+                 // the longest passes, 2 those of every pass; 0 none
     uint32_t wait_polls = kSplitWaitPolls;   // the heads' bounded wait
 };
 

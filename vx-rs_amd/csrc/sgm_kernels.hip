@@ -268,7 +268,7 @@ Schedule default_schedule(int D) {
 bool schedule_valid(Schedule sc, int D) {
     const int n = kClassCount[layout_class(D)];
     return sc.horiz >= 0 && sc.horiz < n && sc.other >= 0 && sc.other < n &&
-           (sc.split == 0 || sc.split == 1);
+           (sc.split >= 0 && sc.split <= 2);
 }
 
 hipError_t launch_zero(uint32_t *p, size_t n, hipStream_t s) {
@@ -354,9 +354,10 @@ hipError_t launch_paths(const PassParams &base, const int *dirs, uint8_t *const 
         blocks += d.nblocks;
         d.nwaves = waves;
     }
-    // Split the passes whose scanlines are long enough and set the critical
-    // path: longer than 2/3 of the longest (at 1280x720 the horizontal ones,
-    // whose halves then run no longer than the vertical scanlines).
+    // Split the passes whose scanlines are long enough: at split level 1
+    // those that set the critical path, longer than 2/3 of the longest (at
+    // 1280x720 the horizontal ones, whose halves then run no longer than the
+    // vertical scanlines); at level 2 every pass of at least kSplitMin.
     int longest = 0;
     for (int j = 0; j < n; ++j) longest = std::max(longest, scan_length(m.pass[j].dir, base.w, base.h));
     int waves_all = 0, nsplit = 0;
@@ -364,7 +365,7 @@ hipError_t launch_paths(const PassParams &base, const int *dirs, uint8_t *const 
     for (int j = 0; j < n; ++j) {
         PassDesc &d = m.pass[j];
         const int len = scan_length(d.dir, base.w, base.h);
-        d.split = sc.split && len >= kSplitMin && 3 * len > 2 * longest;
+        d.split = sc.split && len >= kSplitMin && (sc.split >= 2 || 3 * len > 2 * longest);
         d.first_wave = waves_all;
         waves_all += d.nwaves;
         nsplit += d.split;

@@ -563,9 +563,12 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
         SPLIT && role == kRoleHead ? min(rounds, __builtin_amdgcn_readfirstlane(vfirst) / NB)
                                    : rounds;
     // Rounds whose steps and prefetches all lie inside every scanline of the
-    // wave run the FULL step (every lane of the wave stores when L * K <= Dp;
-    // a tail's warm-up does not store).
-    const bool full_ok = L * K <= Dp && !(SPLIT && role == kRoleTail);
+    // wave run the FULL step (every lane of the wave stores when L * K <= Dp).
+    // A tail's warm-up stores nothing, so its FULL rounds start after the
+    // warm-up (every split scanline of a tail wave has the same warm-up, and
+    // a wave holding a scanline too short to split has no FULL rounds).
+    const bool full_ok = L * K <= Dp;
+    const int full_from = SPLIT && role == kRoleTail ? (kSplitWarm + NB - 1) / NB : 0;
     const int full_rounds =
         full_ok ? min(rounds_main, max(0, (__builtin_amdgcn_readfirstlane(minlen) - PF) / NB)) : 0;
     PairBuf<K> buf[NB];
@@ -576,9 +579,22 @@ __device__ __forceinline__ void agg_pair_wave(const PassParams &p, int s_first,
         __builtin_amdgcn_sched_barrier(0);
     }
     int r = 0;
+    if constexpr (SPLIT) {
+        // A tail's warm-up rounds (predicated stores).
+        const int warm = min(full_from, rounds_main);
+        for (; r < warm; ++r) {
+            const int t = r * NB;
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                issue(buf[(u + PF) % NB], t + u + PF, std::false_type{});
+                step(buf[u], tbuf[0], t + u, std::false_type{}, std::false_type{});
+            }
+            if (r == publish_round) publish();
+        }
+    }
 #pragma unroll
     for (int hh = 0; hh < 2; ++hh)
-        fptr[hh] = cost + (ptrdiff_t)(pix0[hh] + PF * pstep) * Dp;
+        fptr[hh] = cost + (ptrdiff_t)(pix0[hh] + (r * NB + PF) * pstep) * Dp;
     for (; r < full_rounds; ++r) {
         const int t = r * NB;
 #pragma unroll

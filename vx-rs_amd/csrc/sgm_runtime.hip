@@ -992,7 +992,7 @@ vxh_status vxh_sgm_set_schedule(vxh_sgm *s, uint32_t nd, uint32_t lanes_h,
 }
 
 vxh_status vxh_sgm_set_split(vxh_sgm *s, uint32_t nd, uint32_t split) {
-    if (!s || nd < 1 || nd > 256 || split > 1)
+    if (!s || nd < 1 || nd > 256 || split > 2)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: split setting");
     const vxh::Schedule cur = schedule_for(s, (int)nd);
     std::lock_guard<std::mutex> g(s->mu);

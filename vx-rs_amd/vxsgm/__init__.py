@@ -417,13 +417,15 @@ class SGMStereo:
         return (out[0], out[1]), (out[2], out[3])
 
     def split(self, num_disparities):
-        """Whether long scanlines run as two halves at once for this D."""
+        """Split level for this D: 0 none, 1 the longest passes' scanlines run
+        as two halves at once, 2 every long pass's."""
         out = (_u32 * 5)()
         check(lib.vxh_sgm_get_schedule(self._h, num_disparities, out))
-        return bool(out[4])
+        return int(out[4])
 
     def set_split(self, num_disparities, split):
-        check(lib.vxh_sgm_set_split(self._h, num_disparities, int(bool(split))))
+        """split: False/0, True/1 (the longest passes) or 2 (every long pass)."""
+        check(lib.vxh_sgm_set_split(self._h, num_disparities, int(split)))
 
     def set_volume_format(self, num_disparities, fmt):
         """Per-direction volume storage for this D: "auto", "u8" or "u16"."""
@@ -524,7 +526,7 @@ class SGMBatch:
         for i, desc in enumerate(pairs):
             l, r, o, w, h, cfg, sched = desc[:7]
             fmt = {"auto": 0, "u8": 1, "u16": 2}[desc[7] if len(desc) > 7 else "auto"]
-            split = int(bool(desc[8])) if len(desc) > 8 else 0
+            split = int(desc[8]) if len(desc) > 8 else 0
             if not sched:
                 hs = os = (0, 0)
             elif isinstance(sched[0], (tuple, list)):

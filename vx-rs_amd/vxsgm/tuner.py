@@ -83,6 +83,11 @@ def _name(sched):
     return f"{a}x{b}/{c}x{d}"
 
 
+def _split_level(name):
+    """Split level of a schedule name: "/split" 1, "/splitall" 2, none 0."""
+    return 2 if name.endswith("/splitall") else (1 if name.endswith("/split") else 0)
+
+
 def _parse(name):
     hs, os_ = name.split("/")[:2]
     return tuple(int(v) for v in hs.split("x")), tuple(int(v) for v in os_.split("x"))
@@ -143,7 +148,7 @@ class Tuner:
             name = self.table[key]["schedule"]
             sched = _parse(name)
             sgm.set_schedule(D, sched)
-            sgm.set_split(D, name.endswith("/split"))
+            sgm.set_split(D, _split_level(name))
             return sched
         left, right, cfg = vxsgm.mixture_pair(w, h, D, 0x5452)
         cfg.num_paths = paths
@@ -153,16 +158,16 @@ class Tuner:
         ws = hip.DeviceBuffer(wsb)
         stream = hip.Stream()
         times = {}
-        for split in (True, False):
+        for split in (2, 1, 0):
             sgm.set_split(D, split)
             for sched in schedules(D):
                 sgm.set_schedule(D, sched)
-                times[_name(sched) + ("/split" if split else "")] = self._time(
+                times[_name(sched) + ("/split" if split == 1 else "/splitall" if split else "")] = self._time(
                     lambda: sgm.enqueue(dl.ptr, dr.ptr, w, h, cfg, out.ptr, ws.ptr, wsb,
                                         stream.handle), stream, reps)
         best = min(times, key=times.get)
         sched = _parse(best)
-        sgm.set_split(D, best.endswith("/split"))
+        sgm.set_split(D, _split_level(best))
         sgm.set_schedule(D, sched)
         self.table[key] = {"schedule": best, "ms": times}
         return sched

@@ -22,8 +22,7 @@ HIP_SOURCES = ["sgm_paths_c0.hip", "sgm_paths_c1.hip", "sgm_paths_c2.hip", "sgm_
                "rectify.hip", "datasets.cpp", "stereo_match.hip"]
 CXX_SOURCES = ["synth.cpp"]
 
-COMMON = ["-O3", "-fPIC", "-std=c++17", "-I", os.path.join(ROOT, "include"),
-          "-I", CSRC, "-Wall"]
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-Wall"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-ffp-contract=off",
              "-fhip-fp32-correctly-rounded-divide-sqrt",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -35,21 +34,21 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose=True, jobs=None, out=OUT, defines=(), objdir=OBJ):
-    """Compiles every source and links `out`; `defines` (NAME=VALUE strings)
-    select compile-time variants of the kernels, built in their own
-    `objdir`."""
+def build(verbose=True, jobs=None, out=OUT, defines=(), objdir=OBJ, csrc=CSRC):
+    """Compiles every source (from `csrc`) and links `out`; `defines`
+    (NAME=VALUE strings) select compile-time variants of the kernels, built in
+    their own `objdir`."""
     os.makedirs(objdir, exist_ok=True)
     cmds, objs = [], []
     dflags = ["-D" + d for d in defines]
     for src in HIP_SOURCES:
         o = os.path.join(objdir, src + ".o")
-        cmds.append([HIPCC, "-x", "hip", *HIP_FLAGS, *COMMON, *dflags, "-c",
-                     os.path.join(CSRC, src), "-o", o])
+        cmds.append([HIPCC, "-x", "hip", *HIP_FLAGS, *COMMON, "-I", csrc, *dflags, "-c",
+                     os.path.join(csrc, src), "-o", o])
         objs.append(o)
     for src in CXX_SOURCES:
         o = os.path.join(objdir, src + ".o")
-        cmds.append(["g++", *COMMON, "-c", os.path.join(CSRC, src), "-o", o])
+        cmds.append(["g++", *COMMON, "-I", csrc, "-c", os.path.join(csrc, src), "-o", o])
         objs.append(o)
     # Independent translation units compile side by side (at most 8 at once:
     # each hipcc instance holds a few GB while optimising the kernels).

@@ -68,7 +68,18 @@ def parse():
                     help="p99 latency target of the served leg's capacity figure")
     ap.add_argument("--serve-streams", type=int, default=4,
                     help="HIP streams of the serving loop")
-    ap.add_argument("--requests", type=int, default=300)
+    ap.add_argument("--work-budget-g", type=float, default=0.3,
+                    help="the serving loop's work budget: admitted W*H*D while any request "
+                         "runs, in G pixel-disparities (0: no cap)")
+    ap.add_argument("--busy-work-budget-g", type=float, default=1.0,
+                    help="the serving loop's work budget while --backlog requests wait")
+    ap.add_argument("--backlog", type=int, default=4,
+                    help="queued requests that open the busy work budget")
+    ap.add_argument("--requests", type=int, default=300,
+                    help="least requests per rate of the served leg")
+    ap.add_argument("--serve-seconds", type=float, default=2.0,
+                    help="arrival span per rate of the served leg: each rate offers "
+                         "max(--requests, rate * this) requests")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB five times (the "
@@ -264,20 +275,31 @@ def served_leg(vx, np, args):
     """Poisson arrivals of the mixture's 32-pair request pool against the
     serving loop and against a FIFO queue into the plain per-pair path (one
     request at a time), over a sweep of rates; both keep their device slots
-    reserved before the timed runs.  Every map is checked against the oracle's
-    hash for its pair.  `capacity_at_slo`: for each form, the highest swept
-    rate whose p99 stays within the SLO, and the Gpx*disp/s it sustained."""
+    reserved before the timed runs.  At every rate the two forms take turns on
+    the same arrival sequence of max(--requests, rate * --serve-seconds)
+    requests, each timed run after an untimed warm-up burst at its rate.  Each
+    form's first map of every pair is checked against the
+    oracle's hash for the pair, and every timed map bit for bit against those.
+    `capacity_at_slo`: for each form, the highest swept rate whose p99 stays
+    within the SLO, and the Gpx*disp/s it sustained."""
     pool = []
     for w, h, d, seed in vx.mixture(32, seed=0x5347_0004):
         left, right, cfg = vx.mixture_pair(w, h, d, seed)
         pool.append((left, right, w, h, cfg))
     want = golden_hashes("serve_pool_32")
-    out = {"requests_per_rate": args.requests, "budget_gb": 64, "slo_p99_ms": args.slo_ms,
+    out = {"requests_per_rate": f"max({args.requests}, rate * {args.serve_seconds:g} s)",
+           "budget_gb": 64, "work_budget_gpxd": args.work_budget_g,
+           "busy_work_budget_gpxd": args.busy_work_budget_g, "backlog_requests": args.backlog,
+           "slo_p99_ms": args.slo_ms,
            "bit_exact_vs_oracle": want is not None}
     shapes = sorted({(w, h, cfg.num_disparities) for _, _, w, h, cfg in pool},
                     key=lambda s: s[0] * s[1] * s[2])
-    forms = (("served", dict(max_inflight=32, num_streams=args.serve_streams), 2),
+    forms = (("served", dict(max_inflight=32, num_streams=args.serve_streams,
+                             work_budget_pxd=int(args.work_budget_g * 1e9),
+                             busy_work_budget_pxd=int(args.busy_work_budget_g * 1e9),
+                             backlog_requests=args.backlog), 2),
              ("plain_fifo", dict(max_inflight=1, num_streams=1), 0))
+    servers, ref_maps = {}, {}
     for name, kw, per_shape in forms:
         srv = vx.SGMServer(hbm_budget_bytes=64 << 30, **kw)
         # Resident slots: `per_shape` per shape of the pool plus the largest.
@@ -291,37 +313,61 @@ def served_leg(vx, np, args):
             srv.wait(srv.submit(l, r, w, h, cfg, o))
             if want is not None and map_sha256(o) != want[k]:
                 out["bit_exact_vs_oracle"] = False
-        rates = {}
-        for rate in [float(x) for x in args.rates.split(",")]:
-            rng = np.random.default_rng(int(rate))
-            gaps = rng.exponential(1.0 / rate, args.requests)
-            srv.reset_stats()
-            outs = [None] * args.requests
+            ref_maps.setdefault(k, o.view(np.uint32))
+        servers[name] = srv
+        out[name] = {}
+    # The two forms take turns at every rate, on the same arrival sequence.
+    for rate in [float(x) for x in args.rates.split(",")]:
+        n = max(args.requests, int(rate * args.serve_seconds))
+        gaps = np.random.default_rng(int(rate)).exponential(1.0 / rate, n)
+        for name, _, _ in forms:
+            srv = servers[name]
+            # An untimed warm-up burst at the same rate first, so that each
+            # form starts its timed run in the same (busy) state.
+            warm = np.random.default_rng(int(rate) + 1).exponential(1.0 / rate, max(50, n // 10))
             t = time.perf_counter()
-            for i in range(args.requests):
+            for i, g in enumerate(warm):
+                t += g
+                while time.perf_counter() < t:
+                    pass
+                l, r, w, h, cfg = pool[i % len(pool)]
+                srv.submit(l, r, w, h, cfg, np.empty(w * h, np.float32))
+            srv.drain()
+            srv.wait_all()
+            # Result buffers are allocated and touched before the timed run
+            # (a client's are resident), so the copies into them measure the
+            # server, not first-touch page faults.
+            outs = [np.zeros(pool[i % len(pool)][2] * pool[i % len(pool)][3], np.float32)
+                    for i in range(n)]
+            srv.reset_stats()
+            t = time.perf_counter()
+            for i in range(n):
                 t += gaps[i]
                 while time.perf_counter() < t:
                     pass
                 l, r, w, h, cfg = pool[i % len(pool)]
-                outs[i] = np.empty(w * h, np.float32)
                 srv.submit(l, r, w, h, cfg, outs[i])
             srv.drain()
+            srv.wait_all()
             st = srv.stats()
             span = st["last_complete_s"] - st["first_submit_s"]
             if want is not None:
-                exact = all(map_sha256(o) == want[i % len(pool)] for i, o in enumerate(outs))
+                exact = all(np.array_equal(o.view(np.uint32), ref_maps[i % len(pool)])
+                            for i, o in enumerate(outs))
                 out["bit_exact_vs_oracle"] = out["bit_exact_vs_oracle"] and exact
-            rates[str(int(rate))] = {
+            rec = {
+                "requests": n,
                 "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
                 "gpxd_per_s": round(st["pxd_total"] / span / 1e9, 3),
                 "offered_gpxd_per_s": round(rate * st["pxd_total"] / st["completed"] / 1e9, 3),
                 "peak_inflight": st["peak_inflight"],
                 "peak_admitted_gb": round(st["peak_admitted_bytes"] / 2**30, 3),
                 "slot_misses": st["slot_misses"]}
-            log(f"served[{name}] {rate:.0f} req/s: {rates[str(int(rate))]}")
-        srv.close()
-        out[name] = rates
-        ok = [(float(r), v) for r, v in rates.items() if v["p99_ms"] <= args.slo_ms]
+            out[name][str(int(rate))] = rec
+            log(f"served[{name}] {rate:.0f} req/s: {rec}")
+    for name, _, _ in forms:
+        servers[name].close()
+        ok = [(float(r), v) for r, v in out[name].items() if v["p99_ms"] <= args.slo_ms]
         best = max(ok, key=lambda kv: kv[1]["gpxd_per_s"]) if ok else None
         out.setdefault("capacity_at_slo", {})[name] = (
             {"rate": int(best[0]), "gpxd_per_s": best[1]["gpxd_per_s"],

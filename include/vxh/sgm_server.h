@@ -26,6 +26,21 @@ typedef struct {
     uint64_t hbm_budget_bytes;  /* sum of admitted requests' volume bytes   */
     uint32_t max_inflight;      /* admitted requests at once (>= 1)         */
     uint32_t num_streams;       /* HIP streams the admitted work spreads on */
+    uint64_t work_budget_pxd;   /* sum of admitted requests' W*H*D while any
+                                   runs (a larger request runs alone); 0: no
+                                   cap.  Bounds how many requests share the
+                                   GPU: a large request shares it with little,
+                                   which keeps the tail latency down. */
+    uint64_t busy_work_budget_pxd;  /* the cap instead while at least
+                                       backlog_requests wait (0: unchanged):
+                                       more overlap, more throughput, once a
+                                       backlog shows the GPU is the limit */
+    uint32_t backlog_requests;      /* 0: no busy state.  Otherwise requests
+                                       admitted without a backlog queue on the
+                                       first stream (pipelined, never sharing
+                                       the GPU) and only those admitted with
+                                       one spread over every stream. */
+    uint32_t reserved;
 } vxh_server_config;
 
 typedef struct {
@@ -40,6 +55,7 @@ typedef struct {
     double mean_queue_ms;            /* submit -> admission                 */
     double pxd_total;                /* sum of W*H*D over completed         */
     double first_submit_s, last_complete_s;  /* steady-clock seconds       */
+    double peak_inflight_pxd;        /* max of the admitted W*H*D           */
 } vxh_server_stats;
 
 typedef struct vxh_server vxh_server;

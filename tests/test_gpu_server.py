@@ -79,6 +79,41 @@ def test_budget_bounds_admitted_bytes():
     srv.close()
 
 
+def test_work_budgets_bound_sharing():
+    """The work cap bounds the pixel-disparities admitted at once, a request
+    above it runs alone, and the busy cap applies only while a backlog waits;
+    every map stays the one the pair gets alone."""
+    l, r, cfg = vxsgm.mixture_pair(320, 180, 64, 5)
+    h, w = l.shape
+    P = w * h * cfg.num_disparities
+    ref = np.empty(w * h, np.float32)
+    solo = vxsgm.SGMServer(hbm_budget_bytes=1 << 30, max_inflight=1, num_streams=1)
+    solo.wait(solo.submit(l, r, w, h, cfg, ref))
+    solo.close()
+
+    def burst(n, **kw):
+        srv = vxsgm.SGMServer(hbm_budget_bytes=4 << 30, max_inflight=16, num_streams=4, **kw)
+        outs = [np.empty(w * h, np.float32) for _ in range(n)]
+        for o in outs:
+            srv.submit(l, r, w, h, cfg, o)
+        srv.drain()
+        st = srv.stats()
+        srv.close()
+        for o in outs:
+            assert (bits(o) == bits(ref)).all()
+        return st
+
+    st = burst(24, work_budget_pxd=int(2.5 * P))
+    assert st["peak_inflight"] <= 2 and st["peak_inflight_pxd"] <= 2.5 * P
+    st = burst(12, work_budget_pxd=P // 2)
+    assert st["peak_inflight"] == 1, "a request above the cap shares the GPU"
+    # 24 queued at once: the backlog of at least 4 opens the busy cap.
+    st = burst(24, work_budget_pxd=int(1.5 * P), busy_work_budget_pxd=int(6.5 * P),
+               backlog_requests=4)
+    assert 2 <= st["peak_inflight"] <= 6 and st["peak_inflight_pxd"] <= 6.5 * P
+    assert st["completed"] == 24
+
+
 def test_concurrent_clients():
     pairs = small_mix(6, seed=23)
     srv = vxsgm.SGMServer(hbm_budget_bytes=1 << 30, max_inflight=6, num_streams=3)

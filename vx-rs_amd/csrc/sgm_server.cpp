@@ -10,8 +10,11 @@
 //                            requests in arrival order while the sum of
 //                            admitted cost-volume bytes (each request's
 //                            pipeline workspace: the per-direction volumes)
-//                            stays within the HBM budget and fewer than
-//                            max_inflight run.  An admitted request takes a
+//                            stays within the HBM budget, fewer than
+//                            max_inflight run and (with a work budget) the
+//                            admitted pixel-disparities stay within it, so
+//                            small requests overlap while a large one runs
+//                            nearly alone.  An admitted request takes a
 //                            device-resident slot (inputs, output, census +
 //                            volume scratch) from a pool of slots reused by
 //                            capacity, as the reference's TexturePool reuses
@@ -22,9 +25,9 @@
 //                            while others are on the GPU overlap with them;
 //                            a completed
 //                            request's slot goes back to the pool at once;
-//   finisher thread          copies each completed map to the caller's
-//                            buffer, checks the request's error word, and
-//                            returns the staging buffers.
+//   finisher threads         copy each completed map to the caller's
+//                            buffer, check the request's error word, and
+//                            return the staging buffers.
 // Work never changes with the mix: each request runs the same kernels on its
 // own buffers, so its result is the one it would get alone.
 #include <hip/hip_runtime.h>
@@ -98,6 +101,11 @@ size_t out_err_offset(size_t npx) { return round_up(npx * 4, 64); }
 
 }  // namespace
 
+// The work a request adds to the GPU: its pixel-disparities.
+inline uint64_t request_pxd(const Request *r) {
+    return (uint64_t)r->w * r->h * r->cfg.num_disparities;
+}
+
 struct vxh_server {
     int device = 0;
     vxh_server_config cfg{};
@@ -116,6 +124,7 @@ struct vxh_server {
     // Scheduler-thread state.
     std::vector<Request *> inflight;
     uint64_t admitted_bytes = 0;
+    uint64_t inflight_pxd = 0;   // W*H*D of the admitted requests
     std::vector<hipEvent_t> events;
 
     // Completed requests waiting for the finisher (guarded by mu).
@@ -132,7 +141,8 @@ struct vxh_server {
     std::mutex pin_mu;
     std::vector<Staging> pin_idle;
 
-    std::thread th, fin;
+    std::thread th;
+    std::vector<std::thread> fins;   // finisher threads
 
     // Statistics, guarded by mu.
     vxh_server_stats st{};
@@ -253,10 +263,17 @@ hipEvent_t take_event(vxh_server *S) {
 // error word and the map out, then the completion event.  Returns false when
 // the device work could not be enqueued; r->stream then names the stream that
 // may hold part of it.
-bool launch(vxh_server *S, Request *r) {
+bool launch(vxh_server *S, Request *r, bool busy) {
     Slot *s = r->slot;
     const size_t npx = (size_t)r->w * r->h;
-    hipStream_t st = S->streams[S->rr++ % S->streams.size()];
+    // Without a backlog the requests queue on the first stream: each one's
+    // copies and kernels are enqueued behind the running one (no idle gap
+    // between them) but never share the GPU with it, which keeps a large
+    // request's latency at its own.  With a backlog they spread over every
+    // stream and overlap.
+    hipStream_t st = busy || S->cfg.backlog_requests == 0
+                         ? S->streams[S->rr++ % S->streams.size()]
+                         : S->streams[0];
     r->ev = take_event(S);
     if (!r->ev) return false;
     r->stream = st;
@@ -281,6 +298,7 @@ void complete(vxh_server *S, Request *r, vxh_status status) {
     if (r->ev) S->events.push_back(r->ev);
     r->ev = nullptr;
     S->admitted_bytes -= r->vol_bytes;
+    S->inflight_pxd -= request_pxd(r);
     if (r->slot) release_slot(S, r->slot, now_s());
     r->slot = nullptr;
     std::lock_guard<std::mutex> g(S->mu);
@@ -289,6 +307,12 @@ void complete(vxh_server *S, Request *r, vxh_status status) {
 }
 
 // Copies results out, checks error words, recycles buffers, records stats.
+// Finisher threads: a map's copy into the caller's buffer (pageable memory,
+// up to 33 MB for 4K) takes milliseconds when its pages are new, so requests
+// that complete together are copied side by side instead of one after the
+// other.
+constexpr int kFinishers = 4;
+
 void finisher(vxh_server *S) {
     (void)hipSetDevice(S->device);
     while (true) {
@@ -318,7 +342,7 @@ void finisher(vxh_server *S) {
         S->lat_ms.push_back(lat);
         S->queue_ms.push_back((r->t_admit - r->t_submit) * 1e3);
         S->st.pxd_total += (double)r->w * r->h * r->cfg.num_disparities;
-        S->st.last_complete_s = r->t_done;
+        S->st.last_complete_s = std::max(S->st.last_complete_s, r->t_done);
         S->cv_done.notify_all();
     }
 }
@@ -342,12 +366,24 @@ void scheduler(vxh_server *S) {
         // Admissions, in arrival order.
         while (true) {
             Request *r = nullptr;
+            bool busy = false;
             {
                 std::unique_lock<std::mutex> g(S->mu);
                 if (S->pending.empty()) break;
                 if (S->inflight.size() >= S->cfg.max_inflight) break;
                 r = S->pending.front();
                 if (S->admitted_bytes + r->vol_bytes > S->cfg.hbm_budget_bytes) break;
+                // The work cap holds while anything runs (a request larger
+                // than the cap runs alone); with a backlog of at least
+                // backlog_requests waiting, the wider busy cap.
+                busy = S->cfg.backlog_requests != 0 &&
+                       S->pending.size() >= S->cfg.backlog_requests;
+                if (!S->inflight.empty()) {
+                    const uint64_t cap =
+                        busy && S->cfg.busy_work_budget_pxd ? S->cfg.busy_work_budget_pxd
+                                                            : S->cfg.work_budget_pxd;
+                    if (cap != 0 && S->inflight_pxd + request_pxd(r) > cap) break;
+                }
                 S->pending.pop_front();
             }
             bool hit = false;
@@ -357,6 +393,7 @@ void scheduler(vxh_server *S) {
                     // Nothing will free memory: fail this request.
                     r->t_admit = now_s();
                     S->admitted_bytes += r->vol_bytes;
+                    S->inflight_pxd += request_pxd(r);
                     complete(S, r, VXH_ERR_BUFFER_ALLOC);
                     progressed = true;
                     continue;
@@ -368,13 +405,14 @@ void scheduler(vxh_server *S) {
             }
             r->t_admit = now_s();
             S->admitted_bytes += r->vol_bytes;
+            S->inflight_pxd += request_pxd(r);
             {
                 std::lock_guard<std::mutex> g(S->mu);
                 if (hit) S->st.slot_hits++;
                 else S->st.slot_misses++;
                 S->st.peak_admitted_bytes = std::max(S->st.peak_admitted_bytes, S->admitted_bytes);
             }
-            if (!launch(S, r)) {
+            if (!launch(S, r, busy)) {
                 // Part of the request may be on its stream already: the slot
                 // goes back only after that work has drained.
                 if (r->stream) (void)hipStreamSynchronize(r->stream);
@@ -383,6 +421,8 @@ void scheduler(vxh_server *S) {
                 S->inflight.push_back(r);
                 std::lock_guard<std::mutex> g(S->mu);
                 S->st.peak_inflight = std::max<uint64_t>(S->st.peak_inflight, S->inflight.size());
+                S->st.peak_inflight_pxd =
+                    std::max(S->st.peak_inflight_pxd, (double)S->inflight_pxd);
             }
             progressed = true;
         }
@@ -431,7 +471,7 @@ vxh_status vxh_server_create(int device, const vxh_server_config *cfg, vxh_serve
         S->streams.push_back(s);
     }
     S->th = std::thread(scheduler, S);
-    S->fin = std::thread(finisher, S);
+    for (int i = 0; i < kFinishers; ++i) S->fins.emplace_back(finisher, S);
     *out = S;
     return VXH_OK;
 }
@@ -586,7 +626,8 @@ void vxh_server_destroy(vxh_server *S) {
         S->stop_finisher = true;
     }
     S->cv_finish.notify_all();
-    if (S->fin.joinable()) S->fin.join();
+    for (std::thread &t : S->fins)
+        if (t.joinable()) t.join();
     (void)hipSetDevice(S->device);
     for (hipStream_t s : S->streams) {
         (void)hipStreamSynchronize(s);

@@ -584,7 +584,9 @@ class SGMBatch:
 
 class vxh_server_config(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", _u32),
-                ("num_streams", _u32)]
+                ("num_streams", _u32), ("work_budget_pxd", ctypes.c_uint64),
+                ("busy_work_budget_pxd", ctypes.c_uint64), ("backlog_requests", _u32),
+                ("reserved", _u32)]
 
 
 class vxh_server_stats(ctypes.Structure):
@@ -594,7 +596,8 @@ class vxh_server_stats(ctypes.Structure):
                 ("slot_misses", ctypes.c_uint64), ("p50_ms", ctypes.c_double),
                 ("p99_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
                 ("mean_queue_ms", ctypes.c_double), ("pxd_total", ctypes.c_double),
-                ("first_submit_s", ctypes.c_double), ("last_complete_s", ctypes.c_double)]
+                ("first_submit_s", ctypes.c_double), ("last_complete_s", ctypes.c_double),
+                ("peak_inflight_pxd", ctypes.c_double)]
 
 
 _sig("vxh_server_request_bytes", ctypes.c_uint64, _u32, _u32, _cfgp)
@@ -625,8 +628,10 @@ def request_bytes(width, height, config):
 class SGMServer:
     """Serving loop with HBM-budget admission (include/vxh/sgm_server.h)."""
 
-    def __init__(self, hbm_budget_bytes=64 << 30, max_inflight=16, num_streams=8, device=0):
-        cfg = vxh_server_config(hbm_budget_bytes, max_inflight, num_streams)
+    def __init__(self, hbm_budget_bytes=64 << 30, max_inflight=16, num_streams=8, device=0,
+                 work_budget_pxd=0, busy_work_budget_pxd=0, backlog_requests=0):
+        cfg = vxh_server_config(hbm_budget_bytes, max_inflight, num_streams, work_budget_pxd,
+                                busy_work_budget_pxd, backlog_requests, 0)
         h = _vp()
         _check_server(lib.vxh_server_create(device, ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
@@ -660,6 +665,18 @@ class SGMServer:
 
     def drain(self):
         _check_server(lib.vxh_server_drain(self._h))
+
+    def wait_all(self):
+        """Waits for (and so releases) every request submitted and not yet
+        waited for; raises the first failure after releasing all."""
+        err = None
+        for rid in list(self._keep):
+            try:
+                self.wait(rid)
+            except Error as e:
+                err = err or e
+        if err is not None:
+            raise err
 
     def stats(self):
         s = vxh_server_stats()

@@ -56,12 +56,22 @@ typedef struct vxh_sgm_batch vxh_sgm_batch;
  *                  1 = streams: the per-pair pipelines dealt to `streams`
  *                  streams, largest pairs first, pairs on one stream in turn
  *                  sharing one workspace.
+ *                  2 = sweeps: chunks as the persistent form; after the
+ *                  census and cost launches, one forward sweep per
+ *                  lane-layout class aggregates the four directions
+ *                  (+1,0) (0,+1) (+1,+1) (-1,+1) together in raster order
+ *                  and stores their sum once (u16), and one backward sweep
+ *                  runs the other four over the image turned by 180 degrees,
+ *                  adds that sum and takes each pixel's winner, so no
+ *                  per-direction volume and no summed volume is stored.
+ *                  Every pair must have num_paths 8; lane layouts, volume
+ *                  format and split settings of the pairs do not apply.
  *   streams        streams form: 1..16 (default 4).
  *   blocks_per_cu  persistent form: blocks of 256 threads of the aggregation
  *                  per CU (default: as many as fit).
- *   budget_bytes   persistent form: workspace bytes of one chunk (default
- *                  64 GiB; a pair larger than the budget forms its own
- *                  chunk). */
+ *   budget_bytes   persistent and sweep forms: workspace bytes of one chunk
+ *                  (default 64 GiB; a pair larger than the budget forms its
+ *                  own chunk). */
 typedef struct {
     uint32_t streams;
     uint32_t form;
@@ -79,8 +89,8 @@ typedef struct {
     uint32_t per_lane_h;
     uint32_t volume_format;   /* 1 = u8, 2 = u16 */
     uint64_t workspace_bytes; /* scratch the pair needs */
-    uint32_t chunk;           /* persistent form: chunk the pair runs in */
-    uint32_t form;            /* 0 persistent, 1 streams */
+    uint32_t chunk;           /* persistent and sweep forms: chunk the pair runs in */
+    uint32_t form;            /* 0 persistent, 1 streams, 2 sweeps */
 } vxh_sgm_batch_stats;
 
 /* Validates every pair, allocates the workspaces (persistent form: one shared
@@ -97,14 +107,15 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs,
 vxh_status vxh_sgm_batch_launch(vxh_sgm_batch *batch, void *stream);
 /* Launch on the batch's own stream, wait, and check the error flag. */
 vxh_status vxh_sgm_batch_run(vxh_sgm_batch *batch);
-/* After a launch has completed: VXH_ERR_GPU if any pair's bounded wait of a
- * split scanline expired (its map is then incomplete), else VXH_OK. */
+/* After a launch has completed: VXH_ERR_GPU if any bounded wait expired (a
+ * split scanline's head for its tail, or a sweep wave for the wave above it;
+ * that pair's map is then incomplete), else VXH_OK. */
 vxh_status vxh_sgm_batch_check(vxh_sgm_batch *batch);
 vxh_status vxh_sgm_batch_stats_get(const vxh_sgm_batch *batch, uint32_t pair,
                                    vxh_sgm_batch_stats *out);
 /* Kernel launches in the batch's graph. */
 uint32_t vxh_sgm_batch_items(const vxh_sgm_batch *batch);
-/* Chunks of the persistent form (1 for the streams form). */
+/* Chunks of the persistent and sweep forms (1 for the streams form). */
 uint32_t vxh_sgm_batch_chunks(const vxh_sgm_batch *batch);
 void vxh_sgm_batch_destroy(vxh_sgm_batch *batch);
 /* Message of the last batch failure on this thread. */

@@ -1,7 +1,8 @@
 """GPU parity of the batched form: every pair of a graph-captured batch (the
-per-pair pipelines over several streams) equals the plain per-pair path and
-the oracle bit for bit, for mixed sizes, disparity ranges, path counts,
-schedules, split settings, stream counts and repeated graph replays."""
+persistent form, the per-pair pipelines over several streams, and the
+two-sweep form) equals the plain per-pair path and the oracle bit for bit,
+for mixed sizes, disparity ranges, path counts, schedules, split settings,
+stream counts and repeated graph replays."""
 import hashlib
 import json
 import os
@@ -179,14 +180,57 @@ def test_batch_schedules_and_splits_per_pair():
             assert (bits(got) == bits(want)).all()
 
 
+@pytest.mark.parametrize("options", [{"form": "sweep"},
+                                     {"form": "sweep", "budget_bytes": 4 << 20}])
+def test_sweep_form_small_mix_matches_oracle(options):
+    """The two-sweep form (forward sums stored once, backward sweep with the
+    winner): odd and tiny heights, widths below and above a wave's lanes,
+    disparity ranges not a multiple of 4, every lane-layout class, P2 that
+    wraps u16 sums, LR checks, repeated replays, one chunk and many chunks."""
+    pairs = []
+    for i, (w, h, d, p2, lr) in enumerate([(16, 8, 8, 120.0, False), (33, 3, 8, 120.0, True),
+                                           (100, 31, 48, 120.0, False),
+                                           (129, 17, 100, 120.0, True),
+                                           (300, 20, 256, 120.0, False),
+                                           (90, 30, 32, 900.0, True), (50, 9, 7, 120.0, False),
+                                           (140, 23, 99, 300.0, False),
+                                           (260, 97, 250, 120.0, True),
+                                           (200, 61, 128, 120.0, False), (97, 1, 50, 120.0, False),
+                                           (64, 2, 64, 65535.0, False)]):
+        l, r = random_pair(w, h, seed=500 + i)
+        pairs.append((l, r, vxsgm.SGMStereoConfig(num_disparities=d, num_paths=8, p2=p2,
+                                                  lr_check=lr)))
+    outs = run_batch(pairs, replays=2, options=options)
+    bad = []
+    for k, rep in enumerate(outs):
+        for i, ((l, r, cfg), got) in enumerate(zip(pairs, rep)):
+            want = O.sgm(l, r, cfg.num_disparities, p2=cfg.p2, num_paths=8,
+                         lr_check=cfg.lr_check, lr_tol=cfg.lr_tolerance).ravel()
+            diff = bits(got) != bits(want)
+            if diff.any():
+                bad.append(f"replay {k} pair {i} ({l.shape[1]}x{l.shape[0]} "
+                           f"D={cfg.num_disparities}): {int(diff.sum())} px")
+    assert not bad, "\n".join(bad)
+
+
+def test_sweep_form_rejects_other_path_counts():
+    l = hip.DeviceBuffer(64 * 8)
+    o = hip.DeviceBuffer(64 * 8 * 4)
+    for paths in (2, 4):
+        cfg = vxsgm.SGMStereoConfig(num_disparities=32, num_paths=paths)
+        with pytest.raises(vxsgm.InvalidConfig):
+            vxsgm.SGMBatch([(l.ptr, l.ptr, o.ptr, 64, 8, cfg, None)], options={"form": "sweep"})
+
+
 @pytest.mark.parametrize("options", [None, {"form": "persistent", "budget_bytes": 16 << 30},
-                                     {"form": "streams"}])
+                                     {"form": "streams"}, {"form": "sweep"}])
 def test_batch_full_mixture_matches_oracle_hashes(options):
     """The batched workload at full size: all 64 pairs of the mixture
     ({320x240, 640x480, 1280x720, 1920x1080} x D {64, 128, 256}, 8 paths) in
     one batch, replayed twice; every map's SHA-256 equals the oracle's
     (tests/golden/mixture_hashes.json).  Persistent form with the default
-    budget and with a 16 GiB budget (several chunks), and the streams form."""
+    budget and with a 16 GiB budget (several chunks), the streams form and the
+    two-sweep form."""
     rows = golden_mixture("batch_64")
     pairs = [vxsgm.mixture_pair(w, h, d, seed) for w, h, d, seed, _ in rows]
     outs = run_batch(pairs, replays=2, options=options)

@@ -17,7 +17,8 @@ OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "vxsgm", "libvxsgm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["sgm_paths_c0.hip", "sgm_paths_c1.hip", "sgm_paths_c2.hip", "sgm_kernels.hip",
+HIP_SOURCES = ["sgm_paths_c0.hip", "sgm_paths_c1.hip", "sgm_paths_c2.hip", "sgm_sweep_c0.hip",
+               "sgm_sweep_c1.hip", "sgm_sweep_c2.hip", "sgm_kernels.hip",
                "sgm_runtime.hip", "sgm_batch.hip", "sgm_server.cpp", "depth.hip", "pipeline.hip",
                "rectify.hip", "datasets.cpp", "stereo_match.hip"]
 CXX_SOURCES = ["synth.cpp"]

@@ -53,7 +53,31 @@ vxh_status bfail(vxh_status st, const std::string &m) {
 constexpr uint32_t kDefaultStreams = 4;
 constexpr uint32_t kMaxStreams = 16;
 constexpr uint64_t kDefaultBudget = (uint64_t)64 << 30;
-constexpr uint32_t kFormPersistent = 0, kFormStreams = 1;
+constexpr uint32_t kFormPersistent = 0, kFormStreams = 1, kFormSweep = 2;
+
+// Scratch of one pair of the sweep form, byte offsets from its base: the
+// census of both images, the cost volume (u8, pix*Dp + d), the forward sums
+// (u16, same layout), the waves' rings, and for the LR check the summed
+// volume and the winners.
+struct SweepLayout {
+    size_t lc, rc, cost, sf, ring, sum, best, total;
+};
+SweepLayout sweep_layout(uint32_t w, uint32_t h, uint32_t D, bool lr) {
+    using namespace vxh;
+    SweepLayout l;
+    const size_t n = (size_t)w * h;
+    const size_t Dp = (size_t)volume_stride((int)D);
+    l.lc = 0;
+    l.rc = l.lc + ws_align(n * 8);
+    l.cost = l.rc + ws_align(n * 8);
+    l.sf = l.cost + ws_align(n * Dp);
+    l.ring = l.sf + ws_align(n * Dp * 2);
+    l.sum = l.ring + ws_align((size_t)sweep_waves((int)h, (int)D) * sweep_ring_slots((int)w) *
+                              sweep_slot_bytes((int)D));
+    l.best = l.sum + (lr ? ws_align(n * Dp * 2) : 0);
+    l.total = l.best + (lr ? ws_align(n * 2) : 0);
+    return l;
+}
 
 // Index of (L, K) among the layouts of class c, -1 if none.
 int layout_index(int c, uint32_t L, uint32_t K) {
@@ -92,7 +116,26 @@ struct Chunk {
         int blocks;
     };
     std::vector<Group> groups;
+    // Sweep form: per lane-layout class present, the forward and backward
+    // sweep launches.
+    struct Sweep {
+        int cls;
+        vxh::SweepArgs fwd, bwd;
+        int blocks_fwd, blocks_bwd;
+    };
+    std::vector<Sweep> sweeps;
 };
+
+int sweep_occupancy(int cls, bool bwd) {
+    return cls == 0 ? vxh::sweep_occupancy_c0(bwd)
+                    : cls == 1 ? vxh::sweep_occupancy_c1(bwd) : vxh::sweep_occupancy_c2(bwd);
+}
+
+hipError_t launch_sweep(int cls, bool bwd, const vxh::SweepArgs &a, int blocks, hipStream_t s) {
+    return cls == 0 ? vxh::launch_sweep_c0(bwd, a, blocks, s)
+                    : cls == 1 ? vxh::launch_sweep_c1(bwd, a, blocks, s)
+                               : vxh::launch_sweep_c2(bwd, a, blocks, s);
+}
 
 int class_occupancy(int cls, int mode) {
     return cls == 0 ? vxh::batch_paths_occupancy_c0(mode)
@@ -123,9 +166,9 @@ struct vxh_sgm_batch {
     std::vector<vxh_sgm_pair_desc> pairs;
     std::vector<PairPlan> plan;
     std::vector<void *> ws;              // streams: one per stream; persistent: one shared
-    void *tables = nullptr;              // persistent: job tables, passes, items, counters
-    uint32_t *counters = nullptr;        // persistent: ticket counters (in `tables`)
-    size_t ncounters = 0;
+    void *tables = nullptr;              // persistent / sweeps: job tables, items, counters
+    uint32_t *counters = nullptr;        // ticket counters (sweeps: and progress words)
+    size_t ncounters = 0;                // words from `counters` cleared at every replay
     std::vector<Chunk> chunks;
     uint32_t *d_error = nullptr;         // OR of every pair's error word
     hipStream_t stream = nullptr;        // the batch's own stream (capture origin)
@@ -387,6 +430,241 @@ hipError_t capture_persistent(vxh_sgm_batch *b) {
     return e;
 }
 
+// The sweep form's plan: chunks as the persistent form; per chunk the census
+// and cost job tables, and per lane-layout class the forward and backward
+// job tables and one item list per XCD.  Each class's pairs are dealt to the
+// XCD lists by work (largest first onto the least loaded list), and a list
+// holds its pairs' waves in wave order (wave 0 of every pair, then wave 1,
+// ...), so a wave's predecessor always holds an earlier ticket of the same
+// list and is running before it waits on it.  Counters and progress words
+// follow each other at the end of the tables and are cleared per replay.
+hipError_t plan_sweep(vxh_sgm_batch *b, const std::vector<uint32_t> &order, uint64_t budget) {
+    using namespace vxh;
+    Chunk cur;
+    for (uint32_t i : order) {
+        const size_t wbytes = ws_align(b->plan[i].ws_bytes);
+        if (!cur.pairs.empty() && cur.ws_bytes + wbytes > budget) {
+            b->chunks.push_back(cur);
+            cur = Chunk();
+        }
+        b->plan[i].ws_off = cur.ws_bytes;
+        b->plan[i].stream = (uint32_t)b->chunks.size();
+        cur.pairs.push_back(i);
+        cur.ws_bytes += wbytes;
+    }
+    b->chunks.push_back(cur);
+    size_t ws_total = 256;
+    for (const Chunk &c : b->chunks) ws_total = std::max(ws_total, c.ws_bytes);
+    void *ws = nullptr;
+    hipError_t e = hipMalloc(&ws, ws_total);
+    if (e != hipSuccess) return e;
+    b->ws.push_back(ws);
+
+    int dev = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, dev)) != hipSuccess) return e;
+    const int cus = prop.multiProcessorCount;
+
+    std::vector<uint8_t> img;
+    // A zeroed ring slot (the row above every first wave): 64 lanes.
+    const size_t fresh_off = stage(img, std::vector<uint8_t>(64 * kSweepRec, 0).data(),
+                                   64 * kSweepRec);
+    struct Off {
+        size_t census, cost;
+        std::vector<size_t> fwd, bwd, items;
+    };
+    std::vector<Off> off(b->chunks.size());
+    // Counter / progress words: per chunk and class 16 counters, then per
+    // pair 2 * nwaves progress words (forward, backward).
+    struct Words {
+        size_t counters;
+        std::vector<size_t> prog;   // per job of the class
+    };
+    std::vector<std::vector<Words>> words(b->chunks.size());
+    size_t nwords = 0;
+    for (size_t ci = 0; ci < b->chunks.size(); ++ci) {
+        Chunk &c = b->chunks[ci];
+        std::vector<CensusJob> cj;
+        std::vector<CostJob> kj;
+        uint32_t cb = 0, kb = 0;
+        std::vector<SweepJob> jobs[kLayoutClasses];
+        for (uint32_t i : c.pairs) {
+            const vxh_sgm_pair_desc &q = b->pairs[i];
+            const PairPlan &p = b->plan[i];
+            const int w = (int)q.width, h = (int)q.height, D = (int)q.cfg.num_disparities;
+            const int Dp = volume_stride(D);
+            const bool lr = q.cfg.lr_check != 0;
+            const SweepLayout L = sweep_layout(q.width, q.height, (uint32_t)D, lr);
+            char *base = (char *)ws + p.ws_off;
+            CensusJob l{q.d_left, (uint2 *)(base + L.lc), nullptr, nullptr, 0, w, h,
+                        (int)q.cfg.census_radius_x, (int)q.cfg.census_radius_y, (w + 63) / 64,
+                        cb};
+            cb += (uint32_t)(l.nbx * ((h + 3) / 4));
+            CensusJob r = l;
+            r.img = q.d_right;
+            r.out = (uint2 *)(base + L.rc);
+            r.first_block = cb;
+            cb += (uint32_t)(r.nbx * ((h + 3) / 4));
+            cj.push_back(l);
+            cj.push_back(r);
+            const uint32_t p1 = (uint32_t)q.cfg.p1 & 0xFFFFu, p2 = (uint32_t)q.cfg.p2 & 0xFFFFu;
+            CostJob k{(const uint2 *)(base + L.lc), (const uint2 *)(base + L.rc),
+                      (uint8_t *)(base + L.cost), w, h, D, Dp, cost_bias(p1),
+                      (w + 255) / 256, kb};
+            kb += (uint32_t)(k.nbx * h);
+            kj.push_back(k);
+            SweepJob j{};
+            j.cost = (const uint8_t *)(base + L.cost);
+            j.sf = (uint16_t *)(base + L.sf);
+            j.ring = (uint8_t *)(base + L.ring);
+            j.prog = nullptr;   // set once the tables are placed
+            j.out = q.d_out;
+            j.best = lr ? (int16_t *)(base + L.best) : nullptr;
+            j.sum = lr ? (uint16_t *)(base + L.sum) : nullptr;
+            j.w = w;
+            j.h = h;
+            j.D = D;
+            j.Dp = Dp;
+            j.p1x2 = p1 | (p1 << 16);
+            j.p2x2 = p2 | (p2 << 16);
+            const uint32_t xq = p1 - cost_bias(p1);
+            j.xqx2 = xq | (xq << 16);
+            j.nwaves = sweep_waves(h, D);
+            j.slot_bytes = (uint32_t)sweep_slot_bytes(D);
+            const int cls = layout_class(D);
+            jobs[cls].push_back(j);
+        }
+        c.ncensus = (int)cj.size();
+        c.census_blocks = cb;
+        c.ncost = (int)kj.size();
+        c.cost_blocks = kb;
+        off[ci].census = stage(img, cj.data(), cj.size() * sizeof(CensusJob));
+        off[ci].cost = stage(img, kj.data(), kj.size() * sizeof(CostJob));
+        for (int cls = 0; cls < kLayoutClasses; ++cls) {
+            if (jobs[cls].empty()) continue;
+            // Deal the pairs (largest first: the chunk's order) to the XCDs.
+            const size_t nj = jobs[cls].size();
+            std::vector<int> xcd_of(nj);
+            uint64_t load[8] = {};
+            for (size_t x = 0; x < nj; ++x) {
+                const SweepJob &j = jobs[cls][x];
+                const int k = (int)(std::min_element(load, load + 8) - load);
+                xcd_of[x] = k;
+                load[k] += (uint64_t)j.w * j.h * j.D;
+            }
+            Chunk::Sweep sw{};
+            sw.cls = cls;
+            std::vector<SweepItem> all;
+            int most_waves = 0;
+            for (const SweepJob &j : jobs[cls]) most_waves = std::max(most_waves, j.nwaves);
+            uint32_t most = 0;
+            for (int k = 0; k < 8; ++k) {
+                sw.fwd.item_off[k] = (uint32_t)all.size();
+                for (int v = 0; v < most_waves; ++v)
+                    for (size_t x = 0; x < nj; ++x)
+                        if (xcd_of[x] == k && v < jobs[cls][x].nwaves)
+                            all.push_back(SweepItem{(uint32_t)x, (uint32_t)v});
+                sw.fwd.item_cnt[k] = (uint32_t)all.size() - sw.fwd.item_off[k];
+                most = std::max(most, sw.fwd.item_cnt[k]);
+            }
+            Words wd;
+            wd.counters = nwords;
+            nwords += 16;
+            for (const SweepJob &j : jobs[cls]) {
+                wd.prog.push_back(nwords);
+                nwords += 2 * (size_t)j.nwaves;
+            }
+            words[ci].push_back(wd);
+            off[ci].fwd.push_back(stage(img, jobs[cls].data(), nj * sizeof(SweepJob)));
+            off[ci].bwd.push_back(stage(img, jobs[cls].data(), nj * sizeof(SweepJob)));
+            off[ci].items.push_back(stage(img, all.data(), all.size() * sizeof(SweepItem)));
+            // Persistent grid: as many one-wave blocks as the chip holds, no
+            // more than the largest list needs on every XCD.
+            const int need = 8 * (int)most;
+            sw.blocks_fwd = std::max(8, std::min(need, std::max(1, sweep_occupancy(cls, false)) * cus));
+            sw.blocks_bwd = std::max(8, std::min(need, std::max(1, sweep_occupancy(cls, true)) * cus));
+            for (int k = 0; k < 8; ++k) {
+                sw.bwd.item_off[k] = sw.fwd.item_off[k];
+                sw.bwd.item_cnt[k] = sw.fwd.item_cnt[k];
+            }
+            c.sweeps.push_back(sw);
+        }
+    }
+    const size_t counters_off = ws_align(img.size());
+    img.resize(counters_off + ws_align(std::max<size_t>(nwords, 1) * 4), 0);
+    // The progress pointers are known now: patch the staged job tables.
+    char *tb = nullptr;
+    if ((e = hipMalloc(&b->tables, img.size())) != hipSuccess) return e;
+    tb = (char *)b->tables;
+    b->counters = (uint32_t *)(tb + counters_off);
+    b->ncounters = nwords;
+    for (size_t ci = 0; ci < b->chunks.size(); ++ci) {
+        Chunk &c = b->chunks[ci];
+        c.census = (const CensusJob *)(tb + off[ci].census);
+        c.cost = (const CostJob *)(tb + off[ci].cost);
+        for (size_t x = 0; x < c.sweeps.size(); ++x) {
+            Chunk::Sweep &sw = c.sweeps[x];
+            const Words &wd = words[ci][x];
+            SweepJob *hf = (SweepJob *)(img.data() + off[ci].fwd[x]);
+            SweepJob *hb = (SweepJob *)(img.data() + off[ci].bwd[x]);
+            for (size_t j = 0; j < wd.prog.size(); ++j) {
+                hf[j].prog = b->counters + wd.prog[j];
+                hb[j].prog = b->counters + wd.prog[j] + hf[j].nwaves;
+            }
+            for (SweepArgs *a : {&sw.fwd, &sw.bwd}) {
+                a->jobs = (const SweepJob *)(tb + (a == &sw.fwd ? off[ci].fwd[x] : off[ci].bwd[x]));
+                a->items = (const SweepItem *)(tb + off[ci].items[x]);
+                a->counters = b->counters + wd.counters + (a == &sw.fwd ? 0 : 8);
+                a->error = b->d_error;
+                a->fresh = (const uint8_t *)(tb + fresh_off);
+                a->wait_polls = kSweepWaitPolls;
+                a->zero = 0;
+            }
+        }
+    }
+    return hipMemcpy(b->tables, img.data(), img.size(), hipMemcpyHostToDevice);
+}
+
+// Records the sweep form's launches on b->stream (being captured): the error
+// word, counters and progress words cleared, then per chunk census, costs,
+// each class's forward then backward sweep on its own sub stream (classes
+// side by side), and the LR checks.
+hipError_t capture_sweep(vxh_sgm_batch *b) {
+    using namespace vxh;
+    hipError_t e = launch_zero(b->d_error, 1, b->stream);
+    if (e == hipSuccess && b->ncounters) e = launch_zero(b->counters, b->ncounters, b->stream);
+    b->launches = 2;
+    for (const Chunk &c : b->chunks) {
+        if (e == hipSuccess) e = launch_census_multi(c.census, c.ncensus, c.census_blocks, b->stream);
+        if (e == hipSuccess) e = launch_cost_multi(c.cost, c.ncost, c.cost_blocks, b->stream);
+        b->launches += 2;
+        if (e == hipSuccess) e = hipEventRecord(b->events[0], b->stream);
+        for (size_t x = 0; x < c.sweeps.size() && e == hipSuccess; ++x) {
+            const Chunk::Sweep &sw = c.sweeps[x];
+            hipStream_t s = b->sub[x];
+            e = hipStreamWaitEvent(s, b->events[0], 0);
+            if (e == hipSuccess) e = launch_sweep(sw.cls, false, sw.fwd, sw.blocks_fwd, s);
+            if (e == hipSuccess) e = launch_sweep(sw.cls, true, sw.bwd, sw.blocks_bwd, s);
+            if (e == hipSuccess) e = hipEventRecord(b->events[1 + x], s);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->events[1 + x], 0);
+            b->launches += 2;
+        }
+        for (uint32_t i : c.pairs) {
+            const vxh_sgm_pair_desc &q = b->pairs[i];
+            if (!q.cfg.lr_check || e != hipSuccess) continue;
+            const SweepLayout L = sweep_layout(q.width, q.height, q.cfg.num_disparities, true);
+            char *base = (char *)b->ws[0] + b->plan[i].ws_off;
+            const int D = (int)q.cfg.num_disparities;
+            e = launch_lr_check((const uint16_t *)(base + L.sum), (int)q.width, (int)q.height, D,
+                                volume_stride(D), (int)q.cfg.lr_tolerance,
+                                (const int16_t *)(base + L.best), q.d_out, b->stream);
+            b->launches += 1;
+        }
+    }
+    return e;
+}
+
 // Records the streams form's launches on b->stream (being captured).
 vxh_status capture_streams(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
                            hipError_t *err) {
@@ -436,7 +714,7 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
     uint32_t streams = kDefaultStreams, form = kFormPersistent, blocks_per_cu = 0;
     uint64_t budget = kDefaultBudget;
     if (opt) {
-        if (opt->streams > kMaxStreams || opt->form > kFormStreams)
+        if (opt->streams > kMaxStreams || opt->form > kFormSweep)
             return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: batch options");
         if (opt->streams) streams = opt->streams;
         form = opt->form;
@@ -458,6 +736,10 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
             return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: null pair buffer");
         if (q.volume_format > 2 || q.split > 2)
             return bfail(VXH_ERR_INVALID_CONFIG, "invalid config: pair options");
+        if (form == kFormSweep && q.cfg.num_paths != 8)
+            return bfail(VXH_ERR_INVALID_CONFIG,
+                         "invalid config: the sweep form needs num_paths 8 (pair " +
+                             std::to_string(i) + ")");
         const int D = (int)q.cfg.num_disparities;
         const int cls = layout_class(D);
         Schedule sc = default_schedule(D);
@@ -472,10 +754,13 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
         plan[i].mode = q.volume_format == 1 ? kOutU8
                        : q.volume_format == 2 ? kOutU16
                                               : auto_volume_mode(q.cfg.p2);
-        plan[i].ws_bytes = workspace_layout(q.width, q.height, (uint32_t)D,
-                                            (int)q.cfg.num_paths, plan[i].mode,
-                                            q.cfg.lr_check != 0)
-                               .total;
+        plan[i].ws_bytes = form == kFormSweep
+                               ? sweep_layout(q.width, q.height, (uint32_t)D,
+                                              q.cfg.lr_check != 0).total
+                               : workspace_layout(q.width, q.height, (uint32_t)D,
+                                                  (int)q.cfg.num_paths, plan[i].mode,
+                                                  q.cfg.lr_check != 0)
+                                     .total;
         plan[i].ws_off = 0;
     }
     // Largest pairs first.
@@ -501,7 +786,7 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
     hipError_t e = hipSuccess;
     // Sub streams: the streams form's pair streams, the persistent form's
     // side-by-side aggregation groups (at most the three layout classes times
-    // two volume formats).
+    // two volume formats), the sweep form's classes.
     const uint32_t nsub = form == kFormStreams ? streams : 6;
     if (form == kFormStreams) {
         // One workspace per stream, sized for the largest pair dealt to it.
@@ -517,12 +802,16 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
             if (e == hipSuccess) b->ws.push_back(p);
         }
         if (e != hipSuccess) return gpu_fail("batch workspaces", e);
-    } else {
-        if ((e = plan_persistent(b, order, budget, blocks_per_cu)) != hipSuccess)
-            return gpu_fail("batch plan", e);
     }
+    // The sweep form's tables point at the error word.
     if ((e = hipMalloc((void **)&b->d_error, 256)) != hipSuccess)
         return gpu_fail("batch error word", e);
+    if (form == kFormPersistent) {
+        if ((e = plan_persistent(b, order, budget, blocks_per_cu)) != hipSuccess)
+            return gpu_fail("batch plan", e);
+    } else if (form == kFormSweep) {
+        if ((e = plan_sweep(b, order, budget)) != hipSuccess) return gpu_fail("batch plan", e);
+    }
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess)
         return gpu_fail("stream", e);
     for (uint32_t s = 0; s < nsub; ++s) {
@@ -545,6 +834,8 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
     if (form == kFormStreams) {
         st = capture_streams(b, order, &e);
         if (st != VXH_OK) what = vxh_last_error();
+    } else if (form == kFormSweep) {
+        e = capture_sweep(b);
     } else {
         e = capture_persistent(b);
     }
@@ -585,7 +876,7 @@ vxh_status vxh_sgm_batch_check(vxh_sgm_batch *b) {
     hipError_t e = hipMemcpy(&err, b->d_error, 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess)
         return bfail(VXH_ERR_GPU, std::string("GPU error: batch check: ") + hipGetErrorString(e));
-    if (err) return bfail(VXH_ERR_GPU, "GPU error: a split scanline's wait timed out");
+    if (err) return bfail(VXH_ERR_GPU, "GPU error: a bounded wait (split scanline or sweep wave) timed out");
     return VXH_OK;
 }
 

@@ -223,6 +223,72 @@ int batch_paths_occupancy_c0(int mode);
 int batch_paths_occupancy_c1(int mode);
 int batch_paths_occupancy_c2(int mode);
 
+// ---- The two-sweep raster form of the batched path ----
+// A forward sweep aggregates the four directions whose predecessor lies on the
+// current or the previous row (+1,0) (0,+1) (+1,+1) (-1,+1) over the image in
+// raster order and stores their sum (u16, pix*Dp + d); a backward sweep runs
+// the other four over the image turned by 180 degrees, adds the forward sum
+// and takes each pixel's winner, so the summed volume is never stored.  A
+// wave owns G = 64 / L groups of L lanes, each group two consecutive rows
+// packed into the halves of every register (K = 4 disparities per lane);
+// row r runs two steps behind row r - 1.  Row states cross a wave boundary
+// through a per-wave ring of slots in global memory (one slot per column,
+// kSweepRec bytes per lane) with a progress word, inside one XCD's L2.
+constexpr int kSweepK = 4;
+// Polls of a sweep wave's bounded wait for the wave above it (tens to a
+// hundred milliseconds; a predecessor always runs, so real waits are
+// microseconds).
+constexpr uint32_t kSweepWaitPolls = 1u << 16;
+constexpr int kSweepRec = 48;   // per lane and slot: three directions x (4 u16, pm, pad)
+inline int sweep_lanes(int D) { return 16 << layout_class(D); }
+inline int sweep_groups(int D) { return 64 / sweep_lanes(D); }
+// Waves of one sweep of a w x h pair.
+inline int sweep_waves(int h, int D) {
+    const int G = sweep_groups(D);
+    return ((h + 1) / 2 + G - 1) / G;
+}
+inline size_t sweep_slot_bytes(int D) {
+    return ((size_t)sweep_lanes(D) * kSweepRec + 127) & ~(size_t)127;
+}
+constexpr size_t sweep_ring_slots(int w) { return (size_t)w + 4; }
+struct SweepJob {
+    const uint8_t *cost;   // matching costs + bias (as PassParams::cost)
+    uint16_t *sf;          // forward sums, pix*Dp + d
+    uint8_t *ring;         // nwaves rings of sweep_ring_slots(w) slots
+    uint32_t *prog;        // nwaves progress words (cleared before each sweep)
+    float *out;            // disparity map
+    int16_t *best;         // LR check: the winners (else null)
+    uint16_t *sum;         // LR check: the summed volume (else null)
+    int w, h, D, Dp;
+    uint32_t p1x2, p2x2, xqx2;   // as PassParams
+    int nwaves;
+    uint32_t slot_bytes;
+};
+struct SweepItem {
+    uint32_t job, wave;
+};
+// One sweep over every job of one lane-layout class.  Waves take items by
+// ticket from the list of the XCD they run on (read from the hardware), each
+// list holding whole pairs, ordered by wave index so that a wave's
+// predecessor always holds an earlier ticket of the same list.
+struct SweepArgs {
+    const SweepJob *jobs;
+    const SweepItem *items;
+    uint32_t *counters;         // 8 ticket counters, zero at launch
+    uint32_t *error;            // raised when a bounded wait expires
+    const uint8_t *fresh;       // zeroed slot (the row above the image)
+    uint32_t item_off[8], item_cnt[8];
+    uint32_t wait_polls;
+    uint32_t zero;              // always 0 (orders a progress store after a load)
+};
+hipError_t launch_sweep_c0(bool bwd, const SweepArgs &a, int blocks, hipStream_t s);
+hipError_t launch_sweep_c1(bool bwd, const SweepArgs &a, int blocks, hipStream_t s);
+hipError_t launch_sweep_c2(bool bwd, const SweepArgs &a, int blocks, hipStream_t s);
+// One-wave blocks of the class's sweep kernel one CU holds at once.
+int sweep_occupancy_c0(bool bwd);
+int sweep_occupancy_c1(bool bwd);
+int sweep_occupancy_c2(bool bwd);
+
 // Census of one image; with `zero_flags` also clears the 8 flag bytes of
 // every pixel (the kOutU8 exception flags of the pass that follows), with
 // `zero_words` the first nwords words there (the split launch's

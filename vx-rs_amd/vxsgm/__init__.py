@@ -487,7 +487,7 @@ class vxh_sgm_batch_stats(ctypes.Structure):
                 ("chunk", _u32), ("form", _u32)]
 
 
-BATCH_FORMS = {"persistent": 0, "streams": 1}
+BATCH_FORMS = {"persistent": 0, "streams": 1, "sweep": 2}
 
 
 _sig("vxh_sgm_batch_create", ctypes.c_int, ctypes.c_int, ctypes.POINTER(vxh_sgm_pair_desc),
@@ -518,7 +518,7 @@ class SGMBatch:
     (lanes, per_lane) for every direction, ((lanes_h, per_lane_h),
     (lanes, per_lane)), or None; volume_format "auto", "u8" or "u16"; split
     a bool.  `options` is None (defaults) or a dict with any of "form"
-    ("persistent" or "streams"), "streams", "blocks_per_cu", "budget_bytes";
+    ("persistent", "streams" or "sweep"), "streams", "blocks_per_cu", "budget_bytes";
     a tuple (streams,) selects the streams form with that many streams."""
 
     def __init__(self, pairs, device=0, options=None):
@@ -561,7 +561,7 @@ class SGMBatch:
         return {"stream": s.stream, "schedule": ((s.lanes_h, s.per_lane_h), (s.lanes, s.per_lane)),
                 "volume_format": {1: "u8", 2: "u16"}[s.volume_format],
                 "workspace_bytes": s.workspace_bytes, "chunk": s.chunk,
-                "form": {0: "persistent", 1: "streams"}[s.form]}
+                "form": {0: "persistent", 1: "streams", 2: "sweep"}[s.form]}
 
     def chunks(self):
         return lib.vxh_sgm_batch_chunks(self._h)

@@ -35,7 +35,8 @@ CLASS_LAYOUTS = [[(32, 2), (16, 4), (8, 8)], [(64, 2), (32, 4), (16, 8)],
 # Batch candidates tried: (options, per-pair layout policy).
 BATCH_CANDIDATES = ([({"form": "streams", "streams": s}, "default") for s in (2, 4, 6)] +
                     [({"form": "persistent", "blocks_per_cu": b}, lay)
-                     for b in (0, 2) for lay in ("default", "widest")])
+                     for b in (0, 2) for lay in ("default", "widest")] +
+                    [({"form": "sweep"}, "default")])
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 
@@ -129,14 +130,18 @@ class Tuner:
             e = self.table[key]
             return e["options"], e["layouts"]
         stream = hip.Stream()
-        times = {}
+        times, chosen = {}, {}
         for opt, lay in BATCH_CANDIDATES:
-            b = vxsgm.SGMBatch(apply_layouts(descs, lay), options=opt)
+            try:
+                b = vxsgm.SGMBatch(apply_layouts(descs, lay), options=opt)
+            except vxsgm.InvalidConfig:
+                continue   # a form the batch's pairs do not allow (sweeps: 8 paths only)
             name = ",".join(f"{k}={v}" for k, v in sorted(opt.items())) + f",layouts={lay}"
             times[name] = self._time(lambda: b.launch(stream.handle), stream, reps)
+            chosen[name] = (opt, lay)
             b.close()
         best = min(times, key=times.get)
-        opt, lay = BATCH_CANDIDATES[list(times).index(best)]
+        opt, lay = chosen[best]
         self.table[key] = {"options": opt, "layouts": lay, "ms": times}
         return opt, lay
 

@@ -251,7 +251,7 @@ hipError_t plan_persistent(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
                         u8 ? (uint2 *)(base + L.flags) : nullptr, (uint32_t *)(base + L.sync),
                         sync_words(q.width, q.height), w, h, (int)q.cfg.census_radius_x,
                         (int)q.cfg.census_radius_y, (w + 63) / 64, cb};
-            cb += (uint32_t)(l.nbx * ((h + 3) / 4));
+            cb += (uint32_t)(l.nbx * ((h + kCensusTileH - 1) / kCensusTileH));
             CensusJob r = l;
             r.img = q.d_right;
             r.out = (uint2 *)(base + L.rc);
@@ -259,7 +259,7 @@ hipError_t plan_persistent(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
             r.zero_words = nullptr;
             r.nwords = 0;
             r.first_block = cb;
-            cb += (uint32_t)(r.nbx * ((h + 3) / 4));
+            cb += (uint32_t)(r.nbx * ((h + kCensusTileH - 1) / kCensusTileH));
             cj.push_back(l);
             cj.push_back(r);
             const uint32_t p1 = (uint32_t)q.cfg.p1 & 0xFFFFu, p2 = (uint32_t)q.cfg.p2 & 0xFFFFu;
@@ -500,12 +500,12 @@ hipError_t plan_sweep(vxh_sgm_batch *b, const std::vector<uint32_t> &order, uint
             CensusJob l{q.d_left, (uint2 *)(base + L.lc), nullptr, nullptr, 0, w, h,
                         (int)q.cfg.census_radius_x, (int)q.cfg.census_radius_y, (w + 63) / 64,
                         cb};
-            cb += (uint32_t)(l.nbx * ((h + 3) / 4));
+            cb += (uint32_t)(l.nbx * ((h + kCensusTileH - 1) / kCensusTileH));
             CensusJob r = l;
             r.img = q.d_right;
             r.out = (uint2 *)(base + L.rc);
             r.first_block = cb;
-            cb += (uint32_t)(r.nbx * ((h + 3) / 4));
+            cb += (uint32_t)(r.nbx * ((h + kCensusTileH - 1) / kCensusTileH));
             cj.push_back(l);
             cj.push_back(r);
             const uint32_t p1 = (uint32_t)q.cfg.p1 & 0xFFFFu, p2 = (uint32_t)q.cfg.p2 & 0xFFFFu;

@@ -161,7 +161,9 @@ struct MultiPass {
 // Each launch covers the blocks of all its jobs; jobs are sorted by
 // first_block, the first at 0, and block b belongs to the job whose
 // [first_block, next job's first_block) holds it.
-struct CensusJob {   // one image: tiles of 64 x 4 pixels, nbx per tile row
+// Rows of one census tile (64 pixels wide; blocks of 64 x 4 threads).
+constexpr int kCensusTileH = 16;
+struct CensusJob {   // one image: tiles of 64 x kCensusTileH pixels, nbx per tile row
     const uint8_t *img;
     uint2 *out;
     uint2 *zero_flags;       // as launch_census
@@ -296,6 +298,11 @@ int sweep_occupancy_c2(bool bwd);
 hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
                          uint2 *out, uint2 *zero_flags, uint32_t *zero_words, size_t nwords,
                          hipStream_t s);
+// launch_census of both images of a pair in one launch (the left one with
+// the clearing of launch_census).
+hipError_t launch_census_pair(const uint8_t *left, const uint8_t *right, int w, int h, int rx,
+                              int ry, uint2 *out_l, uint2 *out_r, uint2 *zero_flags,
+                              uint32_t *zero_words, size_t nwords, hipStream_t s);
 // Matching-cost volume of a pair from its census (layout of
 // PassParams::cost, Dp = volume_stride(D)), every entry plus `bias`
 // (cost_bias(p1)).

@@ -9,14 +9,24 @@ namespace {
 
 using namespace dev;
 
-// Census of 64 x 4 pixels per block (blockDim (64, 4)).  The default 9 x 7
-// window reads its 10 x 72 tile of clamped image rows once into LDS, which
-// reproduces clamp-to-edge exactly, so every pixel then takes the same path:
-// 63 LDS byte reads at constant offsets, and each bit shifted in with one
-// subtraction (n - c has bit 31 set iff n < c) and one alignbit, from the last
-// neighbour down so that neighbour k ends at bit k.  Other windows read the
-// image directly.
-constexpr int kCtW = 64 + 8, kCtH = 4 + 6;
+// Census of 64 x kCensusTileH pixels per block (blockDim (64, 4)); each
+// thread computes one column of 4 consecutive rows.  The default 9 x 7 window
+// first reads the block's (kCensusTileH + 6) x 72 tile of clamped image rows
+// into LDS, which reproduces clamp-to-edge exactly, so every pixel then takes
+// the same path.  A thread reads each of its 10 window rows as three aligned
+// dwords and aligns them to its column with two alignbytes and a shift: the 9
+// bytes of a row then sit at constant byte positions, which the byte-select
+// operands of the subtractions read directly.  Each bit is shifted in with
+// one subtraction (n - c has bit 31 set iff n < c) and one alignbit, from the
+// last neighbour down so that neighbour k ends at bit k.  Other windows read
+// the image directly.
+constexpr int kCtW = 64 + 8, kCtH = kCensusTileH + 6;
+constexpr int kCtRows = kCensusTileH / 4;   // rows per thread
+
+// Byte b of a 9-byte window row held as three dwords (b constant).
+__device__ __forceinline__ uint32_t row_byte(const uint32_t (&r)[3], int b) {
+    return (r[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+}
 
 // Tile (bx, by) of the census of one image (block-uniform arguments).
 __device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, int w, int h, int rx,
@@ -25,36 +35,58 @@ __device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, int
                                             uint32_t *__restrict__ zero_words, size_t nwords,
                                             int bx, int by) {
     const int x = bx * 64 + threadIdx.x;
-    const int y = by * 4 + threadIdx.y;
-    uint2 word;
+    const int yt = by * kCensusTileH + threadIdx.y * kCtRows;   // first row of this thread
+    uint2 words[kCtRows];
     if (rx == 4 && ry == 3) {   // block-uniform
-        __shared__ uint8_t tile[kCtH * kCtW];
-        const int x0 = bx * 64 - 4, y0 = by * 4 - 3;
+        __shared__ __align__(16) uint8_t tile[kCtH * kCtW];
+        const int x0 = bx * 64 - 4, y0 = by * kCensusTileH - 3;
         for (int t = threadIdx.y * 64 + threadIdx.x; t < kCtH * kCtW; t += 256) {
             const int r = t / kCtW, c = t - r * kCtW;
             tile[t] = img[(size_t)min(max(y0 + r, 0), h - 1) * w + min(max(x0 + c, 0), w - 1)];
         }
         __syncthreads();
-        if (x >= w || y >= h) return;
-        const uint8_t *win = tile + threadIdx.y * kCtW + threadIdx.x;   // window's top-left
-        const uint32_t c = win[3 * kCtW + 4];
-        uint32_t lo = 0, hi = 0;
+        if (x >= w || yt >= h) return;
+        // Window rows of this thread: tile rows threadIdx.y * kCtRows + 0 .. kCtRows + 5,
+        // columns threadIdx.x .. threadIdx.x + 8.
+        const uint32_t *src = (const uint32_t *)(tile + threadIdx.y * kCtRows * kCtW) +
+                              (threadIdx.x >> 2);
+        const uint32_t sh = threadIdx.x & 3;
+        uint32_t rows[kCtRows + 6][3];
 #pragma unroll
-        for (int k = 61; k >= 0; --k) {
-            const int j = k < 31 ? k : k + 1;   // window index, the centre (31) skipped
-            const uint32_t d = (uint32_t)win[(j / 9) * kCtW + j % 9] - c;
-            if (k >= 32) hi = __builtin_amdgcn_alignbit(hi, d, 31);
-            else lo = __builtin_amdgcn_alignbit(lo, d, 31);
+        for (int r = 0; r < kCtRows + 6; ++r) {
+            const uint32_t w0 = src[r * (kCtW / 4)], w1 = src[r * (kCtW / 4) + 1],
+                           w2 = src[r * (kCtW / 4) + 2];
+            rows[r][0] = __builtin_amdgcn_alignbyte(w1, w0, sh);
+            rows[r][1] = __builtin_amdgcn_alignbyte(w2, w1, sh);
+            rows[r][2] = w2 >> (8 * sh);
         }
-        word = make_uint2(lo, hi);
+#pragma unroll
+        for (int p = 0; p < kCtRows; ++p) {
+            const uint32_t c = row_byte(rows[p + 3], 4);
+            uint32_t lo = 0, hi = 0;
+#pragma unroll
+            for (int k = 61; k >= 0; --k) {
+                const int j = k < 31 ? k : k + 1;   // window index, the centre (31) skipped
+                const uint32_t d = row_byte(rows[p + j / 9], j % 9) - c;
+                if (k >= 32) hi = __builtin_amdgcn_alignbit(hi, d, 31);
+                else lo = __builtin_amdgcn_alignbit(lo, d, 31);
+            }
+            words[p] = make_uint2(lo, hi);
+        }
     } else {
-        if (x >= w || y >= h) return;
-        word = census_at(img, w, h, rx, ry, x, y);
+        if (x >= w || yt >= h) return;
+#pragma unroll
+        for (int p = 0; p < kCtRows; ++p)
+            words[p] = yt + p < h ? census_at(img, w, h, rx, ry, x, yt + p) : make_uint2(0, 0);
     }
-    const size_t i = (size_t)y * w + x;
-    out[i] = word;
-    if (zero_flags) zero_flags[i] = make_uint2(0, 0);
-    for (size_t k = i; k < nwords; k += (size_t)w * h) zero_words[k] = 0;
+#pragma unroll
+    for (int p = 0; p < kCtRows; ++p) {
+        if (yt + p >= h) break;
+        const size_t i = (size_t)(yt + p) * w + x;
+        out[i] = words[p];
+        if (zero_flags) zero_flags[i] = make_uint2(0, 0);
+        for (size_t k = i; k < nwords; k += (size_t)w * h) zero_words[k] = 0;
+    }
 }
 
 __global__ void __launch_bounds__(256)
@@ -63,6 +95,18 @@ census_kernel(const uint8_t *__restrict__ img, int w, int h, int rx, int ry,
               uint32_t *__restrict__ zero_words, size_t nwords) {
     census_tile(img, w, h, rx, ry, out, zero_flags, zero_words, nwords, (int)blockIdx.x,
                 (int)blockIdx.y);
+}
+
+// Census of both images of a pair in one launch (blockIdx.z: 0 left, 1
+// right); only the left one clears the flags and words.
+__global__ void __launch_bounds__(256)
+census_pair_kernel(const uint8_t *__restrict__ left, const uint8_t *__restrict__ right, int w,
+                   int h, int rx, int ry, uint2 *__restrict__ out_l, uint2 *__restrict__ out_r,
+                   uint2 *__restrict__ zero_flags, uint32_t *__restrict__ zero_words,
+                   size_t nwords) {
+    const bool r = blockIdx.z != 0;
+    census_tile(r ? right : left, w, h, rx, ry, r ? out_r : out_l, r ? nullptr : zero_flags,
+                r ? nullptr : zero_words, r ? 0 : nwords, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 // Index of the table entry whose [first_block, next first_block) holds block
@@ -91,18 +135,114 @@ census_multi_kernel(const CensusJob *__restrict__ jobs, int njobs) {
 
 // Matching-cost volume of a pair from its census (SGMStereo.metal:133-142),
 // computed once and read by every direction pass.  A block (4 waves) owns
-// 256 pixels of one row: it stages the left census words of the tile and the
-// right census words the tile reads (the tile and the D - 1 pixels before
-// it) in LDS, then each wave computes the cost slices of its 64 pixels, one
-// pixel per lane (a lane reads 16 consecutive right-census words per chunk
-// at constant offsets), into an LDS tile that it copies out as contiguous
-// 16-byte chunks.  Slices wider than 128 entries go in two 128-entry passes,
-// each a whole 128-byte line per pixel.
+// kCcTile pixels of one row and stages in LDS the left census words of the
+// tile and the right census words it reads (the tile and the 255 pixels
+// before it).  Each wave then walks 64 consecutive pixels, NG pixels per
+// step: a group of 64 / NG lanes covers one pixel's slice, lane s of the
+// group the four entries 4s .. 4s + 3, stored as one dword, so one step
+// stores NG whole slices contiguously.
+//
+// A lane's right words form one ascending stream: entry j of the group's
+// pixel x is the word of x - 4s - j, and the group's next pixel is NG
+// further.  The lane reads the stream in aligned 16-byte chunks of four words
+// (one per dword array: the low and the high halves are staged apart), three
+// chunks ahead of use, so the LDS reads 8 / 4 / 2 bytes per entry for slices
+// of <= 64 / 128 / 256 entries without waiting on them.  The lanes of a
+// group read adjacent chunks (no bank conflicts); each group has its own copy
+// of the arrays, shifted so that its chunks are 16-byte aligned.
 constexpr int kCcTile = 256;                      // pixels per block
-constexpr int kCcSeg = 128;                       // slice entries per pass
-constexpr int kCcOutRow = kCcSeg + 16;            // LDS row of one pixel (bank spread)
-constexpr int kCcSpanR = kCcTile + 256;           // right census words (D - 1 <= 255 before)
-constexpr size_t kCcLds = (size_t)(kCcSpanR + kCcTile) * 8 + 4 * 64 * kCcOutRow;
+constexpr int kCcSpanR = kCcTile + 256 + 4;       // right census words: 255 before, 4 after
+constexpr int kCcArr = kCcSpanR + 4;              // dwords per array (room for the shift)
+constexpr size_t kCcLds = (size_t)4 * 2 * kCcArr * 4 + kCcTile * 8;   // up to 4 copies + left
+
+// popcount(x) + acc in one instruction (the compiler otherwise adds the bias
+// and the two halves' counts separately).
+__device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc) {
+    uint32_t r;
+    asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+    return r;
+}
+
+__device__ __forceinline__ uint32_t elem(const uint4 &v, int i) {
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+
+// Packed costs of four entries: byte j is popcount(l ^ r_j) + bias.
+__device__ __forceinline__ uint32_t cost_word(uint2 l, const uint32_t (&lo)[4],
+                                              const uint32_t (&hi)[4], uint32_t bias) {
+    uint32_t c[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[j] = bcnt_acc(l.y ^ hi[j], bcnt_acc(l.x ^ lo[j], bias));
+    // Three byte permutes: (c0, c1) and (c2, c3) into halves, then the halves.
+    const uint32_t v01 = __builtin_amdgcn_perm(c[1], c[0], 0x0c0c0400u);
+    const uint32_t v23 = __builtin_amdgcn_perm(c[3], c[2], 0x0c0c0400u);
+    return __builtin_amdgcn_perm(v23, v01, 0x05040100u);
+}
+
+// One wave's run of 64 pixels from xw.  `slo` / `shi` point at the lane's
+// stream (16-byte aligned): element m is the word of pixel x - 4s - 3 + m.
+// cen_l[k] is the left word of pixel x0 + k.  FAST: every pixel of the run
+// is inside the row, every lane stores, and no entry reads a pixel left of
+// the row or lies past D; otherwise those entries get 64 + bias and the
+// stores are predicated.
+template <int NG, bool FAST>
+__device__ __forceinline__ void cost_run(const uint4 *slo, const uint4 *shi, const uint2 *cen_l,
+                                         int x0, int xw, int w, int D, int Dp, uint32_t bias,
+                                         uint8_t *__restrict__ row_out) {
+    constexpr int G = 64 / NG;                    // lanes per pixel
+    constexpr int NC = (NG * (64 / NG - 1) + 3) / 4 + 1;   // chunks of the stream used
+    const int lane = (int)(threadIdx.x & 63);
+    const int g = lane / G, d0 = 4 * (lane % G);
+    const int x = xw + g;
+    const uint32_t fill = (64u + bias) * 0x01010101u;
+    // Entries d >= D of this lane (a slice's padding up to Dp).
+    const uint32_t mask_d = d0 >= D ? ~0u : (uint32_t)(~0ull << (8 * min(D - d0, 4)));
+    const bool stores = d0 < Dp;
+    // A fast run's slices are exactly the group's 4G entries: the stride is
+    // then a constant and the stores take immediate offsets.
+    const int Ds = FAST ? 4 * G : Dp;
+    uint32_t *dst = (uint32_t *)(row_out + (size_t)x * Ds + d0);
+    const int step = NG * Ds / 4;                 // dwords from one pixel of the group to its next
+    const uint2 *lp = cen_l + (x - x0);
+    // Four slots: chunk q in slot q % 4, fetched three chunks ahead.  The
+    // steps run as 4 rounds of 4 chunks each, so slots are compile-time.
+    constexpr int SR = 16 / NG;                   // steps per round
+    uint4 bl[4], bh[4];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        bl[q] = slo[q];
+        bh[q] = shi[q];
+    }
+#pragma unroll 1
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int ii = 0; ii < SR; ++ii) {
+            const int m0 = NG * ii, qq = m0 / 4;          // chunk 4r + qq
+            if (m0 % 4 < NG && 4 * r + qq + 3 < NC) {     // first step in the chunk
+                bl[(qq + 3) % 4] = slo[4 * r + qq + 3];
+                bh[(qq + 3) % 4] = shi[4 * r + qq + 3];
+            }
+            uint32_t lo[4], hi[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {                 // entry j: element m0 + 3 - j
+                const int m = m0 + 3 - j;
+                lo[j] = elem(bl[(m / 4) % 4], m % 4);
+                hi[j] = elem(bh[(m / 4) % 4], m % 4);
+            }
+            const int i = SR * r + ii;
+            uint32_t v = cost_word(lp[NG * i], lo, hi, bias);
+            if (FAST) {
+                dst[i * step] = v;
+            } else {
+                // Entries with x - d < 0 (j > x - d0) and d >= D cost 64.
+                const int t = min(max(x + NG * i - d0 + 1, 0), 4);
+                const uint32_t mk = (uint32_t)(~0ull << (8 * t)) | mask_d;
+                v = (mk & fill) | (~mk & v);
+                if (stores && x + NG * i < w) dst[i * step] = v;
+            }
+        }
+    }
+}
 
 // Tile bx of row y of a pair's cost volume (block-uniform arguments; `lds`
 // holds kCcLds bytes).
@@ -110,84 +250,47 @@ __device__ __forceinline__ void cost_tile(const uint2 *__restrict__ lc,
                                           const uint2 *__restrict__ rc, int w, int h, int D,
                                           int Dp, uint32_t bias, uint8_t *__restrict__ cost,
                                           int bx, int y, uint8_t *lds) {
-    uint2 *cen_r = (uint2 *)lds;                            // kCcSpanR words
-    uint2 *cen_l = cen_r + kCcSpanR;                        // kCcTile words
-    uint8_t *out_all = (uint8_t *)(cen_l + kCcTile);        // 4 x 64 x kCcOutRow, 16-aligned
+    const int ng = Dp <= 64 ? 4 : (Dp <= 128 ? 2 : 1);   // groups (pixels) per wave step
+    uint32_t *arr = (uint32_t *)lds;                      // copy c: low at arr + c * 2 * kCcArr
+    uint2 *cen_l = (uint2 *)(arr + 4 * 2 * kCcArr);       // + shift(c), high kCcArr further
     const int t = threadIdx.x;
     const int x0 = bx * kCcTile;
-    const int xs0 = max(0, x0 - (D - 1));                   // first right-census pixel
-    const int xs1 = min(w, x0 + kCcTile);
-    const int nr = xs1 - xs0, nl = xs1 - x0;
-    {
-        const size_t row = (size_t)y * w;
-        for (int i = t; i < nr + nl; i += 256) {
-            if (i < nr) cen_r[i] = rc[row + xs0 + i];
-            else cen_l[i - nr] = lc[row + x0 + i - nr];
+    const int xr0 = x0 - 255;                             // pixel of stream index 0
+    const size_t row = (size_t)y * w;
+    // Copy c holds index k at dword shift(c) + k, shift(c) = (4 - c) % 4: group
+    // c's stream starts at an index = c (mod 4), so its chunks are aligned.
+    // Words outside the row are clamped: every entry that reads one is
+    // replaced by 64 + bias (x - d < 0) or lies past the row's end.
+    for (int i = t; i < kCcSpanR; i += 256) {
+        const uint2 v = rc[row + min(max(xr0 + i, 0), w - 1)];
+        for (int c = 0; c < ng; ++c) {
+            uint32_t *lo = arr + c * 2 * kCcArr + ((4 - c) & 3);
+            lo[i] = v.x;
+            lo[i + kCcArr] = v.y;
         }
     }
+    cen_l[t] = lc[row + min(x0 + t, w - 1)];
     __syncthreads();
-    const int wv = t >> 6, lane = t & 63;
-    const int p0 = wv * 64;                                 // first pixel of this wave
-    const int npx = min(64, nl - p0);
-    if (npx <= 0) return;                                   // wave-uniform
-    uint8_t *out = out_all + wv * 64 * kCcOutRow;
-    const int x = x0 + p0 + lane;
-    const bool live = lane < npx;
-    const uint2 lv = live ? cen_l[p0 + lane] : make_uint2(0, 0);
-    const int rel = x - xs0;                                // cen_r index of x - d is rel - d
-    const bool inside = x0 + p0 >= D - 1;                   // no lane has x - d < 0 for d < D
-    uint8_t *dst = cost + ((size_t)y * w + x0 + p0) * Dp;
-    for (int seg = 0; seg < Dp; seg += kCcSeg) {
-        const int segw = min(kCcSeg, Dp - seg);
-        if (live) {
-            for (int dc = 0; dc < segw; dc += 16) {
-                const int d0 = seg + dc;
-                uint32_t wd[4];
-                auto chunk = [&](auto checked) {
-                    // Words rel - d0 - 15 .. rel - d0, read at constant offsets.
-                    const uint2 *rb = cen_r + (decltype(checked)::value ? 0 : rel - d0 - 15);
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        uint32_t v = 0;
-#pragma unroll
-                        for (int b = 0; b < 4; ++b) {
-                            const int i = 4 * q + b, d = d0 + i;
-                            uint32_t c;
-                            if constexpr (decltype(checked)::value) {
-                                const uint2 r = cen_r[max(rel - d, 0)];
-                                c = (d < D && d <= x) ? __popc(lv.x ^ r.x) + __popc(lv.y ^ r.y)
-                                                      : 64u;
-                            } else {
-                                const uint2 r = rb[15 - i];
-                                c = __popc(lv.x ^ r.x) + __popc(lv.y ^ r.y);
-                            }
-                            v |= (c + bias) << (8 * b);
-                        }
-                        wd[q] = v;
-                    }
-                };
-                if (inside && d0 + 16 <= D) chunk(std::false_type{});
-                else chunk(std::true_type{});
-                *(uint4 *)(out + lane * kCcOutRow + dc) = make_uint4(wd[0], wd[1], wd[2], wd[3]);
-            }
-        }
-        // The tile's rows come from other lanes of this wave.
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // Chunk i is chunk c of pixel px: i / nch by a multiply, exact for
-        // nch <= 8 and i < 64 * nch (the error is below 1/128, the gap to the
-        // next multiple at least 1/8); offsets from dst fit 32 bits.
-        const int nch = segw / 16;
-        const uint32_t magic = (65536u + (uint32_t)nch - 1u) / (uint32_t)nch;
-        for (int i = lane; i < npx * nch; i += 64) {
-            const int px = (int)(((uint32_t)i * magic) >> 16), c = i - px * nch;
-            *(uint4 *)(dst + (uint32_t)(px * Dp + seg + c * 16)) =
-                *(const uint4 *)(out + px * kCcOutRow + c * 16);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int xw = x0 + (t >> 6) * 64;                    // first pixel of this wave
+    if (xw >= w) return;                                  // wave-uniform
+    uint8_t *row_out = cost + (size_t)y * w * Dp;
+    const int G = 64 / ng, lane = t & 63;
+    const int g = lane / G, d0 = 4 * (lane % G);
+    // The lane's stream starts at the word of pixel xw + g - d0 - 3.
+    const uint32_t *lo = arr + g * 2 * kCcArr + ((4 - g) & 3) + (xw + g - d0 - 3 - xr0);
+    const uint4 *slo = (const uint4 *)lo, *shi = (const uint4 *)(lo + kCcArr);
+    // Fast runs: inside the row, no entry left of it (x - d >= 0 for every
+    // d < Dp), no padding entries, and every lane of a group stores.
+    const bool fast = xw >= Dp - 1 && xw + 64 <= w && D == Dp && Dp == 4 * G;
+    if (ng == 4) {
+        if (fast) cost_run<4, true>(slo, shi, cen_l, x0, xw, w, D, Dp, bias, row_out);
+        else cost_run<4, false>(slo, shi, cen_l, x0, xw, w, D, Dp, bias, row_out);
+    } else if (ng == 2) {
+        if (fast) cost_run<2, true>(slo, shi, cen_l, x0, xw, w, D, Dp, bias, row_out);
+        else cost_run<2, false>(slo, shi, cen_l, x0, xw, w, D, Dp, bias, row_out);
+    } else {
+        if (fast) cost_run<1, true>(slo, shi, cen_l, x0, xw, w, D, Dp, bias, row_out);
+        else cost_run<1, false>(slo, shi, cen_l, x0, xw, w, D, Dp, bias, row_out);
     }
 }
 
@@ -280,9 +383,19 @@ hipError_t launch_census(const uint8_t *img, int w, int h, int rx, int ry,
                          uint2 *out, uint2 *zero_flags, uint32_t *zero_words, size_t nwords,
                          hipStream_t s) {
     dim3 block(64, 4);
-    dim3 grid((w + 63) / 64, (h + 3) / 4);
+    dim3 grid((w + 63) / 64, (h + kCensusTileH - 1) / kCensusTileH);
     hipLaunchKernelGGL(census_kernel, grid, block, 0, s, img, w, h, rx, ry, out, zero_flags,
                        zero_words, zero_words ? nwords : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_census_pair(const uint8_t *left, const uint8_t *right, int w, int h, int rx,
+                              int ry, uint2 *out_l, uint2 *out_r, uint2 *zero_flags,
+                              uint32_t *zero_words, size_t nwords, hipStream_t s) {
+    dim3 block(64, 4);
+    dim3 grid((w + 63) / 64, (h + kCensusTileH - 1) / kCensusTileH, 2);
+    hipLaunchKernelGGL(census_pair_kernel, grid, block, 0, s, left, right, w, h, rx, ry, out_l,
+                       out_r, zero_flags, zero_words, zero_words ? nwords : 0);
     return hipGetLastError();
 }
 

@@ -690,11 +690,9 @@ vxh_status run_pipeline(const uint8_t *d_left, const uint8_t *d_right, uint32_t 
     uint2 *lc = (uint2 *)(base + L.lc);
     uint2 *rc = (uint2 *)(base + L.rc);
     uint2 *flags = mode == kOutU8 ? (uint2 *)(base + L.flags) : nullptr;
-    VXH_HIP(launch_census(d_left, w, h, c->census_radius_x, c->census_radius_y, lc, flags,
-                          (uint32_t *)(base + L.sync), sync_words(w, h), st),
-            "census");
-    VXH_HIP(launch_census(d_right, w, h, c->census_radius_x, c->census_radius_y, rc, nullptr,
-                          nullptr, 0, st),
+    VXH_HIP(launch_census_pair(d_left, d_right, (int)w, (int)h, c->census_radius_x,
+                               c->census_radius_y, lc, rc, flags, (uint32_t *)(base + L.sync),
+                               sync_words(w, h), st),
             "census");
     uint8_t *cost = (uint8_t *)(base + L.cost);
     VXH_HIP(launch_cost(lc, rc, (int)w, (int)h, D, cost_bias((uint32_t)c->p1 & 0xFFFFu), cost,

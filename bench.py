@@ -62,6 +62,8 @@ def parse():
                     help="720p pairs per replay of the stream form")
     ap.add_argument("--batch-pairs", type=int, default=64)
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--batch-layouts", default="default", choices=["default", "widest"],
+                    help="per-pair lane layouts of the batched leg with --batch-options")
     ap.add_argument("--rates", default="200,500,1000,1300,1600,2000",
                     help="Poisson arrival rates of the served leg (req/s)")
     ap.add_argument("--slo-ms", type=float, default=25.0,
@@ -147,7 +149,7 @@ def batched_leg(vx, hip, np, args, sgm, tuner):
     from vxsgm.tuner import apply_layouts
     lay = "default"
     if args.batch_options:
-        opts = json.loads(args.batch_options)
+        opts, lay = json.loads(args.batch_options), args.batch_layouts
     else:
         opts, lay = (None, "default") if tuner is None else tuner.tune_batch(descs)
     log(f"batched: {len(host)} pairs, options {opts}, layouts {lay}")

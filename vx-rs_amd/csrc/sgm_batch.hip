@@ -107,13 +107,15 @@ struct Chunk {
     const vxh::CostJob *cost = nullptr;
     int ncost = 0;
     uint32_t cost_blocks = 0;
-    const vxh::WtaJob *wta = nullptr;
-    int nwta = 0;
-    uint32_t wta_blocks = 0;
+    // One aggregation launch per lane-layout class and volume format, each
+    // followed on its own stream by the sums + winners of its pairs.
     struct Group {
         int cls, mode;
         vxh::BatchPaths args;
         int blocks;
+        const vxh::WtaJob *wta;
+        int nwta;
+        uint32_t wta_blocks;
     };
     std::vector<Group> groups;
     // Sweep form: per lane-layout class present, the forward and backward
@@ -218,8 +220,8 @@ hipError_t plan_persistent(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
     // it is allocated.
     std::vector<uint8_t> img;
     struct Off {
-        size_t census, cost, wta;
-        std::vector<size_t> passes, items;
+        size_t census, cost;
+        std::vector<size_t> passes, items, wta;
     };
     std::vector<Off> off(b->chunks.size());
     size_t ncounters = 0;
@@ -227,12 +229,13 @@ hipError_t plan_persistent(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
         Chunk &c = b->chunks[ci];
         std::vector<CensusJob> cj;
         std::vector<CostJob> kj;
-        std::vector<WtaJob> wj;
-        uint32_t cb = 0, kb = 0, wb = 0;
+        uint32_t cb = 0, kb = 0;
         struct G {
             int cls, mode;
             std::vector<BatchPass> passes;
             std::vector<BatchItem> items[8];
+            std::vector<WtaJob> wta;   // blocks numbered within the group
+            uint32_t wb = 0;
         };
         std::vector<G> groups;
         for (uint32_t i : c.pairs) {
@@ -268,23 +271,23 @@ hipError_t plan_persistent(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
                       (w + 255) / 256, kb};
             kb += (uint32_t)(k.nbx * h);
             kj.push_back(k);
+            // The group of the pair's layout class and volume format: its
+            // direction passes (longest scanlines first) and its winner job.
+            const int cls = layout_class(D);
+            size_t gi = groups.size();
+            for (size_t x = 0; x < groups.size(); ++x)
+                if (groups[x].cls == cls && groups[x].mode == p.mode) gi = x;
+            if (gi == groups.size()) groups.push_back(G{cls, p.mode, {}, {}, {}, 0});
+            G &g = groups[gi];
             const bool lr = q.cfg.lr_check != 0;
             WtaJob t{(const uint8_t *)(base + L.vol), L.vol_bytes,
                      u8 ? (const uint8_t *)(base + L.hi) : nullptr,
                      u8 ? (const uint2 *)(base + L.flags) : nullptr, q.d_out,
                      lr ? (int16_t *)(base + L.best) : nullptr,
                      lr ? (uint16_t *)(base + L.sum) : nullptr, (int)q.cfg.num_paths, p.mode,
-                     w * h, D, Dp, wb};
-            wb += (uint32_t)((w * h + 15) / 16);
-            wj.push_back(t);
-            // Direction passes, longest scanlines first, into the group of
-            // the pair's layout class and volume format.
-            const int cls = layout_class(D);
-            size_t gi = groups.size();
-            for (size_t x = 0; x < groups.size(); ++x)
-                if (groups[x].cls == cls && groups[x].mode == p.mode) gi = x;
-            if (gi == groups.size()) groups.push_back(G{cls, p.mode, {}, {}});
-            G &g = groups[gi];
+                     w * h, D, Dp, g.wb};
+            g.wb += (uint32_t)((w * h + 15) / 16);
+            g.wta.push_back(t);
             PassParams pp{};
             pp.cost = (const uint8_t *)(base + L.cost);
             pp.flags = u8 ? (uint8_t *)(base + L.flags) : nullptr;
@@ -327,13 +330,10 @@ hipError_t plan_persistent(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
         c.census_blocks = cb;
         c.ncost = (int)kj.size();
         c.cost_blocks = kb;
-        c.nwta = (int)wj.size();
-        c.wta_blocks = wb;
         off[ci].census = stage(img, cj.data(), cj.size() * sizeof(CensusJob));
         off[ci].cost = stage(img, kj.data(), kj.size() * sizeof(CostJob));
-        off[ci].wta = stage(img, wj.data(), wj.size() * sizeof(WtaJob));
         for (G &g : groups) {
-            Chunk::Group cg{g.cls, g.mode, {}, 0};
+            Chunk::Group cg{g.cls, g.mode, {}, 0, nullptr, (int)g.wta.size(), g.wb};
             std::vector<BatchItem> all;
             uint32_t most = 0;
             for (int xk = 0; xk < 8; ++xk) {
@@ -345,6 +345,7 @@ hipError_t plan_persistent(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
             off[ci].passes.push_back(
                 stage(img, g.passes.data(), g.passes.size() * sizeof(BatchPass)));
             off[ci].items.push_back(stage(img, all.data(), all.size() * sizeof(BatchItem)));
+            off[ci].wta.push_back(stage(img, g.wta.data(), g.wta.size() * sizeof(WtaJob)));
             // Persistent grid: as many blocks as the chip holds at once (or
             // blocks_per_cu per CU), no more than the items need, and a
             // multiple of 8 so every XCD's list has waves.
@@ -368,11 +369,11 @@ hipError_t plan_persistent(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
         Chunk &c = b->chunks[ci];
         c.census = (const CensusJob *)(tb + off[ci].census);
         c.cost = (const CostJob *)(tb + off[ci].cost);
-        c.wta = (const WtaJob *)(tb + off[ci].wta);
         for (size_t g = 0; g < c.groups.size(); ++g) {
             BatchPaths &a = c.groups[g].args;
             a.passes = (const BatchPass *)(tb + off[ci].passes[g]);
             a.items = (const BatchItem *)(tb + off[ci].items[g]);
+            c.groups[g].wta = (const WtaJob *)(tb + off[ci].wta[g]);
             a.counters = b->counters + next_counter;
             next_counter += 8;
         }
@@ -382,8 +383,9 @@ hipError_t plan_persistent(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
 
 // Records the persistent form's launches on b->stream (being captured): the
 // counters and the error word cleared, then per chunk census, costs, the
-// aggregation groups side by side on the sub streams, sums + winners and the
-// LR checks.
+// aggregation groups side by side on the sub streams, each followed on its
+// stream by the sums + winners of its pairs (a group that finishes early
+// takes its winners while the others still aggregate), and the LR checks.
 hipError_t capture_persistent(vxh_sgm_batch *b) {
     using namespace vxh;
     hipError_t e = launch_zero(b->d_error, 1, b->stream);
@@ -396,23 +398,22 @@ hipError_t capture_persistent(vxh_sgm_batch *b) {
         if (e == hipSuccess && c.groups.size() == 1) {
             const Chunk::Group &g = c.groups[0];
             e = launch_class(g.cls, g.mode, g.args, g.blocks, b->stream);
-            b->launches += 1;
+            if (e == hipSuccess) e = launch_wta_multi(g.wta, g.nwta, g.wta_blocks, b->stream);
+            b->launches += 2;
         } else if (e == hipSuccess) {
             // Groups side by side: fork onto the sub streams, join back.
             e = hipEventRecord(b->events[0], b->stream);
             for (size_t g = 0; g < c.groups.size() && e == hipSuccess; ++g) {
                 hipStream_t s = b->sub[g];
                 e = hipStreamWaitEvent(s, b->events[0], 0);
-                if (e == hipSuccess)
-                    e = launch_class(c.groups[g].cls, c.groups[g].mode, c.groups[g].args,
-                                     c.groups[g].blocks, s);
+                const Chunk::Group &cg = c.groups[g];
+                if (e == hipSuccess) e = launch_class(cg.cls, cg.mode, cg.args, cg.blocks, s);
+                if (e == hipSuccess) e = launch_wta_multi(cg.wta, cg.nwta, cg.wta_blocks, s);
                 if (e == hipSuccess) e = hipEventRecord(b->events[1 + g], s);
                 if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->events[1 + g], 0);
-                b->launches += 1;
+                b->launches += 2;
             }
         }
-        if (e == hipSuccess) e = launch_wta_multi(c.wta, c.nwta, c.wta_blocks, b->stream);
-        b->launches += 1;
         for (uint32_t i : c.pairs) {
             const vxh_sgm_pair_desc &q = b->pairs[i];
             if (!q.cfg.lr_check || e != hipSuccess) continue;

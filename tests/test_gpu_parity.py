@@ -71,6 +71,9 @@ VOLUME_CASES = [
     (65, 12, 64, 8), (80, 11, 65, 8), (150, 9, 100, 8), (130, 13, 128, 8),
     (140, 6, 129, 4), (200, 7, 192, 8), (256, 5, 255, 8), (300, 6, 256, 8),
     (257, 1, 64, 8), (3, 40, 3, 8), (1, 1, 1, 8),
+    # Rows long enough for the cost kernel's unpredicated runs (a run of 64
+    # pixels inside the row with x - d >= 0 for every d), then a partial run.
+    (330, 4, 64, 4), (333, 5, 128, 8), (600, 3, 256, 8),
 ]
 
 

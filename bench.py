@@ -1,6 +1,6 @@
 """SGM stereo benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms plain,batched,stream,served]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms plain,batched,stream,served,next]
 
 Headline (`value`): the plain per-pair path.  A step is one pass of the SGM
 pipeline (census of both images, matching costs, path aggregation over every
@@ -57,7 +57,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle threads of the all-cores CPU baseline (0 = every CPU this "
                          "process may run on)")
-    ap.add_argument("--forms", default="plain,batched,stream,served")
+    ap.add_argument("--forms", default="plain,batched,stream,served,next")
+    ap.add_argument("--next-reps", type=int, default=50,
+                    help="launches timed per kernel of the next-rows leg")
     ap.add_argument("--stream-pairs", type=int, default=8,
                     help="720p pairs per replay of the stream form")
     ap.add_argument("--batch-pairs", type=int, default=64)
@@ -271,6 +273,89 @@ def golden_hashes(name):
 def map_sha256(a):
     import hashlib
     return hashlib.sha256(a.view("uint32").tobytes()).hexdigest()
+
+
+def next_rows_leg(vx, hip, np, args):
+    """The kernels after and before the SGM path, each timed over `reps`
+    stream-ordered launches with HIP events on its own stream: disparity to
+    depth, the median and bilateral depth filters (vx-vision's DepthFilter
+    bench sizes: 640x480, radius 1 median, radius 3 bilateral), the gray8
+    remap through rectify maps and the map builder, at 640x480 and 1280x720;
+    and the sparse ORB stereo match at 512 x 512 keypoints.  Per kernel: us per
+    launch, the bytes it must move (one read of its inputs, one write of its
+    outputs) and that rate against 8 TB/s.  Outputs are not checked here (the
+    GPU tests compare every one of these kernels with the oracle)."""
+    reps = args.next_reps
+    stream = hip.Stream()
+    e0, e1 = hip.Event(), hip.Event()
+
+    def timed(launch):
+        launch()
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            launch()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / reps * 1e3
+
+    out = {"reps": reps, "unit": "us per launch", "peak_gbs": 8000.0}
+    rng = np.random.default_rng(0x5347_0006)
+    for (w, h) in ((640, 480), (1280, 720)):
+        n = w * h
+        disp = rng.uniform(0.0, 127.0, n).astype(np.float32)
+        disp[rng.random(n) < 0.05] = -1.0      # invalid pixels, as the LR check leaves them
+        depth = (0.12 * 700.0 / np.maximum(disp, 1.0)).astype(np.float32)
+        img = rng.integers(0, 256, n, dtype=np.uint8)
+        d_disp, d_depth = hip.DeviceBuffer.from_array(disp), hip.DeviceBuffer.from_array(depth)
+        d_out, d_img, d_img2 = hip.DeviceBuffer(n * 4), hip.DeviceBuffer.from_array(img), \
+            hip.DeviceBuffer(n)
+        d_mx, d_my = hip.DeviceBuffer(n * 4), hip.DeviceBuffer(n * 4)
+        rp = vx.RectifyParams(fx=700.0, fy=700.0, cx=w / 2, cy=h / 2, k1=-0.05, k2=0.01,
+                              R=(0.9998, -0.0175, 0.0, 0.0175, 0.9998, 0.0, 0.0, 0.0, 1.0))
+        s = stream.handle
+        cases = [
+            ("disparity_to_depth", 8 * n, lambda: vx.disparity_to_depth(
+                d_disp.ptr, d_out.ptr, w, h, 700.0, 0.12, s)),
+            ("depth_median_r1", 8 * n, lambda: vx.DepthFilter.enqueue_median(
+                d_depth.ptr, d_out.ptr, w, h, vx.DepthMedianConfig(1), s)),
+            ("depth_bilateral_r3", 8 * n, lambda: vx.DepthFilter.enqueue_bilateral(
+                d_depth.ptr, d_out.ptr, w, h, vx.DepthFilterConfig(3, 5.0, 0.05), s)),
+            ("rectify_maps", 8 * n, lambda: vx.rectify_maps_enqueue(
+                rp, w, h, d_mx.ptr, d_my.ptr, s)),
+            ("remap_gray8", 10 * n, lambda: vx.remap_enqueue(
+                d_img.ptr, 1, w, h, d_mx.ptr, d_my.ptr, d_img2.ptr, 1, w, h, s)),
+        ]
+        for name, nbytes, launch in cases:
+            us = timed(launch)
+            out[f"{name}_{w}x{h}"] = {"us": round(us, 2), "bytes": nbytes,
+                                       "gbs": round(nbytes / us / 1e3, 1),
+                                       "frac": round(nbytes / us / 1e3 / 8000.0, 4)}
+    nk = 512
+    kp = np.zeros(nk, vx.CORNER_DTYPE)
+    kp["x"] = rng.uniform(0, 640, nk)
+    kp["y"] = rng.uniform(0, 480, nk)
+    kq = kp.copy()
+    kq["x"] -= rng.uniform(0, 60, nk)
+    kq["y"] += rng.uniform(-1, 1, nk)
+    dl = np.zeros(nk, vx.ORB_DTYPE)
+    dl["desc"] = rng.integers(0, 1 << 32, (nk, 8), dtype=np.uint32)
+    dr = dl.copy()
+    flip = rng.integers(0, 1 << 32, (nk, 8), dtype=np.uint32) & \
+        rng.integers(0, 1 << 32, (nk, 8), dtype=np.uint32) & \
+        rng.integers(0, 1 << 32, (nk, 8), dtype=np.uint32)
+    dr["desc"] ^= flip
+    d_lk, d_rk = hip.DeviceBuffer.from_array(kp), hip.DeviceBuffer.from_array(kq)
+    d_ld = hip.DeviceBuffer.from_array(np.ascontiguousarray(dl["desc"]))
+    d_rd = hip.DeviceBuffer.from_array(np.ascontiguousarray(dr["desc"]))
+    d_m, d_n = hip.DeviceBuffer(nk * vx.MATCH_DTYPE.itemsize), hip.DeviceBuffer(4)
+    cfg = vx.StereoConfig()
+    us = timed(lambda: vx.stereo_match_enqueue(d_lk.ptr, d_rk.ptr, d_ld.ptr, d_rd.ptr, nk, nk,
+                                               cfg, d_m.ptr, d_n.ptr, stream.handle))
+    nbytes = 2 * nk * (16 + 32) + nk * vx.MATCH_DTYPE.itemsize
+    out["stereo_match_512x512"] = {"us": round(us, 2), "bytes": nbytes,
+                                   "note": "launch-bound: the descriptors fit in L2"}
+    return out
 
 
 def served_leg(vx, np, args):
@@ -526,6 +611,9 @@ def main():
     if "served" in forms:
         log(f"served leg (rates {args.rates})")
         result["served"] = served_leg(vx, np, args)
+    if "next" in forms:
+        log("next-rows leg (depth, filters, rectify, sparse stereo)")
+        result["next_rows"] = next_rows_leg(vx, hip, np, args)
     print(json.dumps(result), flush=True)
 
 

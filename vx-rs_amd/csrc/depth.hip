@@ -6,8 +6,9 @@
 //
 // All three are per-pixel maps over w*h fp32 values; each thread owns one
 // pixel (blocks of 64 x 4: one wave per row segment, so every read of a
-// window row is coalesced and the window's rows stay in L1).  The filters
-// read their clamp-to-edge window straight from global memory.
+// window row is coalesced and the window's rows stay in L1).  The median and
+// the bilateral filter above radius 7 read their clamp-to-edge window straight
+// from global memory; the bilateral filter up to radius 7 stages it in LDS.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -52,13 +53,66 @@ bilateral_kernel(const float *__restrict__ in, float *__restrict__ out, int w, i
     out[(size_t)y * w + x] = weight_sum > 1e-8f ? sum / weight_sum : center;
 }
 
+// bilateral_kernel for radius R <= 7 (the default is 3): the block's
+// (4 + 2R) x (64 + 2R) window tile, clamp-to-edge, is read once into LDS, and
+// the spatial weights exp(-(dx^2+dy^2) * inv_2ss), which depend only on the
+// offset, are computed once per block into an LDS table with the same
+// expression.  Each pixel then reads its taps from LDS at constant offsets and
+// evaluates one exp per tap, accumulating in the same order: the results are
+// bit-identical to bilateral_kernel's.
+template <int R>
+__global__ void __launch_bounds__(256)
+bilateral_tile_kernel(const float *__restrict__ in, float *__restrict__ out, int w, int h,
+                      float inv_2ss, float inv_2sd) {
+    constexpr int S = 2 * R + 1, TW = 64 + 2 * R, TH = 4 + 2 * R;
+    __shared__ float tile[TH * TW];
+    __shared__ float spatial[S * S];
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, t = ty * 64 + tx;
+    const int x0 = (int)blockIdx.x * 64 - R, y0 = (int)blockIdx.y * 4 - R;
+    for (int i = t; i < TH * TW; i += 256) {
+        const int r = i / TW, c = i - r * TW;
+        tile[i] = in[(size_t)clampi(y0 + r, 0, h - 1) * w + clampi(x0 + c, 0, w - 1)];
+    }
+    for (int i = t; i < S * S; i += 256) {
+        const int dy = i / S - R, dx = i % S - R;
+        const float sd2 = (float)(dx * dx + dy * dy);
+        spatial[i] = expf(-sd2 * inv_2ss);
+    }
+    __syncthreads();
+    const int x = (int)blockIdx.x * 64 + tx, y = (int)blockIdx.y * 4 + ty;
+    if (x >= w || y >= h) return;
+    const float *win = tile + ty * TW + tx;   // the window's top-left tap
+    const float center = win[R * TW + R];
+    if (center <= 0.0f) {
+        out[(size_t)y * w + x] = 0.0f;
+        return;
+    }
+    float sum = 0.0f, weight_sum = 0.0f;
+#pragma unroll
+    for (int dy = 0; dy < S; ++dy) {
+#pragma unroll
+        for (int dx = 0; dx < S; ++dx) {
+            const float n = win[dy * TW + dx];
+            if (n <= 0.0f) continue;
+            const float ws = spatial[dy * S + dx];
+            const float diff = n - center;
+            const float wd = expf(-diff * diff * inv_2sd);
+            const float wgt = ws * wd;
+            sum += n * wgt;
+            weight_sum += wgt;
+        }
+    }
+    out[(size_t)y * w + x] = weight_sum > 1e-8f ? sum / weight_sum : center;
+}
+
 // The median of DepthFilter.metal:66-120: the values > 0 of the window (the
 // first 121 in scan order), sorted ascending, the one at index count / 2; 0
-// when there is none.  Sorting is replaced by rank selection: v is the
-// answer iff #(values < v) <= count / 2 < #(values <= v), which picks the
-// same value.  Radius R <= 2 (at most 25 values, so the cap never binds):
-// the window lives in registers, an invalid entry as +inf, which sorts after
-// every valid value and so never changes the rank of one below it.
+// when there is none.  Radius R <= 2 (at most 25 values, so the cap never
+// binds): the window lives in registers, an invalid entry as +inf, which
+// sorts after every valid value; a sorting network orders the window and the
+// entry at count / 2 is the answer.  Larger radii (median_kernel) replace the
+// sort by rank selection: v is the answer iff #(values < v) <= count / 2 <
+// #(values <= v), which picks the same value.
 template <int R>
 __global__ void __launch_bounds__(256)
 median_small_kernel(const float *__restrict__ in, float *__restrict__ out, int w, int h) {
@@ -79,18 +133,21 @@ median_small_kernel(const float *__restrict__ in, float *__restrict__ out, int w
             count += ok ? 1 : 0;
         }
     }
-    const int k = count / 2;
-    float res = 0.0f;
+    // Odd-even transposition sort (N rounds sort N values; every value is
+    // positive or +inf, so min / max are exact), then the entry at count / 2.
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        int lt = 0, le = 0;
+    for (int r = 0; r < N; ++r) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-            lt += v[j] < v[i] ? 1 : 0;
-            le += v[j] <= v[i] ? 1 : 0;
+        for (int i = r & 1; i + 1 < N; i += 2) {
+            const float a = v[i], b = v[i + 1];
+            v[i] = fminf(a, b);
+            v[i + 1] = fmaxf(a, b);
         }
-        if (lt <= k && k < le) res = v[i];
     }
+    const int k = count / 2;
+    float res = v[0];
+#pragma unroll
+    for (int i = 1; i <= N / 2; ++i) res = k == i ? v[i] : res;
     out[(size_t)y * w + x] = count > 0 ? res : 0.0f;
 }
 
@@ -212,8 +269,19 @@ vxh_status vxh_depth_bilateral_enqueue(const float *d_in, float *d_out, uint32_t
     // The shader's per-thread constants, in fp32 (DepthFilter.metal:33-34).
     const float inv_2ss = 1.0f / (2.0f * cfg->sigma_spatial * cfg->sigma_spatial);
     const float inv_2sd = 1.0f / (2.0f * cfg->sigma_depth * cfg->sigma_depth);
-    hipLaunchKernelGGL(bilateral_kernel, grid_of(w, h), dim3(64, 4), 0, (hipStream_t)stream,
-                       d_in, d_out, (int)w, (int)h, (int)cfg->radius, inv_2ss, inv_2sd);
+    const hipStream_t s = (hipStream_t)stream;
+    switch (cfg->radius) {
+#define VXH_BIL(RR)                                                                        \
+    case RR:                                                                               \
+        hipLaunchKernelGGL(bilateral_tile_kernel<RR>, grid_of(w, h), dim3(64, 4), 0, s, d_in, \
+                           d_out, (int)w, (int)h, inv_2ss, inv_2sd);                       \
+        break;
+        VXH_BIL(1) VXH_BIL(2) VXH_BIL(3) VXH_BIL(4) VXH_BIL(5) VXH_BIL(6) VXH_BIL(7)
+#undef VXH_BIL
+    default:
+        hipLaunchKernelGGL(bilateral_kernel, grid_of(w, h), dim3(64, 4), 0, s, d_in, d_out,
+                           (int)w, (int)h, (int)cfg->radius, inv_2ss, inv_2sd);
+    }
     return launched("depth_bilateral");
 }
 

@@ -281,7 +281,9 @@ def next_rows_leg(vx, hip, np, args):
     depth, the median and bilateral depth filters (vx-vision's DepthFilter
     bench sizes: 640x480, radius 1 median, radius 3 bilateral), the gray8
     remap through rectify maps and the map builder, at 640x480 and 1280x720;
-    and the sparse ORB stereo match at 512 x 512 keypoints.  Per kernel: us per
+    SGM and the three depth stages as one relaunched pipeline against the same
+    stages called one by one; and the sparse ORB stereo match at 512 x 512
+    keypoints.  Per kernel: us per
     launch, the bytes it must move (one read of its inputs, one write of its
     outputs) and that rate against 8 TB/s.  Outputs are not checked here (the
     GPU tests compare every one of these kernels with the oracle)."""
@@ -331,6 +333,63 @@ def next_rows_leg(vx, hip, np, args):
             out[f"{name}_{w}x{h}"] = {"us": round(us, 2), "bytes": nbytes,
                                        "gbs": round(nbytes / us / 1e3, 1),
                                        "frac": round(nbytes / us / 1e3 / 8000.0, 4)}
+    # Row f2: SGM -> 3x3 median -> 7x7 bilateral -> depth on the headline pair,
+    # encoded once into a pipeline (one graph) and relaunched, against the
+    # same four stages enqueued call by call on one stream.
+    w, h, D, paths, _, _, _ = vx.WORKLOADS[args.workload]
+    left, right, _, cfg = vx.workload_pair(args.workload)
+    ctx = vx.Context(0)
+    sgm = vx.SGMStereo(ctx)
+    filt = vx.DepthFilter(ctx)
+    tl, tr = ctx.texture_gray8(left, w, h), ctx.texture_gray8(right, w, h)
+    disp, med, bil, dep = (ctx.texture_output_r32float(w, h) for _ in range(4))
+    medc, bilc = vx.DepthMedianConfig(1), vx.DepthFilterConfig(3, 5.0, 0.05)
+    pipe = vx.Pipeline.begin(ctx)
+    sgm.encode(pipe, tl, tr, disp, cfg)
+    filt.encode_median(pipe, disp, med, medc)
+    filt.encode_bilateral(pipe, med, bil, bilc)
+    vx.disparity_to_depth(bil.device_ptr, dep.device_ptr, w, h, 700.0, 0.12,
+                          stream=pipe.stream)
+    pipe.commit()
+    pipe.wait()
+    graph_ms = []
+    for _ in range(reps // 5 + 3):
+        t0 = time.perf_counter()
+        pipe.relaunch()
+        pipe.wait()
+        graph_ms.append((time.perf_counter() - t0) * 1e3)
+    wsb = vx.workspace_bytes(w, h, cfg)
+    ws = hip.DeviceBuffer(wsb)
+    d_l, d_r = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
+    d_a, d_b = hip.DeviceBuffer(w * h * 4), hip.DeviceBuffer(w * h * 4)
+    s = stream.handle
+
+    def calls():
+        sgm.enqueue(d_l.ptr, d_r.ptr, w, h, cfg, d_a.ptr, ws.ptr, wsb, s)
+        vx.DepthFilter.enqueue_median(d_a.ptr, d_b.ptr, w, h, medc, s)
+        vx.DepthFilter.enqueue_bilateral(d_b.ptr, d_a.ptr, w, h, bilc, s)
+        vx.disparity_to_depth(d_a.ptr, d_b.ptr, w, h, 700.0, 0.12, s)
+
+    calls()
+    stream.synchronize()
+    call_ms = []
+    for _ in range(reps // 5 + 3):
+        t0 = time.perf_counter()
+        calls()
+        stream.synchronize()
+        call_ms.append((time.perf_counter() - t0) * 1e3)
+    out["pipeline_sgm_median_bilateral_depth"] = {
+        "workload": args.workload, "unit": "ms per frame (host wall clock, launch to wait)",
+        "graph_relaunch_ms": round(float(np.median(graph_ms)), 4),
+        "stream_calls_ms": round(float(np.median(call_ms)), 4),
+        "graph_nodes": pipe.node_count()}
+    pipe.close()
+    for t in (tl, tr, disp, med, bil, dep):
+        t.close()
+    filt.close()
+    sgm.close()
+    ctx.close()
+
     nk = 512
     kp = np.zeros(nk, vx.CORNER_DTYPE)
     kp["x"] = rng.uniform(0, 640, nk)

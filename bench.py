@@ -1,6 +1,7 @@
 """SGM stereo benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms plain,batched,stream,served,next]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--forms plain,batched,stream,served,next,config2]
 
 Headline (`value`): the plain per-pair path.  A step is one pass of the SGM
 pipeline (census of both images, matching costs, path aggregation over every
@@ -57,7 +58,7 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle threads of the all-cores CPU baseline (0 = every CPU this "
                          "process may run on)")
-    ap.add_argument("--forms", default="plain,batched,stream,served,next")
+    ap.add_argument("--forms", default="plain,batched,stream,served,next,config2")
     ap.add_argument("--next-reps", type=int, default=50,
                     help="launches timed per kernel of the next-rows leg")
     ap.add_argument("--stream-pairs", type=int, default=8,
@@ -273,6 +274,64 @@ def golden_hashes(name):
 def map_sha256(a):
     import hashlib
     return hashlib.sha256(a.view("uint32").tobytes()).hexdigest()
+
+
+def config2_leg(vx, hip, np, args, tuner):
+    """The plain per-pair path on 1920x1080, D=256, 8 paths (the workload whose
+    cost volume, 531 MB, exceeds the 256 MB Infinity Cache), timed like the
+    headline over min(--steps, 20) steps with the stage marks, its map compared
+    with the committed SHA-256 of the oracle's map for the same pair."""
+    wl = "1080p_d256_8path"
+    w, h, D, paths, _, _, _ = vx.WORKLOADS[wl]
+    left, right, _, cfg = vx.workload_pair(wl)
+    ctx = vx.Context(0)
+    sgm = vx.SGMStereo(ctx)
+    if tuner is not None:
+        sched = tuner.lookup("plain", w, h, D, paths)
+        if sched:
+            sgm.set_schedule(D, sched)
+    d_l, d_r = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
+    d_o = hip.DeviceBuffer(w * h * 4)
+    wsb = vx.workspace_bytes(w, h, cfg)
+    ws = hip.DeviceBuffer(wsb)
+    stream = hip.Stream()
+    ev = [hip.Event() for _ in range(3)]
+    e0 = hip.Event()
+    for _ in range(2):
+        sgm.enqueue(d_l.ptr, d_r.ptr, w, h, cfg, d_o.ptr, ws.ptr, wsb, stream.handle)
+    stream.synchronize()
+    steps = min(args.steps, 20)
+    stage = {"census_and_cost": [], "aggregate_all_paths": [], "sum_wta": [], "total": []}
+    for _ in range(steps):
+        e0.record(stream)
+        sgm.enqueue_marked(d_l.ptr, d_r.ptr, w, h, cfg, d_o.ptr, ws.ptr, wsb, stream.handle, ev)
+        ev[2].synchronize()
+        stage["census_and_cost"].append(e0.elapsed_ms(ev[0]))
+        stage["aggregate_all_paths"].append(ev[0].elapsed_ms(ev[1]))
+        stage["sum_wta"].append(ev[1].elapsed_ms(ev[2]))
+        stage["total"].append(e0.elapsed_ms(ev[2]))
+    got = d_o.download(np.float32)
+    path = os.path.join(ROOT, "tests", "golden", "workload_hashes.json")
+    try:
+        with open(path) as f:
+            want = json.load(f)[wl]
+    except (OSError, KeyError):
+        want = None
+    ms = float(np.mean(stage["total"]))
+    agg_ms = float(np.mean(stage["aggregate_all_paths"]))
+    alg = w * h * (2 * D + 2)
+    out = {"workload": wl, "steps": steps, "ms_per_pair": round(ms, 4),
+           "value": round(w * h * D / (ms * 1e-3) / 1e9, 3), "unit": "Gpx*disp/s",
+           "stage_ms": {k: round(float(np.mean(v)), 4) for k, v in stage.items() if k != "total"},
+           "schedule": {"horizontal": list(sgm.schedule(D)[0]), "other": list(sgm.schedule(D)[1])},
+           "roofline": {"kernel": "paths_kernel (aggregation, all directions)",
+                        "alg_bytes_per_launch": alg,
+                        "achieved_gbs": round(alg / (agg_ms * 1e-3) / 1e9, 2),
+                        "frac": round(alg / (agg_ms * 1e-3) / 8e12, 4)},
+           "bit_exact_vs_oracle_hash": None if want is None else map_sha256(got) == want}
+    sgm.close()
+    ctx.close()
+    return out
 
 
 def next_rows_leg(vx, hip, np, args):
@@ -673,6 +732,9 @@ def main():
     if "next" in forms:
         log("next-rows leg (depth, filters, rectify, sparse stereo)")
         result["next_rows"] = next_rows_leg(vx, hip, np, args)
+    if "config2" in forms:
+        log("1920x1080 D=256 leg")
+        result["config2"] = config2_leg(vx, hip, np, args, tuner)
     print(json.dumps(result), flush=True)
 
 

@@ -26,14 +26,20 @@ scalar C++ oracle (the checker) on the same pair with the host's threads and
 confirms that the timed GPU map is bit-identical to it.
 
 `batched`: the 64-pair mixture of sizes {320x240, 640x480, 1280x720,
-1920x1080} x D {64, 128, 256}, 8 paths, as per-pair pipelines over several
-streams replayed from one hipGraph, against a loop of plain per-pair calls on
-one stream; all maps compared bit for bit.  `stream`: the headline workload as a stream of
+1920x1080} x D {64, 128, 256}, 8 paths, in the tuned batched form (persistent
+multi-pair launches, per-pair pipelines over several streams, or two sweeps)
+replayed from one hipGraph, against a loop of plain per-pair calls on one
+stream; all maps compared bit for bit and with the oracle's hashes.  `stream`: the headline workload as a stream of
 identical 720p pairs overlapped over 2 or 4 streams (one graph replay of 8 pairs), per-pair time
 and Gpx*disp/s, every map compared with a plain call; `value` stays the plain per-pair figure.
-`served`: Poisson arrivals at 200/500/1000 req/s of the same mixture
-against the serving loop and against a FIFO queue into the plain path;
-p50/p99 latency and Gpx*disp/s for each.
+`served`: Poisson arrivals at 200-2000 req/s of the same mixture against the
+serving loop and against a FIFO queue into the plain path; p50/p99 latency and
+Gpx*disp/s for each, and the rate each sustains within the p99 SLO.
+`next_rows`: the kernels around the path (depth conversion, median and
+bilateral filters, rectify maps and remap, sparse ORB stereo) and the
+SGM-to-depth pipeline relaunch, each timed with its bytes and rate.
+`config2`: the plain path on 1920x1080, D=256, 8 paths, checked against the
+oracle's hash.
 """
 import argparse
 import ctypes

@@ -496,3 +496,27 @@ def test_enqueue_reports_expired_split_wait(ctx):
         st.close()
     finally:
         s.close()
+
+
+# Seeded random configurations through the whole pipeline, bit for bit against
+# the oracle: size, D, penalties, census window, path count and LR check drawn
+# together (windows of at most 64 bits, D <= width).
+WINDOWS = [(4, 3), (3, 3), (3, 4), (1, 1), (2, 2), (0, 5), (5, 2), (31, 0)]
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_configs_match_oracle(sgm, seed):
+    rng = np.random.default_rng(0x5347_1000 + seed)
+    w, h = int(rng.integers(1, 300)), int(rng.integers(1, 48))
+    D = int(rng.integers(1, min(w, 256) + 1))
+    paths = int(rng.choice([2, 4, 8]))
+    rx, ry = WINDOWS[int(rng.integers(len(WINDOWS)))]
+    p1, p2 = float(rng.integers(0, 260)), float(rng.integers(0, 1500))
+    lr, tol = bool(rng.integers(0, 2)), int(rng.integers(0, 4))
+    left, right = random_pair(w, h, seed=1000 + seed)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, p1=p1, p2=p2, census_radius_x=rx,
+                                census_radius_y=ry, num_paths=paths, lr_check=lr,
+                                lr_tolerance=tol)
+    got = sgm.compute_host(left, right, w, h, cfg)
+    want = O.sgm(left, right, D, p1, p2, rx, ry, num_paths=paths, lr_check=lr, lr_tol=tol)
+    assert_same_disp(got, want)

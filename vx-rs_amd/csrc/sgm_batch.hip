@@ -424,7 +424,8 @@ hipError_t capture_persistent(vxh_sgm_batch *b) {
             const int D = (int)q.cfg.num_disparities;
             e = launch_lr_check((const uint16_t *)(base + L.sum), (int)q.width, (int)q.height, D,
                                 volume_stride(D), (int)q.cfg.lr_tolerance,
-                                (const int16_t *)(base + L.best), q.d_out, b->stream);
+                                (const int16_t *)(base + L.best), q.d_out,
+                                (int16_t *)(base + L.lc), b->stream);
             b->launches += 1;
         }
     }
@@ -659,7 +660,8 @@ hipError_t capture_sweep(vxh_sgm_batch *b) {
             const int D = (int)q.cfg.num_disparities;
             e = launch_lr_check((const uint16_t *)(base + L.sum), (int)q.width, (int)q.height, D,
                                 volume_stride(D), (int)q.cfg.lr_tolerance,
-                                (const int16_t *)(base + L.best), q.d_out, b->stream);
+                                (const int16_t *)(base + L.best), q.d_out,
+                                (int16_t *)(base + L.lc), b->stream);
             b->launches += 1;
         }
     }

@@ -191,31 +191,5 @@ __device__ __forceinline__ T *store_sink() {
     return (T *)&g_store_sink[(threadIdx.x & 63) * 2];
 }
 
-// Left-right check of pixel i: D_R(x') is the lowest d minimising
-// S(x'+d, y, d) over x'+d < w; the pixel is kept iff x - best >= 0 and
-// |best - D_R(x - best)| <= tol, otherwise it becomes -1.0f.
-__device__ __forceinline__ void lr_pixel(const uint16_t *__restrict__ vol, int w, int D,
-                                         int Dp, int tol, size_t i,
-                                         const int16_t *__restrict__ best,
-                                         float *__restrict__ out) {
-    const int x = (int)(i % w), y = (int)(i / w);
-    const int d = best[i];
-    const int xr = x - d;
-    bool ok = false;
-    if (xr >= 0) {
-        uint32_t bc = 0xFFFFu;
-        int bd = 0;
-        for (int k = 0; k < D && xr + k < w; ++k) {
-            const uint32_t c = vol[((size_t)y * w + xr + k) * Dp + k];
-            if (c < bc) {
-                bc = c;
-                bd = k;
-            }
-        }
-        ok = abs(d - bd) <= tol;
-    }
-    if (!ok) out[i] = -1.0f;
-}
-
 }  // namespace dev
 }  // namespace vxh

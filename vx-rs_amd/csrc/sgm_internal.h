@@ -339,8 +339,11 @@ hipError_t launch_wta_sum(const uint8_t *vol0, size_t vol_stride, const uint8_t 
                           const uint8_t *flags, int nvol, int mode, int npx, int D, int Dp,
                           float *out, int16_t *best, uint16_t *sum_out, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
+// Left-right check of a pair's map `out` from its summed u16 volume and
+// winners; `dr` holds w * h int16 of scratch (the right view's winners; the
+// census area of the workspace, free once the costs are computed).
 hipError_t launch_lr_check(const uint16_t *vol, int w, int h, int D, int Dp,
-                           int tol, const int16_t *best, float *out,
+                           int tol, const int16_t *best, float *out, int16_t *dr,
                            hipStream_t s);
 
 }  // namespace vxh

@@ -344,12 +344,90 @@ wta_multi_kernel(const WtaJob *__restrict__ jobs, int njobs) {
                   c.npx, c.D, c.Dp, c.out, c.best, c.sum_out);
 }
 
+// Left-right check in two passes.  lr_right_kernel finds D_R(xr) for every
+// right pixel xr of a row: the lowest k < D with xr + k < w minimising
+// S(xr + k, y, k), scanned upwards from 0xFFFF with d = 0 kept when nothing
+// is smaller (SGMStereo has no right view; this is the check as the build
+// defines it).  A block owns kLrT right pixels of one row and walks d in
+// chunks of kLrC: it stages the chunk's entries of the kLrT + kLrC - 1
+// pixels the anti-diagonals cross (each a contiguous run of the pixel's
+// slice) in LDS, rows padded to kLrRow entries so that the lanes' reads of
+// one anti-diagonal fall in different banks, and each lane scans its own.
+// lr_apply_kernel then keeps pixel x iff x - best >= 0 and
+// |best - D_R(x - best)| <= tol, else writes -1.0f.  Reading the volume once
+// per right pixel along anti-diagonals straight from memory would touch a
+// different cache line per entry.
+constexpr int kLrT = 256, kLrC = 64, kLrRow = kLrC + 2;
+
 __global__ void __launch_bounds__(256)
-lr_kernel(const uint16_t *__restrict__ vol, int w, int h, int D, int Dp, int tol,
-          const int16_t *__restrict__ best, float *__restrict__ out) {
+lr_right_kernel(const uint16_t *__restrict__ vol, int w, int D, int Dp,
+                int16_t *__restrict__ dr) {
+    __shared__ __align__(16) uint16_t tile[(kLrT + kLrC - 1) * kLrRow];
+    const int t = (int)threadIdx.x, y = (int)blockIdx.y, x0 = (int)blockIdx.x * kLrT;
+    const int xr = x0 + t;
+    const uint16_t *__restrict__ row = vol + (size_t)y * w * Dp;
+    uint32_t bc = 0xFFFFu;
+    int bd = 0;
+    for (int c0 = 0; c0 < D; c0 += kLrC) {
+        const int ne = min(kLrC, D - c0);                 // entries of this chunk
+        const int nw = (ne + 1) >> 1;                      // dwords (c0 + 2 nw <= Dp)
+        const int p0 = x0 + c0;                            // pixel of tile row 0
+        const int np = min(kLrT + kLrC - 1, w - p0);       // pixels inside the row
+        __syncthreads();                                   // the last chunk's reads are done
+        if (ne == kLrC) {
+            // Whole chunks: 16-byte loads, 8 per pixel (aligned: Dp and c0 are
+            // multiples of 16 entries), written as four dwords.
+            for (int i = t; i < np * (kLrC / 8); i += 256) {
+                const int p = i >> 3, q = i & 7;
+                const uint4 v = *(const uint4 *)(row + (size_t)(p0 + p) * Dp + c0 + 8 * q);
+                uint32_t *dst = (uint32_t *)(tile + p * kLrRow + 8 * q);
+                dst[0] = v.x;
+                dst[1] = v.y;
+                dst[2] = v.z;
+                dst[3] = v.w;
+            }
+        } else {
+            for (int i = t; i < np * nw; i += 256) {
+                const int p = i / nw, q = i - p * nw;
+                const uint32_t v = *(const uint32_t *)(row + (size_t)(p0 + p) * Dp + c0 + 2 * q);
+                *(uint32_t *)(tile + p * kLrRow + 2 * q) = v;
+            }
+        }
+        __syncthreads();
+        const int kmax = min(ne, w - xr - c0);             // k = c0 + dd valid for dd < kmax
+        const uint16_t *diag = tile + t * kLrRow;          // entry dd of pixel xr + c0 + dd
+        if (kmax == kLrC) {
+#pragma unroll 16
+            for (int dd = 0; dd < kLrC; ++dd) {
+                const uint32_t c = diag[dd * (kLrRow + 1)];
+                if (c < bc) {
+                    bc = c;
+                    bd = c0 + dd;
+                }
+            }
+        } else {
+            for (int dd = 0; dd < kmax; ++dd) {
+                const uint32_t c = diag[dd * (kLrRow + 1)];
+                if (c < bc) {
+                    bc = c;
+                    bd = c0 + dd;
+                }
+            }
+        }
+    }
+    if (xr < w) dr[(size_t)y * w + xr] = (int16_t)bd;
+}
+
+__global__ void __launch_bounds__(256)
+lr_apply_kernel(const int16_t *__restrict__ best, const int16_t *__restrict__ dr, int w, int h,
+                int tol, float *__restrict__ out) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)w * h) return;
-    lr_pixel(vol, w, D, Dp, tol, i, best, out);
+    const int x = (int)(i % w);
+    const int d = best[i];
+    const int xr = x - d;
+    const bool ok = xr >= 0 && abs(d - (int)dr[i - d]) <= tol;
+    if (!ok) out[i] = -1.0f;
 }
 
 __global__ void __launch_bounds__(256)
@@ -532,10 +610,13 @@ hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) 
 }
 
 hipError_t launch_lr_check(const uint16_t *vol, int w, int h, int D, int Dp, int tol,
-                           const int16_t *best, float *out, hipStream_t s) {
+                           const int16_t *best, float *out, int16_t *dr, hipStream_t s) {
+    if (w < 1 || h < 1 || D < 1 || D > Dp) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(lr_right_kernel, dim3((unsigned)((w + kLrT - 1) / kLrT), (unsigned)h),
+                       dim3(256), 0, s, vol, w, D, Dp, dr);
     const size_t n = (size_t)w * h;
-    hipLaunchKernelGGL(lr_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256),
-                       0, s, vol, w, h, D, Dp, tol, best, out);
+    hipLaunchKernelGGL(lr_apply_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, best,
+                       dr, w, h, tol, out);
     return hipGetLastError();
 }
 

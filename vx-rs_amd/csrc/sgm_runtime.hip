@@ -706,7 +706,7 @@ vxh_status run_pipeline(const uint8_t *d_left, const uint8_t *d_right, uint32_t 
     if (v != VXH_OK) return v;
     if (lr)
         VXH_HIP(launch_lr_check(sum, w, h, D, volume_stride(D), (int)c->lr_tolerance, best,
-                                d_out, st),
+                                d_out, (int16_t *)(base + L.lc), st),
                 "lr_check");
     if (marks) VXH_HIP(hipEventRecord(marks[2], st), "hipEventRecord");
     return VXH_OK;

@@ -264,17 +264,20 @@ vxh_status vxh_sgm_wta(const uint16_t *d_volume, uint32_t w, uint32_t h,
  * exactly `num_disparities` disparities: (lanes_h, per_lane_h) for the
  * horizontal directions, whose scanlines are the longest, and
  * (lanes, per_lane) for the others, each as for vxh_sgm_aggregate_layout;
- * all zero restores the default.  Results do not depend on the layouts, only
+ * all zero restores the default, which depends on the pair's size and path
+ * count (vxh_sgm_schedule_for).  Results do not depend on the layouts, only
  * the speed does.  No reference counterpart (the reference has one fixed
  * thread per scanline, vx-vision/src/kernels/sgm.rs:346-357). */
 vxh_status vxh_sgm_set_schedule(vxh_sgm *sgm, uint32_t num_disparities,
                                 uint32_t lanes_h, uint32_t per_lane_h,
                                 uint32_t lanes, uint32_t per_lane);
 /* Whether this handle splits scanlines into two halves aggregated at once,
- * for pairs with exactly `num_disparities` disparities: 0 never (the
- * default), 1 in the directions whose scanlines are longer than 2/3 of the
- * longest direction's, 2 in every direction; in either case only scanlines
- * of at least 256 pixels.  The second half
+ * for pairs with exactly `num_disparities` disparities: 0 never, 1 in the
+ * directions whose scanlines are longer than 2/3 of the longest direction's,
+ * 2 in every direction; in either case only scanlines of at least 256
+ * pixels.  0xFFFFFFFF restores the default, which depends on the pair's size
+ * and path count (level 1 for 2 and 4 paths up to D = 128, see
+ * vxh_sgm_schedule_for).  The second half
  * starts from the state every path starts from; the first half's wave then
  * continues into it until its state equals the second half's bit for bit
  * and overwrites the values before that point, so the result is identical
@@ -288,10 +291,17 @@ vxh_status vxh_sgm_set_split(vxh_sgm *sgm, uint32_t num_disparities, uint32_t sp
  * With 0 every split call reports the error: the way a caller exercises its
  * error path.  No reference counterpart. */
 vxh_status vxh_sgm_set_split_wait(vxh_sgm *sgm, uint32_t polls);
-/* The schedule in effect for `num_disparities`: out5 = lanes_h, per_lane_h,
- * lanes, per_lane, split. */
+/* The schedule in effect for 1280x720 8-path pairs with `num_disparities`
+ * disparities: out5 = lanes_h, per_lane_h, lanes, per_lane, split. */
 vxh_status vxh_sgm_get_schedule(vxh_sgm *sgm, uint32_t num_disparities,
                                 uint32_t *out5);
+/* The schedule vxh_sgm_compute_disparity and vxh_sgm_enqueue use for a
+ * w x h pair with `cfg`, as out5 above: what the handle was set to for
+ * cfg->num_disparities, else the default for that size and path count.
+ * vxh_sgm_enqueue_static uses the same layouts with split 0.  No reference
+ * counterpart. */
+vxh_status vxh_sgm_schedule_for(vxh_sgm *sgm, uint32_t w, uint32_t h,
+                                const vxh_sgm_config *cfg, uint32_t *out5);
 
 /* Storage of the per-direction path volumes for pairs with exactly
  * `num_disparities` disparities: 0 = automatic (u8 when 64 + P2 <= 255,

@@ -344,6 +344,49 @@ def test_schedule_setting_validated(ctx):
         s.close()
 
 
+def test_default_schedule_by_path_count(ctx):
+    """A lone pair's default schedule depends on its path count and size;
+    what the handle is set to overrides the layouts and the split level
+    separately, and None restores each default.  Maps stay the oracle's
+    under every default."""
+    s = vxsgm.SGMStereo(ctx)
+    cfg = lambda D, paths: vxsgm.SGMStereoConfig(num_disparities=D, num_paths=paths)
+    try:
+        assert s.schedule_for(1280, 720, cfg(128, 8)) == ((32, 4), (16, 8), 0)
+        assert s.schedule_for(1280, 720, cfg(128, 2)) == ((32, 4), (32, 4), 1)
+        assert s.schedule_for(1920, 1080, cfg(128, 4)) == ((16, 8), (16, 8), 1)
+        assert s.schedule_for(1920, 1080, cfg(256, 4)) == ((64, 4), (32, 8), 0)
+        assert s.schedule_for(1920, 1080, cfg(256, 2)) == ((32, 8), (32, 8), 1)
+        # D <= 64, 8 paths: split only where level 1 splits the longest passes alone.
+        assert s.schedule_for(1280, 720, cfg(64, 8)) == ((16, 4), (16, 4), 1)
+        assert s.schedule_for(320, 240, cfg(64, 8)) == ((16, 4), (16, 4), 1)
+        assert s.schedule_for(640, 480, cfg(64, 8)) == ((16, 4), (16, 4), 0)
+        assert s.schedule_for(480, 640, cfg(64, 8)) == ((16, 4), (16, 4), 0)
+        assert s.schedule_for(240, 400, cfg(64, 8)) == ((16, 4), (16, 4), 1)
+        # The 1280x720 8-path view of the query by D alone.
+        assert s.schedule(128) == ((32, 4), (16, 8)) and s.split(128) == 0
+        s.set_split(128, 0)
+        assert s.schedule_for(1280, 720, cfg(128, 2)) == ((32, 4), (32, 4), 0)
+        s.set_schedule(128, (64, 2))
+        assert s.schedule_for(1280, 720, cfg(128, 2)) == ((64, 2), (64, 2), 0)
+        s.set_split(128, None)
+        assert s.schedule_for(1280, 720, cfg(128, 2)) == ((64, 2), (64, 2), 1)
+        s.set_schedule(128, None)
+        assert s.schedule_for(1280, 720, cfg(128, 2)) == ((32, 4), (32, 4), 1)
+        with pytest.raises(vxsgm.InvalidConfig):
+            s.set_split(128, 3)
+        with pytest.raises(vxsgm.InvalidConfig):
+            s.schedule_for(1280, 720, cfg(300, 8))
+        for (w, h, D, paths) in [(600, 300, 64, 8), (330, 250, 64, 8), (520, 200, 128, 2),
+                                 (300, 520, 128, 4), (400, 260, 200, 2)]:
+            left, right, _ = vxsgm.synth_pair(w, h, D, 4, 1, 7100 + w + D + paths)
+            c = cfg(D, paths)
+            want = O.sgm(left, right, D, num_paths=paths)
+            assert_same_disp(s.compute_host(left, right, w, h, c), want)
+    finally:
+        s.close()
+
+
 @pytest.mark.parametrize("w,h,D", [(600, 80, 64), (110, 420, 100), (512, 96, 128), (300, 40, 256),
                                    (257, 30, 32), (96, 96, 16), (300, 290, 48), (270, 700, 96)])
 @pytest.mark.parametrize("fmt", ["u8", "u16"])

@@ -315,8 +315,11 @@ inline uint32_t cost_bias(uint32_t p1_u16) { return p1_u16 <= 191u ? p1_u16 : 0u
 hipError_t launch_or_word(uint32_t *dst, const uint32_t *src, hipStream_t s);
 // Zeroes n words with a kernel (safe inside captured graphs).
 hipError_t launch_zero(uint32_t *p, size_t n, hipStream_t s);
-// Default schedule for D disparities.
+// Default schedule for D disparities: the fastest for a lone 8-path pair,
+// and the per-pair schedule of the batch forms.
 Schedule default_schedule(int D);
+// Default schedule of a lone w x h pair with `paths` paths and D disparities.
+Schedule plain_schedule(int D, int paths, uint32_t w, uint32_t h);
 bool schedule_valid(Schedule sc, int D);
 // The aggregation launchers of layout classes 0, 1 and 2 (one translation
 // unit each): paths_split_kernel when `split`, else paths_kernel, in storage

@@ -446,6 +446,31 @@ Schedule default_schedule(int D) {
     return layout_class(D) == 2 ? Schedule{0, 1, 0} : Schedule{1, 2, 0};
 }
 
+// A lone pair's schedule by path count, from per-pair timings of every
+// layout pair and split level at 320x240 to 1920x1080 for each class:
+//  - 2 paths: one horizontal and one vertical pass leave most of the chip
+//    idle, and halving the longest chain of steps wins everywhere: the
+//    middle layout, split (10-30 % faster).
+//  - 4 paths: split again; the widest per-lane layout for D in (64, 128]
+//    (it leads at 1920x1080 and stays within 10 % elsewhere); D > 128
+//    keeps the 8-path schedule, which no split beat.
+//  - 8 paths: the 8-path schedule, except that D <= 64 uses the middle
+//    layout everywhere and splits when level 1 would split only the
+//    longest passes (320x240 and 1280x720: 4-10 % faster); where it would
+//    split the diagonals and the shorter axis too (480x360, 640x480) the
+//    doubled wave count costs up to 40 %.
+Schedule plain_schedule(int D, int paths, uint32_t w, uint32_t h) {
+    const int c = layout_class(D);
+    if (paths == 2) return Schedule{1, 1, 1};
+    if (paths == 4) return c == 2 ? Schedule{0, 1, 0} : Schedule{c == 1 ? 2 : 1, c == 1 ? 2 : 1, 1};
+    if (c == 0) {
+        const uint32_t lo = std::min(w, h), hi = std::max(w, h);
+        const bool longest_only = lo < (uint32_t)kSplitMin || 3 * lo <= 2 * hi;
+        return Schedule{1, 1, longest_only ? 1 : 0};
+    }
+    return default_schedule(D);
+}
+
 bool schedule_valid(Schedule sc, int D) {
     const int n = kClassCount[layout_class(D)];
     return sc.horiz >= 0 && sc.horiz < n && sc.other >= 0 && sc.other < n &&

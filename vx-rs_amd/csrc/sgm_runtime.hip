@@ -124,8 +124,9 @@ struct vxh_sgm {
     std::mutex mu;
     std::vector<Slot *> free_slots;
     std::vector<Slot *> all_slots;
-    vxh::Schedule sched[257] = {};   // per D, used when sched_set[D]
-    bool sched_set[257] = {};
+    vxh::Schedule sched[257] = {};   // per D: layouts used when layout_set[D],
+    bool layout_set[257] = {};       // the split level when split_set[D]
+    bool split_set[257] = {};
     uint8_t format[257] = {};        // per D: 0 auto, 1 u8, 2 u16
     std::atomic<uint32_t> wait_polls{vxh::kSplitWaitPolls};   // split heads' bounded wait
 };
@@ -248,9 +249,16 @@ static vxh_status copy_blocking(vxh_context *ctx, void *dst, const void *src, si
     return e == hipSuccess ? VXH_OK : hip_fail(e, "hipMemcpyWithStream");
 }
 
-static vxh::Schedule schedule_for(vxh_sgm *s, int D) {
+// The schedule of a w x h pair with D disparities and `paths` paths: what the
+// handle was set to, else the default for that pair.
+static vxh::Schedule schedule_for(vxh_sgm *s, int D, int paths, uint32_t w, uint32_t h) {
     std::lock_guard<std::mutex> g(s->mu);
-    vxh::Schedule sc = s->sched_set[D] ? s->sched[D] : vxh::default_schedule(D);
+    vxh::Schedule sc = vxh::plain_schedule(D, paths, w, h);
+    if (s->layout_set[D]) {
+        sc.horiz = s->sched[D].horiz;
+        sc.other = s->sched[D].other;
+    }
+    if (s->split_set[D]) sc.split = s->sched[D].split;
     sc.wait_polls = s->wait_polls.load();
     return sc;
 }
@@ -734,7 +742,7 @@ static vxh_status enqueue_impl(vxh_sgm *s, const uint8_t *d_left, const uint8_t 
         return fail(VXH_ERR_INVALID_CONFIG, buf);
     }
     return vxh::run_pipeline(d_left, d_right, w, h, c, d_out, (char *)d_ws, mode,
-                             schedule_for(s, D), st, marks);
+                             schedule_for(s, D, (int)c->num_paths, w, h), st, marks);
 }
 
 vxh_status vxh_sgm_enqueue(vxh_sgm *s, const uint8_t *d_left, const uint8_t *d_right,
@@ -774,7 +782,7 @@ vxh_status vxh_sgm_enqueue_static(vxh_sgm *s, const uint8_t *d_left, const uint8
     VXH_HIP(hipSetDevice(s->ctx->device), "hipSetDevice");
     void *ws = static_scratch(s->ctx->device);
     if (!ws) return fail(VXH_ERR_SHADER_MISSING, "shader missing: static scratch not resolved");
-    vxh::Schedule sc = schedule_for(s, (int)c->num_disparities);
+    vxh::Schedule sc = schedule_for(s, (int)c->num_disparities, (int)c->num_paths, w, h);
     sc.split = 0;
     return vxh::run_pipeline(d_left, d_right, w, h, c, d_out, (char *)ws, mode, sc,
                              (hipStream_t)stream);
@@ -877,7 +885,8 @@ vxh_status vxh_sgm_encode(vxh_sgm *s, vxh_pipeline *p, const vxh_texture *left,
     if ((v = vxh::pipeline_scratch(p, L.total, &ws)) != VXH_OK) return v;
     v = vxh::run_pipeline((const uint8_t *)left->dptr, (const uint8_t *)right->dptr, w, h, c,
                           (float *)out->dptr, (char *)ws, mode,
-                          schedule_for(s, (int)c->num_disparities), st);
+                          schedule_for(s, (int)c->num_disparities, (int)c->num_paths, w, h),
+                          st);
     if (v != VXH_OK) return v;
     // The split launch's error word (a head whose wait for its tail timed out).
     for (const vxh_texture *t : {left, right, (const vxh_texture *)out}) vxh::pipeline_use(p, t);
@@ -971,12 +980,9 @@ vxh_status vxh_sgm_set_schedule(vxh_sgm *s, uint32_t nd, uint32_t lanes_h,
                                 uint32_t per_lane_h, uint32_t lanes, uint32_t per_lane) {
     if (!s || nd < 1 || nd > 256)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: schedule target");
-    const vxh::Schedule cur = schedule_for(s, (int)nd);
     if (lanes_h == 0 && per_lane_h == 0 && lanes == 0 && per_lane == 0) {
-        const vxh::Schedule d = vxh::default_schedule((int)nd);
         std::lock_guard<std::mutex> g(s->mu);
-        s->sched[nd] = vxh::Schedule{d.horiz, d.other, cur.split};
-        s->sched_set[nd] = true;
+        s->layout_set[nd] = false;
         return VXH_OK;
     }
     const int ih = layout_index((int)nd, lanes_h, per_lane_h);
@@ -984,31 +990,44 @@ vxh_status vxh_sgm_set_schedule(vxh_sgm *s, uint32_t nd, uint32_t lanes_h,
     if (ih < 0 || io < 0)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: unsupported lane layout");
     std::lock_guard<std::mutex> g(s->mu);
-    s->sched[nd] = vxh::Schedule{ih, io, cur.split};
-    s->sched_set[nd] = true;
+    s->sched[nd].horiz = ih;
+    s->sched[nd].other = io;
+    s->layout_set[nd] = true;
     return VXH_OK;
 }
 
 vxh_status vxh_sgm_set_split(vxh_sgm *s, uint32_t nd, uint32_t split) {
-    if (!s || nd < 1 || nd > 256 || split > 2)
+    if (!s || nd < 1 || nd > 256 || (split > 2 && split != 0xFFFFFFFFu))
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: split setting");
-    const vxh::Schedule cur = schedule_for(s, (int)nd);
     std::lock_guard<std::mutex> g(s->mu);
-    s->sched[nd] = vxh::Schedule{cur.horiz, cur.other, (int)split};
-    s->sched_set[nd] = true;
+    s->split_set[nd] = split != 0xFFFFFFFFu;
+    if (s->split_set[nd]) s->sched[nd].split = (int)split;
     return VXH_OK;
 }
 
-vxh_status vxh_sgm_get_schedule(vxh_sgm *s, uint32_t nd, uint32_t *out4) {  // out4[0..5)
-    if (!s || nd < 1 || nd > 256 || !out4)
+static void put_schedule(const vxh::Schedule &sc, int D, uint32_t *out5) {
+    const int c = vxh::layout_class(D);
+    out5[0] = (uint32_t)vxh::kClassLayouts[c][sc.horiz][0];
+    out5[1] = (uint32_t)vxh::kClassLayouts[c][sc.horiz][1];
+    out5[2] = (uint32_t)vxh::kClassLayouts[c][sc.other][0];
+    out5[3] = (uint32_t)vxh::kClassLayouts[c][sc.other][1];
+    out5[4] = (uint32_t)sc.split;
+}
+
+vxh_status vxh_sgm_get_schedule(vxh_sgm *s, uint32_t nd, uint32_t *out5) {
+    if (!s || nd < 1 || nd > 256 || !out5)
         return fail(VXH_ERR_INVALID_CONFIG, "invalid config: schedule target");
-    const vxh::Schedule sc = schedule_for(s, (int)nd);
-    const int c = vxh::layout_class((int)nd);
-    out4[0] = (uint32_t)vxh::kClassLayouts[c][sc.horiz][0];
-    out4[1] = (uint32_t)vxh::kClassLayouts[c][sc.horiz][1];
-    out4[2] = (uint32_t)vxh::kClassLayouts[c][sc.other][0];
-    out4[3] = (uint32_t)vxh::kClassLayouts[c][sc.other][1];
-    out4[4] = (uint32_t)sc.split;
+    put_schedule(schedule_for(s, (int)nd, 8, 1280, 720), (int)nd, out5);
+    return VXH_OK;
+}
+
+vxh_status vxh_sgm_schedule_for(vxh_sgm *s, uint32_t w, uint32_t h, const vxh_sgm_config *c,
+                                uint32_t *out5) {
+    if (!s || !out5) return fail(VXH_ERR_INVALID_CONFIG, "invalid config: schedule target");
+    vxh_status v = vxh_sgm_config_validate(c, w, h);
+    if (v != VXH_OK) return v;
+    const int D = (int)c->num_disparities;
+    put_schedule(schedule_for(s, D, (int)c->num_paths, w, h), D, out5);
     return VXH_OK;
 }
 

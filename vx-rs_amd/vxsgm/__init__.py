@@ -111,6 +111,7 @@ _sig("vxh_sgm_wta", ctypes.c_int, _vp, _u32, _u32, _cfgp, _vp, _vp)
 _sig("vxh_sgm_volume_stride", _u32, _u32)
 _sig("vxh_sgm_set_schedule", ctypes.c_int, _vp, _u32, _u32, _u32, _u32, _u32)
 _sig("vxh_sgm_get_schedule", ctypes.c_int, _vp, _u32, ctypes.POINTER(_u32))
+_sig("vxh_sgm_schedule_for", ctypes.c_int, _vp, _u32, _u32, _cfgp, ctypes.POINTER(_u32))
 _sig("vxh_sgm_set_split", ctypes.c_int, _vp, _u32, _u32)
 _sig("vxh_sgm_set_volume_format", ctypes.c_int, _vp, _u32, _u32)
 _sig("vxh_util_copy", ctypes.c_int, _vp, _vp, _sz, _vp)
@@ -411,21 +412,33 @@ class SGMStereo:
         check(lib.vxh_sgm_set_schedule(self._h, num_disparities, h[0], h[1], o[0], o[1]))
 
     def schedule(self, num_disparities):
-        """((lanes_h, per_lane_h), (lanes, per_lane)) in effect for this D."""
+        """((lanes_h, per_lane_h), (lanes, per_lane)) in effect for this D
+        (for 1280x720 8-path pairs)."""
         out = (_u32 * 5)()
         check(lib.vxh_sgm_get_schedule(self._h, num_disparities, out))
         return (out[0], out[1]), (out[2], out[3])
 
+    def schedule_for(self, width, height, config):
+        """((lanes_h, per_lane_h), (lanes, per_lane), split) that a call on a
+        width x height pair with `config` uses."""
+        out = (_u32 * 5)()
+        c = config.to_c()
+        check(lib.vxh_sgm_schedule_for(self._h, width, height, ctypes.byref(c), out))
+        return (out[0], out[1]), (out[2], out[3]), int(out[4])
+
     def split(self, num_disparities):
-        """Split level for this D: 0 none, 1 the longest passes' scanlines run
-        as two halves at once, 2 every long pass's."""
+        """Split level for this D (for 1280x720 8-path pairs): 0 none, 1 the
+        longest passes' scanlines run as two halves at once, 2 every long
+        pass's."""
         out = (_u32 * 5)()
         check(lib.vxh_sgm_get_schedule(self._h, num_disparities, out))
         return int(out[4])
 
     def set_split(self, num_disparities, split):
-        """split: False/0, True/1 (the longest passes) or 2 (every long pass)."""
-        check(lib.vxh_sgm_set_split(self._h, num_disparities, int(split)))
+        """split: False/0, True/1 (the longest passes), 2 (every long pass)
+        or None (the default, which depends on size and path count)."""
+        level = 0xFFFFFFFF if split is None else int(split)
+        check(lib.vxh_sgm_set_split(self._h, num_disparities, level))
 
     def set_volume_format(self, num_disparities, fmt):
         """Per-direction volume storage for this D: "auto", "u8" or "u16"."""

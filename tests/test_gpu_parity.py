@@ -482,6 +482,23 @@ def test_720p_d128_8path_bit_exact(ctx, sgm):
         s.close()
 
 
+@pytest.mark.parametrize("workload,D", [("720p_d128_8path", 128), ("1080p_d256_8path", 256),
+                                        ("720p_d128_8path", 64)])
+def test_default_config_two_paths_full_size(sgm, workload, D):
+    """The reference's default configuration (2 paths, P1 10, P2 120, 9x7
+    census) at full size, whose default schedule splits the 1280- and
+    1920-pixel scanlines, against the threaded oracle, with and without the
+    LR check."""
+    left, right, _, _ = vxsgm.workload_pair(workload)
+    h, w = left.shape
+    for lr in (False, True):
+        cfg = vxsgm.SGMStereoConfig(num_disparities=D, lr_check=lr)
+        assert cfg.num_paths == 2
+        assert sgm.schedule_for(w, h, cfg)[2] == 1
+        want = O.sgm(left, right, D, num_paths=2, lr_check=lr, nthreads=16)
+        assert_same_disp(sgm.compute_host(left, right, w, h, cfg), want)
+
+
 def test_1080p_d256_8path_matches_golden_hash(ctx, sgm):
     """Workload 1080p_d256_8path: the GPU map's SHA-256 equals the oracle's,
     recorded in tests/golden/workload_hashes.json, in both volume formats."""

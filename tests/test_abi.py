@@ -119,20 +119,26 @@ def align256(v):
     return (v + 255) // 256 * 256
 
 
-@pytest.mark.parametrize("w,h,D,paths,p2", [(640, 480, 100, 2, 120.0), (1920, 1080, 128, 8, 120.0),
-                                            (320, 240, 64, 4, 400.0), (7, 5, 3, 8, 120.0)])
-def test_server_request_bytes_is_workspace(w, h, D, paths, p2):
+@pytest.mark.parametrize("w,h,D,paths,p2,lr", [(640, 480, 100, 2, 120.0, False),
+                                               (1920, 1080, 128, 8, 120.0, False),
+                                               (320, 240, 64, 4, 400.0, False),
+                                               (7, 5, 3, 8, 120.0, False),
+                                               (1280, 720, 128, 8, 120.0, True),
+                                               (33, 9, 40, 4, 300.0, True)])
+def test_server_request_bytes_is_workspace(w, h, D, paths, p2, lr):
     """A served request is admitted against its whole pipeline workspace: census
     of both images, the u8 matching-cost volume (Dp bytes per pixel), the
     exception flags (8 bytes per pixel), the split launch's synchronisation
     words (error flag, a ticket counter per XCD, two flags per scanline
     group), and per direction a u8
     volume plus its high-byte volume, or a u16 volume; the larger of the two
-    formats is reserved."""
-    cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=paths, p2=p2)
+    formats is reserved.  The LR check adds the integer winners (2 bytes per
+    pixel): its right-view keys reuse the census area, and no summed volume
+    is stored."""
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, num_paths=paths, p2=p2, lr_check=lr)
     n, Dp = w * h, (D + 15) // 16 * 16
     sync = align256(4 * (10 + 16 * ((w + h) // 2 + 1)))
-    common = 2 * align256(n * 8) + align256(n * Dp) + sync
+    common = 2 * align256(n * 8) + align256(n * Dp) + sync + (align256(n * 2) if lr else 0)
     u8 = common + align256(n * 8) + 2 * paths * align256(n * Dp)
     u16 = common + paths * align256(n * Dp * 2)
     assert vxsgm.workspace_bytes(w, h, cfg) == max(u8, u16)

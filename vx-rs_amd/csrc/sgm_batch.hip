@@ -116,6 +116,7 @@ struct Chunk {
         const vxh::WtaJob *wta;
         int nwta;
         uint32_t wta_blocks;
+        int wta_kinds;   // as for launch_wta_multi
     };
     std::vector<Group> groups;
     // Sweep form: per lane-layout class present, the forward and backward
@@ -280,13 +281,15 @@ hipError_t plan_persistent(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
             if (gi == groups.size()) groups.push_back(G{cls, p.mode, {}, {}, {}, 0});
             G &g = groups[gi];
             const bool lr = q.cfg.lr_check != 0;
+            // LR check: the right view's keys in the left census's area (the
+            // winner launch comes after the costs).
             WtaJob t{(const uint8_t *)(base + L.vol), L.vol_bytes,
                      u8 ? (const uint8_t *)(base + L.hi) : nullptr,
                      u8 ? (const uint2 *)(base + L.flags) : nullptr, q.d_out,
                      lr ? (int16_t *)(base + L.best) : nullptr,
-                     lr ? (uint16_t *)(base + L.sum) : nullptr, (int)q.cfg.num_paths, p.mode,
-                     w * h, D, Dp, g.wb};
-            g.wb += (uint32_t)((w * h + 15) / 16);
+                     lr ? (uint32_t *)(base + L.lc) : nullptr, (int)q.cfg.num_paths, p.mode,
+                     w * h, w, D, Dp, g.wb};
+            g.wb += wta_job_blocks(t);
             g.wta.push_back(t);
             PassParams pp{};
             pp.cost = (const uint8_t *)(base + L.cost);
@@ -333,7 +336,9 @@ hipError_t plan_persistent(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
         off[ci].census = stage(img, cj.data(), cj.size() * sizeof(CensusJob));
         off[ci].cost = stage(img, kj.data(), kj.size() * sizeof(CostJob));
         for (G &g : groups) {
-            Chunk::Group cg{g.cls, g.mode, {}, 0, nullptr, (int)g.wta.size(), g.wb};
+            int kinds = 0;
+            for (const WtaJob &t : g.wta) kinds |= t.rkey ? 2 : 1;
+            Chunk::Group cg{g.cls, g.mode, {}, 0, nullptr, (int)g.wta.size(), g.wb, kinds};
             std::vector<BatchItem> all;
             uint32_t most = 0;
             for (int xk = 0; xk < 8; ++xk) {
@@ -395,11 +400,21 @@ hipError_t capture_persistent(vxh_sgm_batch *b) {
         if (e == hipSuccess) e = launch_census_multi(c.census, c.ncensus, c.census_blocks, b->stream);
         if (e == hipSuccess) e = launch_cost_multi(c.cost, c.ncost, c.cost_blocks, b->stream);
         b->launches += 2;
+        // The right view's keys of the pairs with the LR check start at zero.
+        for (uint32_t i : c.pairs) {
+            const vxh_sgm_pair_desc &q = b->pairs[i];
+            if (!q.cfg.lr_check || e != hipSuccess) continue;
+            const WsLayout L = workspace_layout(q.width, q.height, q.cfg.num_disparities,
+                                                (int)q.cfg.num_paths, b->plan[i].mode, true);
+            char *base = (char *)b->ws[0] + b->plan[i].ws_off;
+            e = launch_zero((uint32_t *)(base + L.lc), (size_t)q.width * q.height, b->stream);
+            b->launches += 1;
+        }
         if (e == hipSuccess && c.groups.size() == 1) {
             const Chunk::Group &g = c.groups[0];
             e = launch_class(g.cls, g.mode, g.args, g.blocks, b->stream);
-            if (e == hipSuccess) e = launch_wta_multi(g.wta, g.nwta, g.wta_blocks, b->stream);
-            b->launches += 2;
+            if (e == hipSuccess) e = launch_wta_multi(g.wta, g.nwta, g.wta_blocks, g.wta_kinds, b->stream);
+            b->launches += 1 + (g.wta_kinds == 3 ? 2 : 1);
         } else if (e == hipSuccess) {
             // Groups side by side: fork onto the sub streams, join back.
             e = hipEventRecord(b->events[0], b->stream);
@@ -408,10 +423,10 @@ hipError_t capture_persistent(vxh_sgm_batch *b) {
                 e = hipStreamWaitEvent(s, b->events[0], 0);
                 const Chunk::Group &cg = c.groups[g];
                 if (e == hipSuccess) e = launch_class(cg.cls, cg.mode, cg.args, cg.blocks, s);
-                if (e == hipSuccess) e = launch_wta_multi(cg.wta, cg.nwta, cg.wta_blocks, s);
+                if (e == hipSuccess) e = launch_wta_multi(cg.wta, cg.nwta, cg.wta_blocks, cg.wta_kinds, s);
                 if (e == hipSuccess) e = hipEventRecord(b->events[1 + g], s);
                 if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->events[1 + g], 0);
-                b->launches += 2;
+                b->launches += 1 + (cg.wta_kinds == 3 ? 2 : 1);
             }
         }
         for (uint32_t i : c.pairs) {
@@ -421,11 +436,9 @@ hipError_t capture_persistent(vxh_sgm_batch *b) {
             const WsLayout L = workspace_layout(q.width, q.height, q.cfg.num_disparities,
                                                 (int)q.cfg.num_paths, p.mode, true);
             char *base = (char *)b->ws[0] + p.ws_off;
-            const int D = (int)q.cfg.num_disparities;
-            e = launch_lr_check((const uint16_t *)(base + L.sum), (int)q.width, (int)q.height, D,
-                                volume_stride(D), (int)q.cfg.lr_tolerance,
-                                (const int16_t *)(base + L.best), q.d_out,
-                                (int16_t *)(base + L.lc), b->stream);
+            e = launch_lr_apply((const int16_t *)(base + L.best), (const uint32_t *)(base + L.lc),
+                                (int)q.width, (int)q.height, (int)q.cfg.lr_tolerance, q.d_out,
+                                b->stream);
             b->launches += 1;
         }
     }
@@ -688,7 +701,8 @@ vxh_status capture_streams(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
         if (st != VXH_OK) break;
         e = launch_or_word(b->d_error, (const uint32_t *)(ws + kErrorWordOffset),
                            b->sub[p.stream]);
-        b->launches += 6 + (q.cfg.lr_check ? 1 : 0);
+        // census, costs, paths, winners, error word; LR: key clear and check.
+        b->launches += 5 + (q.cfg.lr_check ? 2 : 0);
     }
     for (uint32_t s = 0; s < streams && e == hipSuccess && st == VXH_OK; ++s) {
         e = hipEventRecord(b->events[1 + s], b->sub[s]);

@@ -98,9 +98,10 @@ inline size_t ws_align(size_t v) { return (v + kWsAlign - 1) & ~(kWsAlign - 1); 
 // matching-cost volume (u8, pix*Dp + d), the exception flags (u8 volumes
 // only, 8 bytes per pixel), one volume per path direction (u8 or u16
 // entries, vol_bytes apart), the high-byte volumes (u8 only, also vol_bytes
-// apart), and for the LR check the summed u16 volume and the winner index.
+// apart), and for the LR check the winner index (the right view's keys go
+// to the left census's area, free by then).
 struct WsLayout {
-    size_t lc, rc, cost, flags, sync, vol, vol_bytes, hi, sum, best, total;
+    size_t lc, rc, cost, flags, sync, vol, vol_bytes, hi, best, total;
 };
 constexpr size_t kErrorWordOffset = 4;
 inline WsLayout workspace_layout(uint32_t w, uint32_t h, uint32_t D, int paths, int mode,
@@ -117,8 +118,7 @@ inline WsLayout workspace_layout(uint32_t w, uint32_t h, uint32_t D, int paths, 
     l.vol = l.flags + (u8 ? ws_align(n * 8) : 0);
     l.vol_bytes = ws_align(n * Dp * (u8 ? 1 : 2));
     l.hi = l.vol + (size_t)paths * l.vol_bytes;
-    l.sum = l.hi + (u8 ? (size_t)paths * l.vol_bytes : 0);
-    l.best = l.sum + (lr ? ws_align(n * Dp * 2) : 0);
+    l.best = l.hi + (u8 ? (size_t)paths * l.vol_bytes : 0);
     l.total = l.best + (lr ? ws_align(n * 2) : 0);
     return l;
 }
@@ -180,21 +180,26 @@ struct CostJob {     // one pair: tiles of 256 pixels of one row, nbx per row
     int nbx;
     uint32_t first_block;
 };
-struct WtaJob {      // one pair, as launch_wta_sum: 16 pixels per block
+struct WtaJob {      // one pair, as launch_wta_sum (16 pixels per block) or, with
+                     // rkey, as launch_wta_lr (wta_job_blocks blocks)
     const uint8_t *vol0;
     size_t vstride;
     const uint8_t *hi0;
     const uint2 *flags;
     float *out;
     int16_t *best;
-    uint16_t *sum_out;
-    int nv, mode, npx, D, Dp;
+    uint32_t *rkey;
+    int nv, mode, npx, w, D, Dp;
     uint32_t first_block;
 };
+// Blocks of a winner job.
+uint32_t wta_job_blocks(const WtaJob &j);
 hipError_t launch_census_multi(const CensusJob *d_jobs, int njobs, uint32_t blocks,
                                hipStream_t s);
 hipError_t launch_cost_multi(const CostJob *d_jobs, int njobs, uint32_t blocks, hipStream_t s);
-hipError_t launch_wta_multi(const WtaJob *d_jobs, int njobs, uint32_t blocks, hipStream_t s);
+// kinds: 1 = some jobs without right-view keys, 2 = some with, 3 = both.
+hipError_t launch_wta_multi(const WtaJob *d_jobs, int njobs, uint32_t blocks, int kinds,
+                            hipStream_t s);
 
 // Persistent aggregation over the direction passes of many pairs.  A pass is
 // one direction of one pair (p carries its volume, direction and scanline
@@ -341,6 +346,14 @@ hipError_t launch_paths(const PassParams &base, const int *dirs, uint8_t *const 
 hipError_t launch_wta_sum(const uint8_t *vol0, size_t vol_stride, const uint8_t *hi0,
                           const uint8_t *flags, int nvol, int mode, int npx, int D, int Dp,
                           float *out, int16_t *best, uint16_t *sum_out, hipStream_t s);
+// launch_wta_sum for a pair with the LR check: the integer winners go to
+// `best` (w * h) and the right view's winners, as keys, to `rkey` (w * h
+// words, zeroed beforehand); launch_lr_apply then checks `out` against them.
+hipError_t launch_wta_lr(const uint8_t *vol0, size_t vol_stride, const uint8_t *hi0,
+                         const uint8_t *flags, int nvol, int mode, int w, int h, int D, int Dp,
+                         float *out, int16_t *best, uint32_t *rkey, hipStream_t s);
+hipError_t launch_lr_apply(const int16_t *best, const uint32_t *rkey, int w, int h, int tol,
+                           float *out, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 // Left-right check of a pair's map `out` from its summed u16 volume and
 // winners; `dr` holds w * h int16 of scratch (the right view's winners; the

@@ -334,14 +334,48 @@ wta_sum_kernel(const uint8_t *__restrict__ vol0, size_t vstride,
                   sum_out);
 }
 
-// Sum + winner-take-all of many pairs in one launch (16 pixels per block).
+// Sum + winner with the right view's keys for the left-right check: block
+// (bx, y) takes kWtaLrP pixels of row y from bx * kWtaLrP (wta_lr_tile).
+template <int NV, int NC, int MODE>
+__global__ void __launch_bounds__(256)
+wta_lr_kernel(const uint8_t *__restrict__ vol0, size_t vstride, const uint8_t *__restrict__ hi0,
+              const uint2 *__restrict__ flags, int w, int h, int D, int Dp,
+              float *__restrict__ out, int16_t *__restrict__ best, uint32_t *__restrict__ rkey) {
+    __shared__ uint32_t lkey[kWtaLrKeys];
+    wta_lr_tile<MODE>(vol0, vstride, hi0, flags, NV, NC, w, h, (int)blockIdx.y,
+                      (int)blockIdx.x * kWtaLrP, D, Dp, out, best, rkey, lkey);
+}
+
+// Sum + winner-take-all of many pairs in one launch: 16 pixels per block, or
+// for a pair with right-view keys (the LR check) kWtaLrP pixels of one row
+// per block as wta_lr_kernel.  Instantiated per kind (LR) so that the plain
+// pairs keep the plain kernel's registers; each instantiation skips the
+// blocks of the other kind's pairs.
+template <bool LR>
 __global__ void __launch_bounds__(256)
 wta_multi_kernel(const WtaJob *__restrict__ jobs, int njobs) {
     const int j = find_entry(jobs, njobs, blockIdx.x);
     const WtaJob &c = jobs[j];
-    const int pix = (int)((blockIdx.x - c.first_block) * 16 + (threadIdx.x >> 4));
-    wta_sum_pixel(c.vol0, c.vstride, c.hi0, c.flags, c.nv, c.Dp > 128 ? 2 : 1, c.mode, pix,
-                  c.npx, c.D, c.Dp, c.out, c.best, c.sum_out);
+    if ((c.rkey != nullptr) != LR) return;
+    const int local = (int)(blockIdx.x - c.first_block);
+    const int nc = c.Dp > 128 ? 2 : 1;
+    if constexpr (LR) {
+        __shared__ uint32_t lkey[kWtaLrKeys];
+        const int nbx = (c.w + kWtaLrP - 1) / kWtaLrP;
+        const int h = c.npx / c.w;
+        if (c.mode == kOutU16)
+            wta_lr_tile<kOutU16>(c.vol0, c.vstride, c.hi0, c.flags, c.nv, nc, c.w, h,
+                                 local / nbx, (local % nbx) * kWtaLrP, c.D, c.Dp, c.out, c.best,
+                                 c.rkey, lkey);
+        else
+            wta_lr_tile<kOutU8>(c.vol0, c.vstride, c.hi0, c.flags, c.nv, nc, c.w, h,
+                                local / nbx, (local % nbx) * kWtaLrP, c.D, c.Dp, c.out, c.best,
+                                c.rkey, lkey);
+    } else {
+        const int pix = local * 16 + (int)(threadIdx.x >> 4);
+        wta_sum_pixel(c.vol0, c.vstride, c.hi0, c.flags, c.nv, nc, c.mode, pix, c.npx, c.D,
+                      c.Dp, c.out, c.best, nullptr);
+    }
 }
 
 // Left-right check in two passes.  lr_right_kernel finds D_R(xr) for every
@@ -416,6 +450,20 @@ lr_right_kernel(const uint16_t *__restrict__ vol, int w, int D, int Dp,
         }
     }
     if (xr < w) dr[(size_t)y * w + xr] = (int16_t)bd;
+}
+
+// The check from the right view's keys (lr_right_of_key): keeps pixel x iff
+// x - best >= 0 and |best - D_R(x - best)| <= tol, else writes -1.0f.
+__global__ void __launch_bounds__(256)
+lr_apply_keys_kernel(const int16_t *__restrict__ best, const uint32_t *__restrict__ rkey, int w,
+                     int h, int tol, float *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)w * h) return;
+    const int x = (int)(i % w);
+    const int d = best[i];
+    const int xr = x - d;
+    const bool ok = xr >= 0 && abs(d - (int)lr_right_of_key(rkey[i - d])) <= tol;
+    if (!ok) out[i] = -1.0f;
 }
 
 __global__ void __launch_bounds__(256)
@@ -515,9 +563,13 @@ hipError_t launch_cost_multi(const CostJob *d_jobs, int njobs, uint32_t blocks, 
     return hipGetLastError();
 }
 
-hipError_t launch_wta_multi(const WtaJob *d_jobs, int njobs, uint32_t blocks, hipStream_t s) {
-    if (njobs < 1 || blocks < 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(wta_multi_kernel, dim3(blocks), dim3(256), 0, s, d_jobs, njobs);
+hipError_t launch_wta_multi(const WtaJob *d_jobs, int njobs, uint32_t blocks, int kinds,
+                            hipStream_t s) {
+    if (njobs < 1 || blocks < 1 || kinds < 1 || kinds > 3) return hipErrorInvalidValue;
+    if (kinds & 1)
+        hipLaunchKernelGGL(wta_multi_kernel<false>, dim3(blocks), dim3(256), 0, s, d_jobs, njobs);
+    if (kinds & 2)
+        hipLaunchKernelGGL(wta_multi_kernel<true>, dim3(blocks), dim3(256), 0, s, d_jobs, njobs);
     return hipGetLastError();
 }
 
@@ -624,6 +676,43 @@ hipError_t launch_wta_sum(const uint8_t *vol0, size_t vstride, const uint8_t *hi
     VXH_WTA_CASE(8, 1, kOutU8) VXH_WTA_CASE(8, 2, kOutU8)
 #undef VXH_WTA_CASE
     return hipErrorInvalidValue;
+}
+
+uint32_t wta_job_blocks(const WtaJob &j) {
+    if (j.rkey) return (uint32_t)(((j.w + dev::kWtaLrP - 1) / dev::kWtaLrP) * (j.npx / j.w));
+    return (uint32_t)((j.npx + 15) / 16);
+}
+
+hipError_t launch_wta_lr(const uint8_t *vol0, size_t vstride, const uint8_t *hi0,
+                         const uint8_t *flags, int nvol, int mode, int w, int h, int D, int Dp,
+                         float *out, int16_t *best, uint32_t *rkey, hipStream_t s) {
+    if (Dp > 256 || Dp % 16 || w < 1 || h < 1 || D < 1 || D > Dp || !best || !rkey)
+        return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((w + kWtaLrP - 1) / kWtaLrP), (unsigned)h);
+#define VXH_WTA_LR_CASE(NV, NC, MM)                                                          \
+    if (mode == MM && nvol == NV && (Dp > 128) == (NC == 2)) {                                \
+        hipLaunchKernelGGL((wta_lr_kernel<NV, NC, MM>), grid, dim3(256), 0, s, vol0, vstride, \
+                           hi0, (const uint2 *)flags, w, h, D, Dp, out, best, rkey);          \
+        return hipGetLastError();                                                             \
+    }
+    VXH_WTA_LR_CASE(1, 1, kOutU16) VXH_WTA_LR_CASE(1, 2, kOutU16)
+    VXH_WTA_LR_CASE(2, 1, kOutU16) VXH_WTA_LR_CASE(2, 2, kOutU16)
+    VXH_WTA_LR_CASE(4, 1, kOutU16) VXH_WTA_LR_CASE(4, 2, kOutU16)
+    VXH_WTA_LR_CASE(8, 1, kOutU16) VXH_WTA_LR_CASE(8, 2, kOutU16)
+    VXH_WTA_LR_CASE(2, 1, kOutU8) VXH_WTA_LR_CASE(2, 2, kOutU8)
+    VXH_WTA_LR_CASE(4, 1, kOutU8) VXH_WTA_LR_CASE(4, 2, kOutU8)
+    VXH_WTA_LR_CASE(8, 1, kOutU8) VXH_WTA_LR_CASE(8, 2, kOutU8)
+#undef VXH_WTA_LR_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_lr_apply(const int16_t *best, const uint32_t *rkey, int w, int h, int tol,
+                           float *out, hipStream_t s) {
+    if (w < 1 || h < 1) return hipErrorInvalidValue;
+    const size_t n = (size_t)w * h;
+    hipLaunchKernelGGL(lr_apply_keys_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       best, rkey, w, h, tol, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {

@@ -741,24 +741,20 @@ __device__ __forceinline__ void wta_load(WtaIn<MODE> &in, const uint8_t *__restr
     }
 }
 
-// Sums the loaded slices mod 2^16 and takes the pixel's winner.
+// Sums the loaded slices mod 2^16.
 //   kOutU16: u16 entries added as u16 pairs.
 //   kOutU8:  the byte sums of at most 8 volumes stay below 2^16, so even and
 //            odd bytes are summed as two u16 pairs without carries; slices
 //            flagged in the pixel's 8 flag bytes (byte k by direction k) add
 //            256 * their high bytes (hi0 + k * vstride, same entry offsets).
-// With sum_out the summed u16 slice is also stored (pixel-major, stride Dp).
-// All lanes of the wave must call it (the winner uses cross-lane moves).
+// v[c][q] receives entries 128c + 8j + 2q and + 1 (pairs in disparity order).
 template <int MODE>
-__device__ __forceinline__ void wta_finish(const WtaIn<MODE> &in,
-                                           const uint8_t *__restrict__ hi0, size_t vstride,
-                                           int nv, int nc, int pix, int npx, int D, int Dp,
-                                           float *__restrict__ out, int16_t *__restrict__ best,
-                                           uint16_t *__restrict__ sum_out) {
+__device__ __forceinline__ void wta_sums(const WtaIn<MODE> &in, const uint8_t *__restrict__ hi0,
+                                         size_t vstride, int nv, int nc, int pix, int npx,
+                                         int Dp, u16x2 (&v)[2][4]) {
     const int j = (int)(threadIdx.x & 15);
     const bool valid = pix < npx;
     const size_t base = (size_t)(valid ? pix : 0) * (size_t)Dp;
-    u16x2 v[2][4];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -818,6 +814,22 @@ __device__ __forceinline__ void wta_finish(const WtaIn<MODE> &in,
             v[c][3] = __builtin_bit_cast(u16x2, perm(o1, e1, 0x07060302u));
         }
     }
+}
+
+// wta_sums, then the pixel's winner; with sum_out the summed u16 slice is
+// also stored (pixel-major, stride Dp).  All lanes of the wave must call it
+// (the winner uses cross-lane moves).
+template <int MODE>
+__device__ __forceinline__ void wta_finish(const WtaIn<MODE> &in,
+                                           const uint8_t *__restrict__ hi0, size_t vstride,
+                                           int nv, int nc, int pix, int npx, int D, int Dp,
+                                           float *__restrict__ out, int16_t *__restrict__ best,
+                                           uint16_t *__restrict__ sum_out) {
+    const int j = (int)(threadIdx.x & 15);
+    const bool valid = pix < npx;
+    const size_t base = (size_t)(valid ? pix : 0) * (size_t)Dp;
+    u16x2 v[2][4];
+    wta_sums(in, hi0, vstride, nv, nc, pix, npx, Dp, v);
     if (sum_out != nullptr && valid) {
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
@@ -851,6 +863,78 @@ __device__ __forceinline__ void wta_sum_pixel(const uint8_t *__restrict__ vol0, 
         WtaIn<kOutU8> in;
         wta_load(in, vol0, vstride, flags, nv, nc, pix, npx, Dp);
         wta_finish(in, hi0, vstride, nv, nc, pix, npx, D, Dp, out, best, sum_out);
+    }
+}
+
+// Left-right check, right view (the check as the build defines it; the
+// reference has no right view): D_R(xr) is the lowest k < D with xr + k < w
+// minimising S(xr + k, y, k), taken strictly below 0xFFFF, and 0 when no
+// entry is below it.  Entry (x, d) of a summed slice is entry k = d of right
+// pixel x - d, so every pixel offers each of its entries with S < 0xFFFF and
+// d <= x as the inverted key ~(S << 16 | d) to right pixel x - d; the largest
+// inverted key is the smallest S and, among equal S, the smallest d.  Keys
+// start at 0 (no offer).
+__device__ __forceinline__ uint32_t lr_right_of_key(uint32_t key) {
+    return key ? (~key) & 0xFFFFu : 0u;
+}
+
+// The offers of one pixel x (its slice in v, 16 lanes, as wta_slice) into
+// lkey[x - d - xb] (LDS words; atomic max).
+__device__ __forceinline__ void wta_offer_right(const u16x2 (&v)[2][4], int nc, int j, int D,
+                                                int x, int xb, uint32_t *lkey) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        if (c >= nc) break;
+        const int d0 = 128 * c + 8 * j;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int d = d0 + e;
+            const uint32_t val = v[c][e / 2][e & 1];
+            if (d < D && d <= x && val != 0xFFFFu)
+                atomicMax(&lkey[x - d - xb], ~((val << 16) | (uint32_t)d));
+        }
+    }
+}
+
+// Sum + winner of kWtaLrP pixels of row y from x0, 16 at a time (the next 16
+// pixels' loads in flight while the current ones are summed), with the
+// right view's offers gathered in lkey (kWtaLrKeys LDS words: right pixels
+// x0 - D + 1 .. x0 + kWtaLrP - 1) and folded into the row's keys
+// rkey[y * w + xr] with one atomic max per right pixel offered to.
+// Block-level: every thread of the 256-thread block must call it.
+constexpr int kWtaLrP = 128;
+constexpr int kWtaLrKeys = kWtaLrP + 255;
+template <int MODE>
+__device__ __forceinline__ void wta_lr_tile(const uint8_t *__restrict__ vol0, size_t vstride,
+                                            const uint8_t *__restrict__ hi0,
+                                            const uint2 *__restrict__ flags, int nv, int nc,
+                                            int w, int h, int y, int x0, int D, int Dp,
+                                            float *__restrict__ out, int16_t *__restrict__ best,
+                                            uint32_t *__restrict__ rkey, uint32_t *lkey) {
+    const int t = (int)threadIdx.x, j = t & 15;
+    const int xb = x0 - (D - 1);
+    const int nk = kWtaLrP + D - 1;
+    const int npx = w * h;
+    for (int i = t; i < nk; i += 256) lkey[i] = 0;
+    __syncthreads();
+    auto pix_of = [&](int x) { return x < w ? y * w + x : npx; };
+    WtaIn<MODE> cur, nxt;
+    wta_load(cur, vol0, vstride, flags, nv, nc, pix_of(x0 + (t >> 4)), npx, Dp);
+    for (int it = 0; it < kWtaLrP / 16; ++it) {
+        const int x = x0 + 16 * it + (t >> 4);
+        if (it + 1 < kWtaLrP / 16) wta_load(nxt, vol0, vstride, flags, nv, nc, pix_of(x + 16), npx, Dp);
+        u16x2 v[2][4];
+        wta_sums(cur, hi0, vstride, nv, nc, pix_of(x), npx, Dp, v);
+        if (x < w) wta_offer_right(v, nc, j, D, x, xb, lkey);
+        wta_slice(v, nc, j, D, x < w, pix_of(x), out, best);
+        cur = nxt;
+    }
+    __syncthreads();
+    uint32_t *row = rkey + (size_t)y * w;
+    for (int i = t; i < nk; i += 256) {
+        const int xr = xb + i;
+        const uint32_t k = lkey[i];
+        if (k != 0 && xr >= 0 && xr < w) atomicMax(&row[xr], k);
     }
 }
 

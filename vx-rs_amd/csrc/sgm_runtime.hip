@@ -582,12 +582,13 @@ size_t vxh_sgm_workspace_bytes(uint32_t w, uint32_t h, const vxh_sgm_config *c) 
 // Aggregates every path of `c` at once, direction i into the i-th volume of
 // the workspace (`mode` storage; the u8 flags must already be clear), then
 // sums the volumes and takes the winner: into `out` (may be null), `best`
-// (null unless the LR check follows) and, with `sum_out`, the summed u16
-// volume.  With `mark`, records it on `st` between the two launches.
+// and `rkey` (the LR check's integer winners and right-view keys, both or
+// neither) or, with `sum_out`, the summed u16 volume.  With `mark`, records
+// it on `st` between the two launches.
 static vxh_status paths_and_wta(const uint8_t *cost, uint32_t w, uint32_t h,
                                 const vxh_sgm_config *c, char *ws, const WsLayout &L,
                                 int mode, vxh::Schedule sc, hipStream_t st, float *out,
-                                int16_t *best, uint16_t *sum_out,
+                                int16_t *best, uint16_t *sum_out, uint32_t *rkey,
                                 hipEvent_t mark = nullptr) {
     const int D = (int)c->num_disparities;
     const int P = (int)c->num_paths;
@@ -615,11 +616,17 @@ static vxh_status paths_and_wta(const uint8_t *cost, uint32_t w, uint32_t h,
     VXH_HIP(vxh::launch_paths(base, dirs, vols, his, P, sc, mode, (uint32_t *)(ws + L.sync), st),
             "aggregate");
     if (mark) VXH_HIP(hipEventRecord(mark, st), "hipEventRecord");
-    VXH_HIP(vxh::launch_wta_sum((const uint8_t *)(ws + L.vol), L.vol_bytes,
-                                mode == vxh::kOutU8 ? (const uint8_t *)(ws + L.hi) : nullptr,
-                                (const uint8_t *)base.flags, P, mode, (int)(w * h), D, base.Dp,
-                                out, best, sum_out, st),
-            "wta");
+    const uint8_t *hi0 = mode == vxh::kOutU8 ? (const uint8_t *)(ws + L.hi) : nullptr;
+    if (rkey)
+        VXH_HIP(vxh::launch_wta_lr((const uint8_t *)(ws + L.vol), L.vol_bytes, hi0,
+                                   (const uint8_t *)base.flags, P, mode, (int)w, (int)h, D,
+                                   base.Dp, out, best, rkey, st),
+                "wta");
+    else
+        VXH_HIP(vxh::launch_wta_sum((const uint8_t *)(ws + L.vol), L.vol_bytes, hi0,
+                                    (const uint8_t *)base.flags, P, mode, (int)(w * h), D,
+                                    base.Dp, out, best, sum_out, st),
+                "wta");
     return VXH_OK;
 }
 
@@ -707,14 +714,16 @@ vxh_status run_pipeline(const uint8_t *d_left, const uint8_t *d_right, uint32_t 
                         st),
             "cost");
     if (marks) VXH_HIP(hipEventRecord(marks[0], st), "hipEventRecord");
+    // LR check: the integer winners and the right view's keys, in the left
+    // census's area (free once the costs are computed).
     int16_t *best = lr ? (int16_t *)(base + L.best) : nullptr;
-    uint16_t *sum = lr ? (uint16_t *)(base + L.sum) : nullptr;
-    vxh_status v = paths_and_wta(cost, w, h, c, base, L, mode, sc, st, d_out, best, sum,
-                                 marks ? marks[1] : nullptr);
+    uint32_t *rkey = lr ? (uint32_t *)(base + L.lc) : nullptr;
+    if (lr) VXH_HIP(launch_zero(rkey, (size_t)w * h, st), "zero");
+    vxh_status v = paths_and_wta(cost, w, h, c, base, L, mode, sc, st, d_out, best, nullptr,
+                                 rkey, marks ? marks[1] : nullptr);
     if (v != VXH_OK) return v;
     if (lr)
-        VXH_HIP(launch_lr_check(sum, w, h, D, volume_stride(D), (int)c->lr_tolerance, best,
-                                d_out, (int16_t *)(base + L.lc), st),
+        VXH_HIP(launch_lr_apply(best, rkey, (int)w, (int)h, (int)c->lr_tolerance, d_out, st),
                 "lr_check");
     if (marks) VXH_HIP(hipEventRecord(marks[2], st), "hipEventRecord");
     return VXH_OK;
@@ -1092,7 +1101,7 @@ vxh_status vxh_sgm_aggregate_layout(const uint32_t *d_lc, const uint32_t *d_rc,
         e = vxh::launch_zero((uint32_t *)(g + L.sync), vxh::sync_words(w, h), st);
     vxh_status v2 = e == hipSuccess
                         ? paths_and_wta((const uint8_t *)(g + L.cost), w, h, c, g, L, mode, sc,
-                                        st, nullptr, nullptr, d_volume)
+                                        st, nullptr, nullptr, d_volume, nullptr)
                         : hip_fail(e, "launch");
     hipError_t e2 = hipFreeAsync(g, st);
     if (v2 != VXH_OK) return v2;

@@ -336,19 +336,22 @@ wta_sum_kernel(const uint8_t *__restrict__ vol0, size_t vstride,
 
 // Sum + winner with the right view's keys for the left-right check: block
 // (bx, y) takes kWtaLrP pixels of row y from bx * kWtaLrP (wta_lr_tile).
+// 16 pixels and 256 threads per block: at 32 or 64 (512 or 1024 threads) the
+// fewer global atomics bought nothing and 720p, D = 128 ran up to 20 us
+// slower.
+constexpr int kWtaLrP = 16;
 template <int NV, int NC, int MODE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(16 * kWtaLrP)
 wta_lr_kernel(const uint8_t *__restrict__ vol0, size_t vstride, const uint8_t *__restrict__ hi0,
               const uint2 *__restrict__ flags, int w, int h, int D, int Dp,
               float *__restrict__ out, int16_t *__restrict__ best, uint32_t *__restrict__ rkey) {
-    __shared__ uint32_t lkey[kWtaLrKeys];
-    wta_lr_tile<MODE>(vol0, vstride, hi0, flags, NV, NC, w, h, (int)blockIdx.y,
-                      (int)blockIdx.x * kWtaLrP, D, Dp, out, best, rkey, lkey);
+    __shared__ uint32_t lkey[kWtaLrP + 255];
+    wta_lr_tile<MODE, kWtaLrP>(vol0, vstride, hi0, flags, NV, NC, w, h, (int)blockIdx.y,
+                               (int)blockIdx.x * kWtaLrP, D, Dp, out, best, rkey, lkey);
 }
 
-// Sum + winner-take-all of many pairs in one launch: 16 pixels per block, or
-// for a pair with right-view keys (the LR check) kWtaLrP pixels of one row
-// per block as wta_lr_kernel.  Instantiated per kind (LR) so that the plain
+// Sum + winner-take-all of many pairs in one launch: 16 pixels per block, of
+// one row for a pair with right-view keys (the LR check, as wta_lr_kernel).  Instantiated per kind (LR) so that the plain
 // pairs keep the plain kernel's registers; each instantiation skips the
 // blocks of the other kind's pairs.
 template <bool LR>
@@ -360,17 +363,17 @@ wta_multi_kernel(const WtaJob *__restrict__ jobs, int njobs) {
     const int local = (int)(blockIdx.x - c.first_block);
     const int nc = c.Dp > 128 ? 2 : 1;
     if constexpr (LR) {
-        __shared__ uint32_t lkey[kWtaLrKeys];
+        __shared__ uint32_t lkey[kWtaLrP + 255];
         const int nbx = (c.w + kWtaLrP - 1) / kWtaLrP;
         const int h = c.npx / c.w;
         if (c.mode == kOutU16)
-            wta_lr_tile<kOutU16>(c.vol0, c.vstride, c.hi0, c.flags, c.nv, nc, c.w, h,
-                                 local / nbx, (local % nbx) * kWtaLrP, c.D, c.Dp, c.out, c.best,
-                                 c.rkey, lkey);
+            wta_lr_tile<kOutU16, kWtaLrP>(c.vol0, c.vstride, c.hi0, c.flags, c.nv, nc, c.w, h,
+                                          local / nbx, (local % nbx) * kWtaLrP, c.D, c.Dp, c.out,
+                                          c.best, c.rkey, lkey);
         else
-            wta_lr_tile<kOutU8>(c.vol0, c.vstride, c.hi0, c.flags, c.nv, nc, c.w, h,
-                                local / nbx, (local % nbx) * kWtaLrP, c.D, c.Dp, c.out, c.best,
-                                c.rkey, lkey);
+            wta_lr_tile<kOutU8, kWtaLrP>(c.vol0, c.vstride, c.hi0, c.flags, c.nv, nc, c.w, h,
+                                         local / nbx, (local % nbx) * kWtaLrP, c.D, c.Dp, c.out,
+                                         c.best, c.rkey, lkey);
     } else {
         const int pix = local * 16 + (int)(threadIdx.x >> 4);
         wta_sum_pixel(c.vol0, c.vstride, c.hi0, c.flags, c.nv, nc, c.mode, pix, c.npx, c.D,
@@ -679,7 +682,7 @@ hipError_t launch_wta_sum(const uint8_t *vol0, size_t vstride, const uint8_t *hi
 }
 
 uint32_t wta_job_blocks(const WtaJob &j) {
-    if (j.rkey) return (uint32_t)(((j.w + dev::kWtaLrP - 1) / dev::kWtaLrP) * (j.npx / j.w));
+    if (j.rkey) return (uint32_t)(((j.w + kWtaLrP - 1) / kWtaLrP) * (j.npx / j.w));
     return (uint32_t)((j.npx + 15) / 16);
 }
 
@@ -691,8 +694,9 @@ hipError_t launch_wta_lr(const uint8_t *vol0, size_t vstride, const uint8_t *hi0
     const dim3 grid((unsigned)((w + kWtaLrP - 1) / kWtaLrP), (unsigned)h);
 #define VXH_WTA_LR_CASE(NV, NC, MM)                                                          \
     if (mode == MM && nvol == NV && (Dp > 128) == (NC == 2)) {                                \
-        hipLaunchKernelGGL((wta_lr_kernel<NV, NC, MM>), grid, dim3(256), 0, s, vol0, vstride, \
-                           hi0, (const uint2 *)flags, w, h, D, Dp, out, best, rkey);          \
+        hipLaunchKernelGGL((wta_lr_kernel<NV, NC, MM>), grid, dim3(16 * kWtaLrP), 0, s,      \
+                           vol0, vstride, hi0, (const uint2 *)flags, w, h, D, Dp, out, best,  \
+                           rkey);                                                             \
         return hipGetLastError();                                                             \
     }
     VXH_WTA_LR_CASE(1, 1, kOutU16) VXH_WTA_LR_CASE(1, 2, kOutU16)

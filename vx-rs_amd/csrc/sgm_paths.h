@@ -896,42 +896,38 @@ __device__ __forceinline__ void wta_offer_right(const u16x2 (&v)[2][4], int nc, 
     }
 }
 
-// Sum + winner of kWtaLrP pixels of row y from x0, 16 at a time (the next 16
-// pixels' loads in flight while the current ones are summed), with the
-// right view's offers gathered in lkey (kWtaLrKeys LDS words: right pixels
-// x0 - D + 1 .. x0 + kWtaLrP - 1) and folded into the row's keys
-// rkey[y * w + xr] with one atomic max per right pixel offered to.
-// Block-level: every thread of the 256-thread block must call it.
-constexpr int kWtaLrP = 128;
-constexpr int kWtaLrKeys = kWtaLrP + 255;
-template <int MODE>
+// Sum + winner of the TP pixels of row y from x0 (16 lanes each, as
+// wta_sum_pixel; the block has 16 TP threads), with their right-view offers
+// gathered in lkey (TP + 255 LDS words: right pixels x0 - D + 1 .. x0 + TP - 1)
+// and folded into the row's keys rkey[y * w + xr] with one atomic max per
+// right pixel offered to, (TP + D - 1) / TP per pixel.  Taking a wider tile
+// 16 pixels at a time instead keeps fewer loads in flight per CU, and the
+// pass is bound by its loads.  Block-level: every thread must call it.
+template <int MODE, int TP>
 __device__ __forceinline__ void wta_lr_tile(const uint8_t *__restrict__ vol0, size_t vstride,
                                             const uint8_t *__restrict__ hi0,
                                             const uint2 *__restrict__ flags, int nv, int nc,
                                             int w, int h, int y, int x0, int D, int Dp,
                                             float *__restrict__ out, int16_t *__restrict__ best,
                                             uint32_t *__restrict__ rkey, uint32_t *lkey) {
+    constexpr int kThreads = 16 * TP;
     const int t = (int)threadIdx.x, j = t & 15;
     const int xb = x0 - (D - 1);
-    const int nk = kWtaLrP + D - 1;
+    const int nk = TP + D - 1;
     const int npx = w * h;
-    for (int i = t; i < nk; i += 256) lkey[i] = 0;
+    const int x = x0 + (t >> 4);
+    const int pix = x < w ? y * w + x : npx;
+    WtaIn<MODE> in;
+    wta_load(in, vol0, vstride, flags, nv, nc, pix, npx, Dp);
+    for (int i = t; i < nk; i += kThreads) lkey[i] = 0;
     __syncthreads();
-    auto pix_of = [&](int x) { return x < w ? y * w + x : npx; };
-    WtaIn<MODE> cur, nxt;
-    wta_load(cur, vol0, vstride, flags, nv, nc, pix_of(x0 + (t >> 4)), npx, Dp);
-    for (int it = 0; it < kWtaLrP / 16; ++it) {
-        const int x = x0 + 16 * it + (t >> 4);
-        if (it + 1 < kWtaLrP / 16) wta_load(nxt, vol0, vstride, flags, nv, nc, pix_of(x + 16), npx, Dp);
-        u16x2 v[2][4];
-        wta_sums(cur, hi0, vstride, nv, nc, pix_of(x), npx, Dp, v);
-        if (x < w) wta_offer_right(v, nc, j, D, x, xb, lkey);
-        wta_slice(v, nc, j, D, x < w, pix_of(x), out, best);
-        cur = nxt;
-    }
+    u16x2 v[2][4];
+    wta_sums(in, hi0, vstride, nv, nc, pix, npx, Dp, v);
+    if (x < w) wta_offer_right(v, nc, j, D, x, xb, lkey);
+    wta_slice(v, nc, j, D, x < w, pix, out, best);
     __syncthreads();
     uint32_t *row = rkey + (size_t)y * w;
-    for (int i = t; i < nk; i += 256) {
+    for (int i = t; i < nk; i += kThreads) {
         const int xr = xb + i;
         const uint32_t k = lkey[i];
         if (k != 0 && xr >= 0 && xr < w) atomicMax(&row[xr], k);

@@ -1,7 +1,7 @@
 """SGM stereo benchmark on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--forms plain,batched,stream,served,next,config2]
+                    [--forms plain,batched,stream,served,next,config2,lr]
 
 Headline (`value`): the plain per-pair path.  A step is one pass of the SGM
 pipeline (census of both images, matching costs, path aggregation over every
@@ -40,6 +40,7 @@ bilateral filters, rectify maps and remap, sparse ORB stereo) and the
 SGM-to-depth pipeline relaunch, each timed with its bytes and rate.
 `config2`: the plain path on 1920x1080, D=256, 8 paths, checked against the
 oracle's hash.
+`lr`: the headline pair with the LR check, checked against the oracle's hash.
 """
 import argparse
 import ctypes
@@ -64,7 +65,7 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle threads of the all-cores CPU baseline (0 = every CPU this "
                          "process may run on)")
-    ap.add_argument("--forms", default="plain,batched,stream,served,next,config2")
+    ap.add_argument("--forms", default="plain,batched,stream,served,next,config2,lr")
     ap.add_argument("--next-reps", type=int, default=50,
                     help="launches timed per kernel of the next-rows leg")
     ap.add_argument("--stream-pairs", type=int, default=8,
@@ -287,9 +288,20 @@ def config2_leg(vx, hip, np, args, tuner):
     cost volume, 531 MB, exceeds the 256 MB Infinity Cache), timed like the
     headline over min(--steps, 20) steps with the stage marks, its map compared
     with the committed SHA-256 of the oracle's map for the same pair."""
-    wl = "1080p_d256_8path"
+    return hashed_pair_leg(vx, hip, np, args, tuner, "1080p_d256_8path", False)
+
+
+def lr_leg(vx, hip, np, args, tuner):
+    """The headline pair (1280x720, D=128, 8 paths) with the LR check
+    (tolerance 1), timed and checked like config2_leg; the winner stage then
+    includes the right view's winners and the check."""
+    return hashed_pair_leg(vx, hip, np, args, tuner, "720p_d128_8path", True)
+
+
+def hashed_pair_leg(vx, hip, np, args, tuner, wl, lr):
     w, h, D, paths, _, _, _ = vx.WORKLOADS[wl]
     left, right, _, cfg = vx.workload_pair(wl)
+    cfg.lr_check = lr
     ctx = vx.Context(0)
     sgm = vx.SGMStereo(ctx)
     if tuner is not None:
@@ -320,13 +332,13 @@ def config2_leg(vx, hip, np, args, tuner):
     path = os.path.join(ROOT, "tests", "golden", "workload_hashes.json")
     try:
         with open(path) as f:
-            want = json.load(f)[wl]
+            want = json.load(f)[wl + ("_lr" if lr else "")]
     except (OSError, KeyError):
         want = None
     ms = float(np.mean(stage["total"]))
     agg_ms = float(np.mean(stage["aggregate_all_paths"]))
     alg = w * h * (2 * D + 2)
-    out = {"workload": wl, "steps": steps, "ms_per_pair": round(ms, 4),
+    out = {"workload": wl, "lr_check": lr, "steps": steps, "ms_per_pair": round(ms, 4),
            "value": round(w * h * D / (ms * 1e-3) / 1e9, 3), "unit": "Gpx*disp/s",
            "stage_ms": {k: round(float(np.mean(v)), 4) for k, v in stage.items() if k != "total"},
            "schedule": {"horizontal": list(sgm.schedule(D)[0]), "other": list(sgm.schedule(D)[1])},
@@ -741,6 +753,9 @@ def main():
     if "config2" in forms:
         log("1920x1080 D=256 leg")
         result["config2"] = config2_leg(vx, hip, np, args, tuner)
+    if "lr" in forms:
+        log("1280x720 D=128 leg with the LR check")
+        result["lr"] = lr_leg(vx, hip, np, args, tuner)
     print(json.dumps(result), flush=True)
 
 

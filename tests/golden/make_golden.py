@@ -5,8 +5,9 @@
 (no argument: all three).
 
 workload_hashes.json: SHA-256 of the oracle's fp32 disparity map (raw little-
-endian bits) for each named workload, so a GPU run at full size is checked bit
-for bit without re-running the oracle.  small_cases.npz: inputs and expected
+endian bits) for each named workload, and for the two 8-path ones also with
+the LR check (name + "_lr"), so a GPU run at full size is checked bit for bit
+without re-running the oracle.  small_cases.npz: inputs and expected
 volumes/maps of tiny pairs for 2, 4 and 8 paths.  mixture_hashes.json: the
 same SHA-256 for every pair of the batched workload (the 64-pair mixture of
 sizes {320x240, 640x480, 1280x720, 1920x1080} x D {64, 128, 256}, 8 paths)
@@ -51,6 +52,13 @@ def workloads():
                      nthreads=os.cpu_count())
         hashes[name] = map_sha256(disp)
         print(name, hashes[name], flush=True)
+    # The two 8-path workloads with the LR check (tolerance 1): name + "_lr".
+    for name in ("720p_d128_8path", "1080p_d256_8path"):
+        left, right, _, cfg = vxsgm.workload_pair(name)
+        disp = O.sgm(left, right, cfg.num_disparities, num_paths=cfg.num_paths, lr_check=True,
+                     lr_tol=1, nthreads=os.cpu_count())
+        hashes[name + "_lr"] = map_sha256(disp)
+        print(name + "_lr", hashes[name + "_lr"], flush=True)
     with open(os.path.join(HERE, "workload_hashes.json"), "w") as f:
         json.dump(hashes, f, indent=1, sort_keys=True)
 

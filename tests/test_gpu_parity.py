@@ -521,6 +521,38 @@ def test_1080p_d256_8path_matches_golden_hash(ctx, sgm):
         s.close()
 
 
+@pytest.mark.parametrize("workload", ["720p_d128_8path", "1080p_d256_8path"])
+def test_lr_check_full_size_matches_golden_hash(ctx, workload):
+    """The two 8-path workloads with the LR check (tolerance 1): the map's
+    SHA-256 equals the oracle's (tests/golden/workload_hashes.json, name +
+    "_lr"), in both volume formats, through the plain call and through the
+    batch's persistent form (multi-pair winner kernel)."""
+    with open(os.path.join(GOLDEN, "workload_hashes.json")) as f:
+        want = json.load(f)[workload + "_lr"]
+    left, right, _, cfg = vxsgm.workload_pair(workload)
+    cfg.lr_check = True
+    h, w = left.shape
+    D = cfg.num_disparities
+    s = vxsgm.SGMStereo(ctx)
+    try:
+        for fmt in ("auto", "u16"):
+            s.set_volume_format(D, fmt)
+            got = s.compute_host(left, right, w, h, cfg)
+            assert hashlib.sha256(bits(got).tobytes()).hexdigest() == want, fmt
+    finally:
+        s.close()
+    dl, dr = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
+    do = hip.DeviceBuffer(w * h * 4)
+    b = vxsgm.SGMBatch([(dl.ptr, dr.ptr, do.ptr, w, h, cfg, None, "auto", False)],
+                       options={"form": "persistent"})
+    try:
+        b.run()
+        got = do.download(np.float32)
+        assert hashlib.sha256(bits(got).tobytes()).hexdigest() == want
+    finally:
+        b.close()
+
+
 def test_enqueue_reports_expired_split_wait(ctx):
     """A device-resident caller learns of an expired split-scanline wait: with
     the bounded wait forced to zero polls from the host (every head gives up at

@@ -71,7 +71,7 @@ def parse():
     ap.add_argument("--stream-pairs", type=int, default=8,
                     help="720p pairs per replay of the stream form")
     ap.add_argument("--batch-pairs", type=int, default=64)
-    ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--batch-reps", type=int, default=7)
     ap.add_argument("--batch-layouts", default="default", choices=["default", "widest"],
                     help="per-pair lane layouts of the batched leg with --batch-options")
     ap.add_argument("--rates", default="200,500,1000,1300,1600,2000",
@@ -171,13 +171,21 @@ def batched_leg(vx, hip, np, args, sgm, tuner):
     batch.launch(stream.handle)
     stream.synchronize()
     e0, e1 = hip.Event(), hip.Event()
-    e0.record(stream)
-    for _ in range(args.batch_reps):
-        batch.launch(stream.handle)
-    e1.record(stream)
-    e1.synchronize()
-    ms_batch = e0.elapsed_ms(e1) / args.batch_reps
-    log(f"batched: {ms_batch:.3f} ms per batch")
+
+    def median_ms(run, reps):
+        # Each repetition timed on its own (HIP events), the median kept: one
+        # slow replay does not move the figure, as in the tuner's timings.
+        ts = []
+        for _ in range(reps):
+            e0.record(stream)
+            run()
+            e1.record(stream)
+            e1.synchronize()
+            ts.append(e0.elapsed_ms(e1))
+        return float(np.median(ts)), [round(t, 3) for t in ts]
+
+    ms_batch, batch_ms_all = median_ms(lambda: batch.launch(stream.handle), args.batch_reps)
+    log(f"batched: {ms_batch:.3f} ms per batch (median of {batch_ms_all})")
     # The plain loop runs each pair with the handle's schedule for its D.
     wsb = max(vx.workspace_bytes(w, h, cfg) for w, h, _, cfg in host)
     ws = hip.DeviceBuffer(wsb)
@@ -188,14 +196,8 @@ def batched_leg(vx, hip, np, args, sgm, tuner):
 
     plain_loop()
     stream.synchronize()
-    reps = max(1, args.batch_reps // 2)
-    e0.record(stream)
-    for _ in range(reps):
-        plain_loop()
-    e1.record(stream)
-    e1.synchronize()
-    ms_plain = e0.elapsed_ms(e1) / reps
-    log(f"batched: plain loop {ms_plain:.3f} ms")
+    ms_plain, plain_ms_all = median_ms(plain_loop, args.batch_reps)
+    log(f"batched: plain loop {ms_plain:.3f} ms (median of {plain_ms_all})")
     exact = all(np.array_equal(ob.download(np.uint32), op.download(np.uint32))
                 for _, _, ob, op in bufs)
     want = golden_hashes("batch_64") if args.batch_pairs == 64 else None
@@ -207,6 +209,7 @@ def batched_leg(vx, hip, np, args, sgm, tuner):
     return {
         "pairs": len(host), "gpxd_per_batch": round(pxd / 1e9, 4),
         "ms_batch": round(ms_batch, 3), "ms_plain_loop": round(ms_plain, 3),
+        "ms_batch_each": batch_ms_all, "ms_plain_loop_each": plain_ms_all,
         "value_batch": round(pxd / (ms_batch * 1e-3) / 1e9, 3),
         "value_plain_loop": round(pxd / (ms_plain * 1e-3) / 1e9, 3),
         "unit": "Gpx*disp/s",

@@ -337,7 +337,7 @@ hipError_t plan_persistent(vxh_sgm_batch *b, const std::vector<uint32_t> &order,
         off[ci].cost = stage(img, kj.data(), kj.size() * sizeof(CostJob));
         for (G &g : groups) {
             int kinds = 0;
-            for (const WtaJob &t : g.wta) kinds |= t.rkey ? 2 : 1;
+            for (const WtaJob &t : g.wta) kinds |= wta_job_kind(t);
             Chunk::Group cg{g.cls, g.mode, {}, 0, nullptr, (int)g.wta.size(), g.wb, kinds};
             std::vector<BatchItem> all;
             uint32_t most = 0;
@@ -413,8 +413,9 @@ hipError_t capture_persistent(vxh_sgm_batch *b) {
         if (e == hipSuccess && c.groups.size() == 1) {
             const Chunk::Group &g = c.groups[0];
             e = launch_class(g.cls, g.mode, g.args, g.blocks, b->stream);
-            if (e == hipSuccess) e = launch_wta_multi(g.wta, g.nwta, g.wta_blocks, g.wta_kinds, b->stream);
-            b->launches += 1 + (g.wta_kinds == 3 ? 2 : 1);
+            if (e == hipSuccess) e = launch_wta_multi(g.wta, g.nwta, g.wta_blocks, g.wta_kinds, g.mode,
+                                                      b->stream);
+            b->launches += 1 + __builtin_popcount((unsigned)g.wta_kinds);
         } else if (e == hipSuccess) {
             // Groups side by side: fork onto the sub streams, join back.
             e = hipEventRecord(b->events[0], b->stream);
@@ -423,10 +424,11 @@ hipError_t capture_persistent(vxh_sgm_batch *b) {
                 e = hipStreamWaitEvent(s, b->events[0], 0);
                 const Chunk::Group &cg = c.groups[g];
                 if (e == hipSuccess) e = launch_class(cg.cls, cg.mode, cg.args, cg.blocks, s);
-                if (e == hipSuccess) e = launch_wta_multi(cg.wta, cg.nwta, cg.wta_blocks, cg.wta_kinds, s);
+                if (e == hipSuccess) e = launch_wta_multi(cg.wta, cg.nwta, cg.wta_blocks, cg.wta_kinds,
+                                                          cg.mode, s);
                 if (e == hipSuccess) e = hipEventRecord(b->events[1 + g], s);
                 if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->events[1 + g], 0);
-                b->launches += 1 + (cg.wta_kinds == 3 ? 2 : 1);
+                b->launches += 1 + __builtin_popcount((unsigned)cg.wta_kinds);
             }
         }
         for (uint32_t i : c.pairs) {

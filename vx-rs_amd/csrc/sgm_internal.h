@@ -192,14 +192,17 @@ struct WtaJob {      // one pair, as launch_wta_sum (16 pixels per block) or, wi
     int nv, mode, npx, w, D, Dp;
     uint32_t first_block;
 };
-// Blocks of a winner job.
+// Blocks of a winner job, and its kind for launch_wta_multi.
 uint32_t wta_job_blocks(const WtaJob &j);
+constexpr int kWtaOne = 1, kWtaTwo = 2, kWtaLr = 4;
+int wta_job_kind(const WtaJob &j);
 hipError_t launch_census_multi(const CensusJob *d_jobs, int njobs, uint32_t blocks,
                                hipStream_t s);
 hipError_t launch_cost_multi(const CostJob *d_jobs, int njobs, uint32_t blocks, hipStream_t s);
-// kinds: 1 = some jobs without right-view keys, 2 = some with, 3 = both.
+// kinds: the bitwise or of wta_job_kind over the jobs (one launch per kind);
+// every job without right-view keys must have volume format `mode`.
 hipError_t launch_wta_multi(const WtaJob *d_jobs, int njobs, uint32_t blocks, int kinds,
-                            hipStream_t s);
+                            int mode, hipStream_t s);
 
 // Persistent aggregation over the direction passes of many pairs.  A pass is
 // one direction of one pair (p carries its volume, direction and scanline

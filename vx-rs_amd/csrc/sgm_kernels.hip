@@ -363,10 +363,14 @@ wta_lr_kernel(const uint8_t *__restrict__ vol0, size_t vstride, const uint8_t *_
 }
 
 // Sum + winner-take-all of many pairs in one launch: 16 pixels per block, of
-// one row for a pair with right-view keys (the LR check, as wta_lr_kernel).  Instantiated per kind (LR) so that the plain
-// pairs keep the plain kernel's registers; each instantiation skips the
-// blocks of the other kind's pairs.
-template <bool LR>
+// one row for a pair with right-view keys (the LR check, as wta_lr_kernel),
+// and for the other pairs with two chunks (D > 128; TWO) 32 pixels per block,
+// two per group as wta_sum_kernel takes them.  Instantiated per kind and, for
+// the plain kinds, per volume format (a launch's pairs share one: the batch
+// groups them by format), so that each keeps its own registers; each
+// instantiation skips the blocks of the other kinds' pairs (wta_job_blocks
+// gives every pair's block count).
+template <bool LR, bool TWO, int MODE>
 __global__ void __launch_bounds__(256)
 wta_multi_kernel(const WtaJob *__restrict__ jobs, int njobs) {
     const int j = find_entry(jobs, njobs, blockIdx.x);
@@ -387,9 +391,16 @@ wta_multi_kernel(const WtaJob *__restrict__ jobs, int njobs) {
                                          local / nbx, (local % nbx) * kWtaLrP, c.D, c.Dp, c.out,
                                          c.best, c.rkey, lkey);
     } else {
-        const int pix = local * 16 + (int)(threadIdx.x >> 4);
-        wta_sum_pixel(c.vol0, c.vstride, c.hi0, c.flags, c.nv, nc, c.mode, pix, c.npx, c.D,
-                      c.Dp, c.out, c.best, nullptr);
+        // Not this instantiation's pair: another launch of launch_wta_multi takes it.
+        if (nc != (TWO ? 2 : 1) || c.mode != MODE) return;
+        const int p0 = local * (TWO ? 32 : 16) + (int)(threadIdx.x >> 4);
+        WtaIn<MODE> a, b;
+        wta_load(a, c.vol0, c.vstride, c.flags, c.nv, nc, p0, c.npx, c.Dp);
+        if constexpr (TWO) wta_load(b, c.vol0, c.vstride, c.flags, c.nv, nc, p0 + 16, c.npx, c.Dp);
+        wta_finish(a, c.hi0, c.vstride, c.nv, nc, p0, c.npx, c.D, c.Dp, c.out, c.best, nullptr);
+        if constexpr (TWO)
+            wta_finish(b, c.hi0, c.vstride, c.nv, nc, p0 + 16, c.npx, c.D, c.Dp, c.out, c.best,
+                       nullptr);
     }
 }
 
@@ -579,12 +590,21 @@ hipError_t launch_cost_multi(const CostJob *d_jobs, int njobs, uint32_t blocks, 
 }
 
 hipError_t launch_wta_multi(const WtaJob *d_jobs, int njobs, uint32_t blocks, int kinds,
-                            hipStream_t s) {
-    if (njobs < 1 || blocks < 1 || kinds < 1 || kinds > 3) return hipErrorInvalidValue;
-    if (kinds & 1)
-        hipLaunchKernelGGL(wta_multi_kernel<false>, dim3(blocks), dim3(256), 0, s, d_jobs, njobs);
-    if (kinds & 2)
-        hipLaunchKernelGGL(wta_multi_kernel<true>, dim3(blocks), dim3(256), 0, s, d_jobs, njobs);
+                            int mode, hipStream_t s) {
+    if (njobs < 1 || blocks < 1 || kinds < 1 || kinds > 7 ||
+        (mode != kOutU8 && mode != kOutU16))
+        return hipErrorInvalidValue;
+    const dim3 g(blocks), b(256);
+    if ((kinds & kWtaOne) && mode == kOutU8)
+        hipLaunchKernelGGL((wta_multi_kernel<false, false, kOutU8>), g, b, 0, s, d_jobs, njobs);
+    if ((kinds & kWtaOne) && mode == kOutU16)
+        hipLaunchKernelGGL((wta_multi_kernel<false, false, kOutU16>), g, b, 0, s, d_jobs, njobs);
+    if ((kinds & kWtaTwo) && mode == kOutU8)
+        hipLaunchKernelGGL((wta_multi_kernel<false, true, kOutU8>), g, b, 0, s, d_jobs, njobs);
+    if ((kinds & kWtaTwo) && mode == kOutU16)
+        hipLaunchKernelGGL((wta_multi_kernel<false, true, kOutU16>), g, b, 0, s, d_jobs, njobs);
+    if (kinds & kWtaLr)
+        hipLaunchKernelGGL((wta_multi_kernel<true, false, kOutU8>), g, b, 0, s, d_jobs, njobs);
     return hipGetLastError();
 }
 
@@ -698,7 +718,12 @@ hipError_t launch_wta_sum(const uint8_t *vol0, size_t vstride, const uint8_t *hi
 
 uint32_t wta_job_blocks(const WtaJob &j) {
     if (j.rkey) return (uint32_t)(((j.w + kWtaLrP - 1) / kWtaLrP) * (j.npx / j.w));
+    if (j.Dp > 128) return (uint32_t)((j.npx + 31) / 32);
     return (uint32_t)((j.npx + 15) / 16);
+}
+
+int wta_job_kind(const WtaJob &j) {
+    return j.rkey ? kWtaLr : (j.Dp > 128 ? kWtaTwo : kWtaOne);
 }
 
 hipError_t launch_wta_lr(const uint8_t *vol0, size_t vstride, const uint8_t *hi0,

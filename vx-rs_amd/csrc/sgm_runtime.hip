@@ -583,7 +583,8 @@ size_t vxh_sgm_workspace_bytes(uint32_t w, uint32_t h, const vxh_sgm_config *c) 
 // the workspace (`mode` storage; the u8 flags must already be clear), then
 // sums the volumes and takes the winner: into `out` (may be null), `best`
 // and `rkey` (the LR check's integer winners and right-view keys, both or
-// neither) or, with `sum_out`, the summed u16 volume.  With `mark`, records
+// neither; the keys are cleared here) or, with `sum_out`, the summed u16
+// volume.  With `mark`, records
 // it on `st` between the two launches.
 static vxh_status paths_and_wta(const uint8_t *cost, uint32_t w, uint32_t h,
                                 const vxh_sgm_config *c, char *ws, const WsLayout &L,
@@ -617,16 +618,18 @@ static vxh_status paths_and_wta(const uint8_t *cost, uint32_t w, uint32_t h,
             "aggregate");
     if (mark) VXH_HIP(hipEventRecord(mark, st), "hipEventRecord");
     const uint8_t *hi0 = mode == vxh::kOutU8 ? (const uint8_t *)(ws + L.hi) : nullptr;
-    if (rkey)
+    if (rkey) {
+        VXH_HIP(vxh::launch_zero(rkey, (size_t)w * h, st), "zero");
         VXH_HIP(vxh::launch_wta_lr((const uint8_t *)(ws + L.vol), L.vol_bytes, hi0,
                                    (const uint8_t *)base.flags, P, mode, (int)w, (int)h, D,
                                    base.Dp, out, best, rkey, st),
                 "wta");
-    else
+    } else {
         VXH_HIP(vxh::launch_wta_sum((const uint8_t *)(ws + L.vol), L.vol_bytes, hi0,
                                     (const uint8_t *)base.flags, P, mode, (int)(w * h), D,
                                     base.Dp, out, best, sum_out, st),
                 "wta");
+    }
     return VXH_OK;
 }
 
@@ -718,7 +721,6 @@ vxh_status run_pipeline(const uint8_t *d_left, const uint8_t *d_right, uint32_t 
     // census's area (free once the costs are computed).
     int16_t *best = lr ? (int16_t *)(base + L.best) : nullptr;
     uint32_t *rkey = lr ? (uint32_t *)(base + L.lc) : nullptr;
-    if (lr) VXH_HIP(launch_zero(rkey, (size_t)w * h, st), "zero");
     vxh_status v = paths_and_wta(cost, w, h, c, base, L, mode, sc, st, d_out, best, nullptr,
                                  rkey, marks ? marks[1] : nullptr);
     if (v != VXH_OK) return v;

@@ -123,7 +123,7 @@ class Tuner:
             ms.append(e0.elapsed_ms(e1))
         return statistics.median(ms)
 
-    def tune_batch(self, descs, key="batch", reps=3, force=False):
+    def tune_batch(self, descs, key="batch", reps=5, force=False):
         """Best (options, layout policy) for the batch of pair descriptors
         `descs` (as for SGMBatch); apply the policy with apply_layouts."""
         if not force and key in self.table:

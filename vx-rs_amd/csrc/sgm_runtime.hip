@@ -584,8 +584,7 @@ size_t vxh_sgm_workspace_bytes(uint32_t w, uint32_t h, const vxh_sgm_config *c) 
 // sums the volumes and takes the winner: into `out` (may be null), `best`
 // and `rkey` (the LR check's integer winners and right-view keys, both or
 // neither; the keys are cleared here) or, with `sum_out`, the summed u16
-// volume.  With `mark`, records
-// it on `st` between the two launches.
+// volume.  With `mark`, records it on `st` between the two launches.
 static vxh_status paths_and_wta(const uint8_t *cost, uint32_t w, uint32_t h,
                                 const vxh_sgm_config *c, char *ws, const WsLayout &L,
                                 int mode, vxh::Schedule sc, hipStream_t st, float *out,

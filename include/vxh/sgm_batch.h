@@ -70,8 +70,9 @@ typedef struct vxh_sgm_batch vxh_sgm_batch;
  *   blocks_per_cu  persistent form: blocks of 256 threads of the aggregation
  *                  per CU (default: as many as fit).
  *   budget_bytes   persistent and sweep forms: workspace bytes of one chunk
- *                  (default 64 GiB; a pair larger than the budget forms its
- *                  own chunk). */
+ *                  (default: 60 % of the device memory free at creation, at
+ *                  least 1 GiB; a pair larger than the budget forms its own
+ *                  chunk). */
 typedef struct {
     uint32_t streams;
     uint32_t form;

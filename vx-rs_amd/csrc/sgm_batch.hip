@@ -20,9 +20,9 @@
 //
 // Streams.  The per-pair pipeline of every pair (the plain path's exact
 // launches, including split scanlines) is captured over several streams:
-// pairs are ordered by work and dealt round-robin to the streams, which run
-// side by side; pairs on one stream run in turn and share that stream's
-// workspace.
+// pairs are ordered by work and each dealt to the least loaded stream; the
+// streams run side by side, and pairs on one stream run in turn and share
+// that stream's workspace.
 //
 // Both forms run the plain path's kernels or their multi-pair wrappers around
 // the same per-tile, per-scanline and per-pixel device code, with each pair's
@@ -821,11 +821,16 @@ vxh_status vxh_sgm_batch_create_ex(int device, const vxh_sgm_pair_desc *pairs, u
     // two volume formats), the sweep form's classes.
     const uint32_t nsub = form == kFormStreams ? streams : 6;
     if (form == kFormStreams) {
-        // One workspace per stream, sized for the largest pair dealt to it.
+        // Pairs, largest first, each to the stream with the least work so far
+        // (W*H*D*paths); one workspace per stream, sized for the largest pair
+        // dealt to it.
         std::vector<size_t> ws_bytes(streams, 0);
+        std::vector<uint64_t> work(streams, 0);
         for (uint32_t k = 0; k < n; ++k) {
             PairPlan &p = b->plan[order[k]];
-            p.stream = k % streams;
+            const vxh_sgm_pair_desc &q = pairs[order[k]];
+            p.stream = (uint32_t)(std::min_element(work.begin(), work.end()) - work.begin());
+            work[p.stream] += (uint64_t)q.width * q.height * q.cfg.num_disparities * q.cfg.num_paths;
             ws_bytes[p.stream] = std::max(ws_bytes[p.stream], p.ws_bytes);
         }
         for (uint32_t s = 0; s < streams && e == hipSuccess; ++s) {

@@ -612,3 +612,55 @@ def test_random_configs_match_oracle(sgm, seed):
     got = sgm.compute_host(left, right, w, h, cfg)
     want = O.sgm(left, right, D, p1, p2, rx, ry, num_paths=paths, lr_check=lr, lr_tol=tol)
     assert_same_disp(got, want)
+
+
+def random_case(seed, max_w, max_h):
+    """A seeded random configuration and pair, as test_random_configs_match_oracle
+    draws them, up to max_w x max_h; returns (left, right, cfg, oracle map)."""
+    rng = np.random.default_rng(0x5347_2000 + seed)
+    w, h = int(rng.integers(1, max_w)), int(rng.integers(1, max_h))
+    D = int(rng.integers(1, min(w, 256) + 1))
+    paths = int(rng.choice([2, 4, 8]))
+    rx, ry = WINDOWS[int(rng.integers(len(WINDOWS)))]
+    p1, p2 = float(rng.integers(0, 260)), float(rng.integers(0, 1500))
+    lr, tol = bool(rng.integers(0, 2)), int(rng.integers(0, 4))
+    left, right = random_pair(w, h, seed=5000 + seed)
+    cfg = vxsgm.SGMStereoConfig(num_disparities=D, p1=p1, p2=p2, census_radius_x=rx,
+                                census_radius_y=ry, num_paths=paths, lr_check=lr,
+                                lr_tolerance=tol)
+    want = O.sgm(left, right, D, p1, p2, rx, ry, num_paths=paths, lr_check=lr, lr_tol=tol,
+                 nthreads=8)
+    return left, right, cfg, want
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_larger_configs_match_oracle(sgm, seed):
+    """Random configurations up to 720 x 320: scanlines of 256 pixels or more
+    in either axis and the diagonals, so the default schedules' split
+    scanlines (2 and 4 paths, and 8 paths at D <= 64) run, with and without
+    the LR check."""
+    left, right, cfg, want = random_case(seed, 720, 320)
+    h, w = left.shape
+    assert_same_disp(sgm.compute_host(left, right, w, h, cfg), want)
+
+
+@pytest.mark.parametrize("form", ["persistent", "streams"])
+def test_random_configs_in_batch_forms(form):
+    """Sixteen of those random configurations in one batch, in the persistent
+    form (multi-pair winner kernels of every kind: one or two chunks, u8 or
+    u16 volumes, LR checks) and the streams form."""
+    cases = [random_case(seed, 720, 320) for seed in range(24, 40)]
+    keep, descs = [], []
+    for left, right, cfg, _ in cases:
+        h, w = left.shape
+        dl, dr = hip.DeviceBuffer.from_array(left), hip.DeviceBuffer.from_array(right)
+        do = hip.DeviceBuffer(w * h * 4)
+        keep.append((dl, dr, do))
+        descs.append((dl.ptr, dr.ptr, do.ptr, w, h, cfg, None, "auto", False))
+    b = vxsgm.SGMBatch(descs, options={"form": form})
+    try:
+        b.run()
+        for (left, _, _, want), (_, _, do) in zip(cases, keep):
+            assert_same_disp(do.download(np.float32), want)
+    finally:
+        b.close()
